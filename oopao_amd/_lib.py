@@ -1,0 +1,59 @@
+"""ctypes binding of the oopao_amd C-ABI (include/oopao_amd.h).
+
+The shared library is built in-tree by ``python -m oopao_amd.build``. There is no
+CPU fallback: importing a GPU entry point without the library raises.
+"""
+import ctypes
+import os
+
+from .tools import OopaoError
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'liboopao_amd.so')
+
+_lib = None
+
+c_int = ctypes.c_int
+c_float = ctypes.c_float
+c_size_t = ctypes.c_size_t
+c_void_p = ctypes.c_void_p
+c_double = ctypes.c_double
+
+# name -> (restype, argtypes); the single source of the binding's signatures
+SIGNATURES = {
+    'oopao_last_error': (ctypes.c_char_p, []),
+    'oopao_pwfs_supported': (c_int, [c_int, c_int, c_int]),
+    'oopao_pwfs_workspace_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
+    'oopao_pwfs_frames': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_size_t,
+                                  c_void_p, c_void_p]),
+    'oopao_pwfs_slopes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_float,
+                                  c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+}
+
+
+def lib():
+    """The loaded library (raises OopaoError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OopaoError('oopao_amd HIP library not found at %s; build it with '
+                             '`python -m oopao_amd.build` (there is no CPU fallback)' % LIB_PATH)
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(handle, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc):
+    """Raise OopaoError with the library's message on a non-zero return code."""
+    if rc != 0:
+        msg = lib().oopao_last_error().decode(errors='replace')
+        raise OopaoError('oopao_amd error %d: %s' % (rc, msg))
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args))

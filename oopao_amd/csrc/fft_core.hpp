@@ -1,0 +1,260 @@
+// Register-resident Stockham FFT building blocks for CDNA4 (gfx950).
+//
+// One transform of length N is owned by TPT lanes of a single wavefront
+// (TPT <= 64), each lane holding E = N / TPT complex values in registers in the
+// "strided" layout: lane t holds element t + m*TPT in v[m], m = 0..E-1.
+// A Stockham stage of radix R does E/R butterflies per lane entirely from its
+// own registers (butterfly j = t + b*TPT reads elements j + r*N/R, which are
+// v[b + r*E/R]), writes the R outputs to their self-sorted positions in the
+// wavefront's LDS exchange buffer, and reads back the strided layout. Because a
+// transform never spans two wavefronts, the exchange needs only wavefront-level
+// ordering (LDS executes one wavefront's instructions in issue order), not a
+// workgroup barrier.
+//
+// Twiddles come from a per-length table tw[m] = exp(-2*pi*i*m/N) computed on
+// the host in double precision (accurate to float rounding).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace oopao {
+
+// Wavefront-scope ordering of LDS accesses between the write and read halves
+// of an exchange: a compiler code-motion barrier plus wavefront-scope fences.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -a.x * b.y));
+}
+__device__ __forceinline__ float cabs2(float2 a) { return fmaf(a.x, a.x, a.y * a.y); }
+
+// a * (i * DIR): DIR = -1 (forward) multiplies by -i, DIR = +1 by +i.
+template <int DIR>
+__device__ __forceinline__ float2 mul_i(float2 a) {
+  return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------------------
+// In-register DFTs of radix 2, 3, 4, 5, 8 (natural order in and out).
+// DIR = -1: X[k] = sum x[j] exp(-2 pi i jk/R); DIR = +1: exp(+...), unscaled.
+// ---------------------------------------------------------------------------
+template <int R, int DIR>
+struct Dft;
+
+template <int DIR>
+struct Dft<2, DIR> {
+  __device__ __forceinline__ static void run(float2* a) {
+    float2 t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  }
+};
+
+template <int DIR>
+struct Dft<4, DIR> {
+  __device__ __forceinline__ static void run(float2* a) {
+    float2 t0 = cadd(a[0], a[2]);
+    float2 t1 = csub(a[0], a[2]);
+    float2 t2 = cadd(a[1], a[3]);
+    float2 t3 = mul_i<DIR>(csub(a[1], a[3]));
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, t3);
+    a[3] = csub(t1, t3);
+  }
+};
+
+template <int DIR>
+struct Dft<3, DIR> {
+  __device__ __forceinline__ static void run(float2* a) {
+    const float s3 = 0.86602540378443864676f;  // sin(2 pi / 3)
+    float2 t1 = cadd(a[1], a[2]);
+    float2 m = make_float2(fmaf(-0.5f, t1.x, a[0].x), fmaf(-0.5f, t1.y, a[0].y));
+    float2 d = cscale(mul_i<DIR>(csub(a[1], a[2])), s3);
+    a[0] = cadd(a[0], t1);
+    a[1] = cadd(m, d);
+    a[2] = csub(m, d);
+  }
+};
+
+template <int DIR>
+struct Dft<5, DIR> {
+  __device__ __forceinline__ static void run(float2* a) {
+    const float c1 = 0.30901699437494742410f;   // cos(2 pi/5)
+    const float c2 = -0.80901699437494742410f;  // cos(4 pi/5)
+    const float s1 = 0.95105651629515357212f;   // sin(2 pi/5)
+    const float s2 = 0.58778525229247312917f;   // sin(4 pi/5)
+    float2 b1 = cadd(a[1], a[4]);
+    float2 b2 = cadd(a[2], a[3]);
+    float2 d1 = csub(a[1], a[4]);
+    float2 d2 = csub(a[2], a[3]);
+    float2 a0 = a[0];
+    float2 m1 = make_float2(fmaf(c2, b2.x, fmaf(c1, b1.x, a0.x)), fmaf(c2, b2.y, fmaf(c1, b1.y, a0.y)));
+    float2 m2 = make_float2(fmaf(c1, b2.x, fmaf(c2, b1.x, a0.x)), fmaf(c1, b2.y, fmaf(c2, b1.y, a0.y)));
+    float2 n1 = make_float2(fmaf(s2, d2.x, s1 * d1.x), fmaf(s2, d2.y, s1 * d1.y));
+    float2 n2 = make_float2(fmaf(-s1, d2.x, s2 * d1.x), fmaf(-s1, d2.y, s2 * d1.y));
+    float2 i1 = mul_i<DIR>(n1);
+    float2 i2 = mul_i<DIR>(n2);
+    a[0] = cadd(a0, cadd(b1, b2));
+    a[1] = cadd(m1, i1);
+    a[4] = csub(m1, i1);
+    a[2] = cadd(m2, i2);
+    a[3] = csub(m2, i2);
+  }
+};
+
+template <int DIR>
+struct Dft<8, DIR> {
+  __device__ __forceinline__ static void run(float2* a) {
+    const float r2 = 0.70710678118654752440f;
+    float2 e[4] = {a[0], a[2], a[4], a[6]};
+    float2 o[4] = {a[1], a[3], a[5], a[7]};
+    Dft<4, DIR>::run(e);
+    Dft<4, DIR>::run(o);
+    // o[k] *= exp(DIR * i * pi * k / 4)
+    float2 o1 = o[1], o3 = o[3];
+    if (DIR < 0) {
+      o[1] = make_float2((o1.x + o1.y) * r2, (o1.y - o1.x) * r2);
+      o[3] = make_float2((o3.y - o3.x) * r2, -(o3.x + o3.y) * r2);
+    } else {
+      o[1] = make_float2((o1.x - o1.y) * r2, (o1.x + o1.y) * r2);
+      o[3] = make_float2(-(o3.x + o3.y) * r2, (o3.x - o3.y) * r2);
+    }
+    o[2] = mul_i<DIR>(o[2]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a[k] = cadd(e[k], o[k]);
+      a[k + 4] = csub(e[k], o[k]);
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Compile-time radix plans. Every radix must divide E so that each lane owns
+// whole butterflies. Larger radices first: fewer LDS exchanges.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int pick_radix(int rem, int E) {
+  return (rem % 8 == 0 && E % 8 == 0) ? 8
+       : (rem % 4 == 0 && E % 4 == 0) ? 4
+       : (rem % 5 == 0 && E % 5 == 0) ? 5
+       : (rem % 3 == 0 && E % 3 == 0) ? 3
+       : (rem % 2 == 0 && E % 2 == 0) ? 2
+       : 0;
+}
+
+__host__ __device__ constexpr bool plan_ok(int rem, int E) {
+  return rem == 1 ? true : (pick_radix(rem, E) == 0 ? false : plan_ok(rem / pick_radix(rem, E), E));
+}
+
+__host__ __device__ constexpr int plan_stages(int rem, int E) {
+  return rem == 1 ? 0 : 1 + plan_stages(rem / pick_radix(rem, E), E);
+}
+
+// Exchange-buffer index padding: one float2 every 32 breaks the power-of-two
+// strides of the early Stockham stages across LDS banks.
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr int lds_len(int N) { return N + N / 32 + 1; }
+
+// Opaque copy of a read-only table pointer. Loads through the returned pointer
+// cannot be hoisted above this point, which keeps loop-invariant twiddle and
+// mask loads next to their use instead of pinning registers across loops.
+template <typename T>
+__device__ __forceinline__ const T* launder(const T* p) {
+  asm volatile("" : "+s"(p)::"memory");
+  return p;
+}
+
+// One Stockham stage (radix R, already-sorted span NS) on the strided layout.
+// When NS divides TPT the twiddle index k = j mod NS equals t mod NS for all of
+// the lane's butterflies, so R-1 twiddles are loaded once per stage.
+template <int N, int TPT, int R, int NS, int DIR>
+__device__ __forceinline__ void stockham_stage(float2* v, float2* lds, const float2* __restrict__ tw, int t) {
+  constexpr int E = N / TPT;
+  constexpr int NB = E / R;
+  constexpr bool kUniform = (TPT % NS) == 0;
+  constexpr int TS = N / (NS * R);
+  static_assert(E % R == 0, "radix must divide the per-lane element count");
+  const float2* twp = launder(tw);
+  float2 wu[R];
+  if (kUniform && NS > 1) {
+    const int k = t % NS;
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      float2 w = twp[r * k * TS];
+      if (DIR > 0) w.y = -w.y;
+      wu[r] = w;
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int j = t + b * TPT;
+    const int k = kUniform ? (t % NS) : (j % NS);
+    float2 x[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = v[b + r * NB];
+    if (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w;
+        if (kUniform) {
+          w = wu[r];
+        } else {
+          w = twp[r * k * TS];
+          if (DIR > 0) w.y = -w.y;
+        }
+        x[r] = cmul(x[r], w);
+      }
+    }
+    Dft<R, DIR>::run(x);
+    const int base = (j / NS) * (NS * R) + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[lds_pad(base + r * NS)] = x[r];
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = lds[lds_pad(t + m * TPT)];
+  wave_lds_sync();
+}
+
+// Remaining stages from span NS (REM = N / NS still to transform).
+template <int N, int TPT, int DIR, int NS, int REM>
+struct Stages {
+  static constexpr int E = N / TPT;
+  static constexpr int R = pick_radix(REM, E);
+  static_assert(R != 0, "length not supported by the radix set for this lane count");
+  __device__ __forceinline__ static void run(float2* v, float2* lds, const float2* __restrict__ tw, int t) {
+    stockham_stage<N, TPT, R, NS, DIR>(v, lds, tw, t);
+    Stages<N, TPT, DIR, NS * R, REM / R>::run(v, lds, tw, t);
+  }
+};
+template <int N, int TPT, int DIR, int NS>
+struct Stages<N, TPT, DIR, NS, 1> {
+  __device__ __forceinline__ static void run(float2*, float2*, const float2* __restrict__, int) {}
+};
+
+// Full transform of the strided-layout registers (natural order in and out).
+template <int N, int TPT, int DIR>
+__device__ __forceinline__ void fft_full(float2* v, float2* lds, const float2* __restrict__ tw, int t) {
+  Stages<N, TPT, DIR, 1, N>::run(v, lds, tw, t);
+}
+
+// Forward transform of an input that is non-zero only on [0, N/P): the first
+// radix-P stage of such an input replicates x[q/P] into position q, so the
+// caller loads v[m] = x[(t + m*TPT) / P] and the remaining stages start at
+// span P.
+template <int N, int TPT, int P>
+__device__ __forceinline__ void fft_fwd_pruned(float2* v, float2* lds, const float2* __restrict__ tw, int t) {
+  Stages<N, TPT, -1, P, N / P>::run(v, lds, tw, t);
+}
+
+}  // namespace oopao

@@ -1,0 +1,360 @@
+// Modulated Pyramid WFS forward model on CDNA4 (gfx950).
+//
+// Per modulation point i and phase screen b the reference computes
+//   E_i   = amp * exp(i (phi_b + TT_i))   on the n x n pupil, zero-padded to N x N
+//   F_i   = fft2(E_i * phasor) ; G_i = F_i * mask ; I_i = |ifft2(G_i)|^2
+//   frame = bin(sum_i I_i)                               (Pyramid.py:612-646, 712-757)
+// For a mask whose phase is additively separable, m[k,l] = a[k] + b[l] (the
+// default pyramid: m = -c(|x_l| + |y_k|), Pyramid.py:453-480), the 2-D operator
+// factorises: ifft2(fft2(S) * my mx^T) = Ky S Kx^T with Kx = IFFT . diag(mx) . FFT
+// along X and Ky likewise along Y. The tip/tilt modulation TT_i = px_i Tip +
+// py_i Tilt and the PSF-centring phasor are separable too. So the frame is
+// computed in two passes with only an n x N intermediate per point:
+//   rows pass : C_i[y,:] = IFFT_N( mx' . FFT_N(pad(u[y,:] tx_i ty_i[y])) )
+//   cols pass : E_i[:,X] = IFFT_N( my' . FFT_N(pad(C_i[:,X])) ), |E_i|^2 summed
+//               over a chunk of points and over 4 x 4 detector pixels
+// followed by a tiny deterministic reduction over point chunks. The zero
+// padding offset c0 and the 1/N of each inverse transform are folded into the
+// mask vectors mx', my' on the host; tx, ty carry exp(i px lin[x]) and the
+// phasor.
+//
+// Layout of the n x N intermediate (per point, per screen): tiled as
+// [N/4][n][4] complex64, so that one detector column (4 pupil-plane columns)
+// is one contiguous n*32-byte slab for the columns pass.
+#include "common.hpp"
+#include "fft_core.hpp"
+
+namespace oopao {
+
+// Lane count per transform for each supported padded length N (fast path: the
+// pupil occupies N/4 samples and the detector bins 4 x 4). Chosen so every
+// radix divides N/TPT and the pruned forward + inverse use few stages.
+__host__ __device__ constexpr int fast_tpt(int N) {
+  return N == 64 ? 16 : N == 128 ? 16 : N == 256 ? 32 : N == 320 ? 16 : N == 512 ? 32
+       : N == 640 ? 16 : N == 1024 ? 64 : N == 1280 ? 64 : N == 1536 ? 64 : N == 2048 ? 64 : 0;
+}
+
+#define OOPAO_FAST_SIZES(X) X(64) X(128) X(256) X(320) X(512) X(640) X(1024) X(1280) X(1536) X(2048)
+
+constexpr int kWaves = 4;  // wavefronts per workgroup
+
+// ---------------------------------------------------------------------------
+// Rows pass. One lane group (TPT lanes) per pupil row y; the row's complex
+// amplitude u = amp exp(i scale opd) is built once and reused for a chunk of
+// modulation points.
+// ---------------------------------------------------------------------------
+template <int N, int TPT>
+__global__ void __launch_bounds__(64 * kWaves)
+k_rows_sep(const float* __restrict__ opd, float scale, const float* __restrict__ amp,
+           const float2* __restrict__ tx, const float2* __restrict__ ty,
+           const float2* __restrict__ mx, const float2* __restrict__ tw,
+           float2* __restrict__ ws, int B, int T, int pts_per_task) {
+  constexpr int n = N / 4;
+  constexpr int E = N / TPT;
+  constexpr int G = 64 / TPT;
+  constexpr int RPB = kWaves * G;  // rows per block
+  __shared__ float2 lds[kWaves * G][lds_len(N)];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int h = lane / TPT;
+  const int t = lane % TPT;
+  constexpr int nrb = (n + RPB - 1) / RPB;
+  const int npc = (T + pts_per_task - 1) / pts_per_task;
+  int bid = blockIdx.x;
+  const int rb = bid % nrb;
+  bid /= nrb;
+  const int pc = bid % npc;
+  const int b = bid / npc;
+  const int y_raw = rb * RPB + wave * G + h;
+  const bool active = y_raw < n;
+  const int y = active ? y_raw : n - 1;
+  float2* xl = lds[wave * G + h];
+
+  float2 u[E];
+  const float* op = opd + ((size_t)b * n + y) * n;
+  const float* ap = amp + (size_t)y * n;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int x = (t + TPT * m) >> 2;
+    float s, c;
+    sincosf(op[x] * scale, &s, &c);
+    const float a = ap[x];
+    u[m] = make_float2(a * c, a * s);
+  }
+  const int i0 = pc * pts_per_task;
+  const int i1 = min(T, i0 + pts_per_task);
+  for (int i = i0; i < i1; ++i) {
+    const float2 tyv = ty[(size_t)i * n + y];
+    const float2* txi = launder(tx) + (size_t)i * n;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = cmul(cmul(u[m], txi[(t + TPT * m) >> 2]), tyv);
+    fft_fwd_pruned<N, TPT, 4>(v, xl, tw, t);
+    {
+      const float2* mxp = launder(mx);
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cmul(v[m], mxp[t + TPT * m]);
+    }
+    fft_full<N, TPT, +1>(v, xl, tw, t);
+    if (active) {
+      float2* dst = ws + ((size_t)b * T + i) * (size_t)(N / 4) * n * 4;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int X = t + TPT * m;
+        dst[((size_t)(X >> 2) * n + y) * 4 + (X & 3)] = v[m];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Columns pass. One wavefront per (screen b, point chunk, detector column g):
+// the 4 pupil-plane columns 4g..4g+3 are transformed for every point of the
+// chunk, |E|^2 accumulated in registers, then binned 4 rows x 4 columns.
+// Output: part[b][chunk][g][r] (detector column-major partial sums).
+// ---------------------------------------------------------------------------
+template <int N, int TPT>
+__global__ void __launch_bounds__(64 * kWaves)
+k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
+           const float2* __restrict__ tw, float* __restrict__ part, int B, int T, int nchunk) {
+  constexpr int n = N / 4;
+  constexpr int E = N / TPT;
+  constexpr int G = 64 / TPT;
+  constexpr int NG = N / 4;
+  __shared__ float2 lds[kWaves * G][lds_len(N)];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int h = lane / TPT;
+  const int t = lane % TPT;
+  int task = blockIdx.x * kWaves + wave;
+  const int g = task % NG;
+  task /= NG;
+  const int chunk = task % nchunk;
+  const int b = task / nchunk;
+  if (b >= B) return;  // whole wavefront uniform
+  const int i0 = (int)(((long)chunk * T) / nchunk);
+  const int i1 = (int)(((long)(chunk + 1) * T) / nchunk);
+  float2* xl = lds[wave * G + h];
+
+  float acc[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) acc[m] = 0.f;
+  for (int i = i0; i < i1; ++i) {
+    const float2* tile = ws + (((size_t)b * T + i) * NG + g) * (size_t)n * 4;
+#pragma unroll 1
+    for (int c0 = 0; c0 < 4; c0 += G) {
+      const int c = c0 + h;
+      float2 v[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + c];
+      fft_fwd_pruned<N, TPT, 4>(v, xl, tw, t);
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cmul(v[m], launder(my)[t + TPT * m]);
+      fft_full<N, TPT, +1>(v, xl, tw, t);
+#pragma unroll
+      for (int m = 0; m < E; ++m) acc[m] += cabs2(v[m]);
+    }
+  }
+  // rows t..t+3 (a lane quad) form one detector row; lane groups hold the
+  // other pupil columns of the same detector column.
+  float* dst = part + (((size_t)b * nchunk + chunk) * NG + g) * (size_t)NG;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    float s = acc[m];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+#pragma unroll
+    for (int off = TPT; off < 64; off <<= 1) s += __shfl_xor(s, off);
+    if (h == 0 && (t & 3) == 0) dst[(t + TPT * m) >> 2] = s;
+  }
+}
+
+// frame[b][r][g] = sum_k part[b][k][g][r]  (fixed summation order)
+__global__ void k_reduce_chunks(const float* __restrict__ part, float* __restrict__ frame, int B,
+                                int nchunk, int cam) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)B * cam * cam;
+  if (idx >= total) return;
+  const int r = (int)(idx % cam);
+  const int g = (int)((idx / cam) % cam);
+  const int b = (int)(idx / ((size_t)cam * cam));
+  float s = 0.f;
+  for (int k = 0; k < nchunk; ++k) s += part[(((size_t)b * nchunk + k) * cam + g) * cam + r];
+  frame[((size_t)b * cam + r) * cam + g] = s;
+}
+
+// ---------------------------------------------------------------------------
+// slopesMaps post-processing (Pyramid.py:838-854), one workgroup per frame.
+// quad[0..3] = row origins, quad[4..7] = column origins of quadrants 1..4.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_slopes(const float* __restrict__ frames, int cam, const unsigned char* __restrict__ valid, int w,
+         const int* __restrict__ quad, const float* __restrict__ ref2d, float units,
+         const int* __restrict__ vidx, int nsig, int nvalid, float* __restrict__ sig2d,
+         float* __restrict__ sig) {
+  const int b = blockIdx.x;
+  const float* f = frames + (size_t)b * cam * cam;
+  __shared__ double red[256];
+  __shared__ float norma_s;
+  const int r1 = quad[0], r2 = quad[1], r3 = quad[2], r4 = quad[3];
+  const int c1 = quad[4], c2 = quad[5], c3 = quad[6], c4 = quad[7];
+  double acc = 0.0;
+  for (int p = threadIdx.x; p < w * w; p += blockDim.x) {
+    if (valid[p]) {
+      const int py = p / w, px = p % w;
+      const float I1 = f[(r1 + py) * cam + c1 + px];
+      const float I2 = f[(r2 + py) * cam + c2 + px];
+      const float I3 = f[(r3 + py) * cam + c3 + px];
+      const float I4 = f[(r4 + py) * cam + c4 + px];
+      acc += (double)I1 + (double)I2 + (double)I3 + (double)I4;
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) norma_s = (float)(red[0] / (double)nvalid);
+  __syncthreads();
+  const float inv = 1.0f / norma_s;
+  float* s2 = sig2d + (size_t)b * 2 * w * w;
+  for (int p = threadIdx.x; p < w * w; p += blockDim.x) {
+    float sx = 0.f, sy = 0.f;
+    if (valid[p]) {
+      const int py = p / w, px = p % w;
+      const float I1 = f[(r1 + py) * cam + c1 + px];
+      const float I2 = f[(r2 + py) * cam + c2 + px];
+      const float I3 = f[(r3 + py) * cam + c3 + px];
+      const float I4 = f[(r4 + py) * cam + c4 + px];
+      sx = (I1 - I2) + (I4 - I3);
+      sy = (I1 - I4) + (I2 - I3);
+    }
+    s2[p] = (sx * inv - ref2d[p]) * units;
+    s2[w * w + p] = (sy * inv - ref2d[w * w + p]) * units;
+  }
+  __syncthreads();
+  float* so = sig + (size_t)b * nsig;
+  for (int j = threadIdx.x; j < nsig; j += blockDim.x) so[j] = s2[vidx[j]];
+}
+
+// ---------------------------------------------------------------------------
+// Host dispatch
+// ---------------------------------------------------------------------------
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static size_t ws_inter_bytes(int n, int N, int T, int B) {
+  return align256((size_t)B * T * (size_t)N * n * sizeof(float2));
+}
+static size_t ws_part_bytes(int N, int B, int nchunk) {
+  return align256((size_t)B * nchunk * (size_t)(N / 4) * (N / 4) * sizeof(float));
+}
+
+template <int N>
+static int launch_fast(const float* opd, float scale, const float* amp, const float2* tx,
+                       const float2* ty, const float2* mx, const float2* my, int T, int B,
+                       int nchunk, int pts_per_task, char* ws, float* frames, hipStream_t st) {
+  constexpr int TPT = fast_tpt(N);
+  constexpr int n = N / 4;
+  constexpr int G = 64 / TPT;
+  const float2* tw = nullptr;
+  int rc = twiddles(N, &tw);
+  if (rc) return rc;
+  float2* inter = reinterpret_cast<float2*>(ws);
+  float* part = reinterpret_cast<float*>(ws + ws_inter_bytes(n, N, T, B));
+  {
+    constexpr int RPB = kWaves * G;
+    const int nrb = (n + RPB - 1) / RPB;
+    const int npc = (T + pts_per_task - 1) / pts_per_task;
+    dim3 grid((unsigned)(nrb * npc * B));
+    hipLaunchKernelGGL((k_rows_sep<N, TPT>), grid, dim3(64 * kWaves), 0, st, opd, scale, amp, tx, ty,
+                       mx, tw, inter, B, T, pts_per_task);
+    OOPAO_LAUNCH_CHECK();
+  }
+  {
+    const long tasks = (long)B * nchunk * (N / 4);
+    dim3 grid((unsigned)((tasks + kWaves - 1) / kWaves));
+    hipLaunchKernelGGL((k_cols_sep<N, TPT>), grid, dim3(64 * kWaves), 0, st, inter, my, tw, part, B,
+                       T, nchunk);
+    OOPAO_LAUNCH_CHECK();
+  }
+  {
+    const int cam = N / 4;
+    const size_t total = (size_t)B * cam * cam;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                       part, frames, B, nchunk, cam);
+    OOPAO_LAUNCH_CHECK();
+  }
+  return OK;
+}
+
+static bool fast_supported(int n, int N, int bin) {
+  return N == 4 * n && bin == 4 && fast_tpt(N) != 0;
+}
+
+}  // namespace oopao
+
+using namespace oopao;
+
+extern "C" {
+
+int oopao_pwfs_workspace_bytes(int n, int N, int T, int B, int nchunk, size_t* bytes) {
+  OOPAO_CHECK_ARG(bytes != nullptr, "bytes is null");
+  OOPAO_CHECK_ARG(n > 0 && N > 0 && T > 0 && B > 0 && nchunk > 0 && nchunk <= T,
+                  "invalid sizes n=%d N=%d T=%d B=%d nchunk=%d", n, N, T, B, nchunk);
+  *bytes = ws_inter_bytes(n, N, T, B) + ws_part_bytes(N, B, nchunk);
+  return OK;
+}
+
+int oopao_pwfs_supported(int n, int N, int bin) { return fast_supported(n, N, bin) ? 1 : 0; }
+
+int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
+                      const void* mx, const void* my, int n, int N, int T, int B, int bin, int nchunk,
+                      int pts_per_task, void* workspace, size_t ws_bytes, float* frames,
+                      void* stream) {
+  OOPAO_CHECK_ARG(opd && amp && tx && ty && mx && my && workspace && frames, "null pointer argument");
+  OOPAO_CHECK_ARG(T > 0 && B > 0 && nchunk > 0 && nchunk <= T && pts_per_task > 0,
+                  "invalid T=%d B=%d nchunk=%d pts_per_task=%d", T, B, nchunk, pts_per_task);
+  if (!fast_supported(n, N, bin)) {
+    set_error("unsupported PWFS geometry n=%d N=%d bin=%d (need N = 4 n, 4x4 binning, N in the "
+              "compiled set)", n, N, bin);
+    return ERR_UNSUPPORTED;
+  }
+  size_t need = ws_inter_bytes(n, N, T, B) + ws_part_bytes(N, B, nchunk);
+  if (ws_bytes < need) {
+    set_error("workspace too small: %zu < %zu bytes", ws_bytes, need);
+    return ERR_WORKSPACE;
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const float2* txp = reinterpret_cast<const float2*>(tx);
+  const float2* typ = reinterpret_cast<const float2*>(ty);
+  const float2* mxp = reinterpret_cast<const float2*>(mx);
+  const float2* myp = reinterpret_cast<const float2*>(my);
+  char* w = reinterpret_cast<char*>(workspace);
+  switch (N) {
+#define OOPAO_CASE(NN) \
+  case NN:             \
+    return launch_fast<NN>(opd, scale, amp, txp, typ, mxp, myp, T, B, nchunk, pts_per_task, w, frames, st);
+    OOPAO_FAST_SIZES(OOPAO_CASE)
+#undef OOPAO_CASE
+    default:
+      break;
+  }
+  set_error("unsupported padded length N=%d", N);
+  return ERR_UNSUPPORTED;
+}
+
+int oopao_pwfs_slopes(const float* frames, int B, int cam, const unsigned char* valid, int w,
+                      const int* quad, const float* ref2d, float units, const int* vidx, int nsig,
+                      int nvalid, float* sig2d, float* sig, void* stream) {
+  OOPAO_CHECK_ARG(frames && valid && quad && ref2d && vidx && sig2d && sig, "null pointer argument");
+  OOPAO_CHECK_ARG(B > 0 && cam > 0 && w > 0 && nsig >= 0 && nvalid > 0,
+                  "invalid sizes B=%d cam=%d w=%d nsig=%d nvalid=%d", B, cam, w, nsig, nvalid);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_slopes, dim3(B), dim3(256), 0, st, frames, cam, valid, w, quad, ref2d, units,
+                     vidx, nsig, nvalid, sig2d, sig);
+  OOPAO_LAUNCH_CHECK();
+  return OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,164 @@
+"""Device tables and calls for the modulated Pyramid WFS (HIP library front end).
+
+The reference builds, per modulation point i, the field
+amp * exp(i (phi + TT_i)) on the n x n pupil, pads it to N x N, multiplies by
+the PSF-centring phasor, and runs fft2 -> * mask -> ifft2 -> |.|^2
+(Pyramid.py:612-646). This module factors the separable parts on the host in
+float64 (rounded once to complex64):
+
+* tx[i, x] = exp(i px_i Tip[x]) * phasor(c0 + x), ty[i, y] likewise with Tilt
+  (Pyramid.py:334-344, 1129-1139);
+* the mask phase m[k, l] = a[k] + b[l] (the default pyramid mask,
+  Pyramid.py:453-480) gives mx[l] = exp(i b[l]) e^{-2 pi i l c0/N} / N and
+  my[k] = exp(i a[k]) e^{-2 pi i k c0/N} / N, folding the zero-padding offset c0
+  and the 1/N of each inverse transform.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .tools import OopaoError
+
+
+def _stream_ptr(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def separable_mask_factors(m, tol=1e-9):
+    """Split a mask phase m[k, l] = a[k] + b[l]; returns (a, b) or None if not separable."""
+    a = m[:, 0].copy()
+    b = m[0, :] - m[0, 0]
+    resid = np.max(np.abs(m - a[:, None] - b[None, :]))
+    scale = max(1.0, np.max(np.abs(m)))
+    if resid > tol * scale:
+        return None
+    return a, b
+
+
+def modulation_phasors(path, tip_lin, tilt_lin, N, n, psfCentering=True):
+    """tx[T, n], ty[T, n] complex128 (see module docstring)."""
+    c0 = N // 2 - n // 2
+    X = np.arange(n) + c0
+    if psfCentering:
+        ph = np.exp(-1j * np.pi * (N + 1) / N * X)
+    else:
+        ph = (-1.0) ** X
+    tx = np.exp(1j * path[:, 0:1] * tip_lin[None, :]) * ph[None, :]
+    ty = np.exp(1j * path[:, 1:2] * tilt_lin[None, :]) * ph[None, :]
+    return tx, ty
+
+
+def mask_vectors(a, b, N, n):
+    """my[N], mx[N] complex128 from the separable mask phase (see module docstring)."""
+    c0 = N // 2 - n // 2
+    k = np.arange(N)
+    shift = np.exp(-2j * np.pi * k * c0 / N) / N
+    return np.exp(1j * a) * shift, np.exp(1j * b) * shift
+
+
+class PWFSEngine:
+    """Device-resident tables, workspace and the HIP calls for one PWFS geometry."""
+
+    def __init__(self, n, N, cam, device=None, nchunk=None, pts_per_task=None):
+        self.device = torch.device('cuda', torch.cuda.current_device()) if device is None else torch.device(device)
+        self.n, self.N, self.cam = n, N, cam
+        self.bin = N // cam if cam and N % cam == 0 else 0
+        if not _lib.lib().oopao_pwfs_supported(n, N, self.bin):
+            raise OopaoError('PWFS geometry n=%d N=%d binning=%d is not supported by the HIP path '
+                             '(needs N = 4 n, 4 x 4 detector binning and a compiled N)'
+                             % (n, N, self.bin))
+        self.nchunk_user = nchunk
+        self.pts_per_task_user = pts_per_task
+        self._ws = None
+        self.T = 0
+
+    # ------------------------------------------------------------------ tables
+    def set_modulation(self, path, tip_lin, tilt_lin, psfCentering=True):
+        tx, ty = modulation_phasors(np.asarray(path, np.float64), tip_lin, tilt_lin, self.N, self.n,
+                                    psfCentering)
+        self.T = tx.shape[0]
+        self.tx = torch.from_numpy(tx.astype(np.complex64)).to(self.device)
+        self.ty = torch.from_numpy(ty.astype(np.complex64)).to(self.device)
+
+    def set_mask(self, m):
+        f = separable_mask_factors(np.asarray(m, np.float64))
+        if f is None:
+            raise OopaoError('the pyramid mask is not separable (rooftop, rotation or per-quadrant '
+                             'shifts); only separable masks are supported by the HIP path')
+        my, mx = mask_vectors(f[0], f[1], self.N, self.n)
+        self.my = torch.from_numpy(my.astype(np.complex64)).to(self.device)
+        self.mx = torch.from_numpy(mx.astype(np.complex64)).to(self.device)
+
+    # -------------------------------------------------------------- execution
+    def _knobs(self, B):
+        nchunk = self.nchunk_user or max(1, min(self.T, 8))
+        ppt = self.pts_per_task_user or max(1, min(self.T, 8))
+        return min(nchunk, self.T), ppt
+
+    def _workspace(self, B, nchunk):
+        nb = ctypes.c_size_t(0)
+        _lib.call('oopao_pwfs_workspace_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
+        if self._ws is None or self._ws.numel() < nb.value:
+            self._ws = torch.empty(nb.value, dtype=torch.uint8, device=self.device)
+        return self._ws, nb.value
+
+    def frames(self, opd, scale, amp, out=None):
+        """Detector frames [B, cam, cam] float32 for OPD/phase maps opd [B, n, n] (phase = opd*scale)."""
+        if self.T == 0:
+            raise OopaoError('modulation tables not set')
+        opd = opd.reshape(-1, self.n, self.n)
+        if opd.dtype != torch.float32 or not opd.is_contiguous() or opd.device != self.device:
+            raise OopaoError('opd must be a contiguous float32 tensor on %s' % self.device)
+        if amp.dtype != torch.float32 or tuple(amp.shape) != (self.n, self.n) or not amp.is_contiguous():
+            raise OopaoError('amp must be a contiguous float32 [n, n] tensor')
+        B = opd.shape[0]
+        nchunk, ppt = self._knobs(B)
+        ws, nb = self._workspace(B, nchunk)
+        if out is None:
+            out = torch.empty((B, self.cam, self.cam), dtype=torch.float32, device=self.device)
+        _lib.call('oopao_pwfs_frames', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx),
+                  _ptr(self.ty), _ptr(self.mx), _ptr(self.my), self.n, self.N, self.T, B, self.bin,
+                  nchunk, ppt, _ptr(ws), ctypes.c_size_t(nb), _ptr(out), _stream_ptr(self.device))
+        return out
+
+
+class SlopesEngine:
+    """Device tables for the slopesMaps reduction (Pyramid.py:835-854, 946-961)."""
+
+    def __init__(self, device, cam, w, quad_origins):
+        self.device = torch.device(device)
+        self.cam = cam
+        self.w = w
+        self.quad = torch.tensor(np.asarray(quad_origins, np.int32).reshape(8), device=self.device)
+        self.units = 1.0
+
+    def set_valid(self, validI4Q):
+        v = np.asarray(validI4Q, bool)
+        self.validI4Q = v
+        self.valid = torch.from_numpy(v.astype(np.uint8).reshape(-1)).to(self.device)
+        vs = np.concatenate((v, v))
+        self.vidx = torch.from_numpy(np.flatnonzero(vs.reshape(-1)).astype(np.int32)).to(self.device)
+        self.nsig = int(self.vidx.numel())
+        self.nvalid = int(v.sum())
+        self.ref2d = torch.zeros((2 * self.w, self.w), dtype=torch.float32, device=self.device)
+
+    def set_reference(self, ref2d):
+        self.ref2d = ref2d.to(self.device, torch.float32).contiguous()
+
+    def slopes(self, frames, sig2d=None, sig=None):
+        frames = frames.reshape(-1, self.cam, self.cam)
+        B = frames.shape[0]
+        if sig2d is None:
+            sig2d = torch.empty((B, 2 * self.w, self.w), dtype=torch.float32, device=self.device)
+        if sig is None:
+            sig = torch.empty((B, self.nsig), dtype=torch.float32, device=self.device)
+        _lib.call('oopao_pwfs_slopes', _ptr(frames), B, self.cam, _ptr(self.valid), self.w, _ptr(self.quad),
+                  _ptr(self.ref2d), ctypes.c_float(self.units), _ptr(self.vidx), self.nsig, self.nvalid,
+                  _ptr(sig2d), _ptr(sig), _stream_ptr(self.device))
+        return sig2d, sig
