@@ -1,0 +1,111 @@
+"""Detector: camera integration of WFS / science frames (Detector.py:13-454).
+
+The PWFS path hands the detector an already sum-binned device frame; integrate()
+applies photon noise, QE, readout noise and gain on the device in the
+reference's order (Detector.py:273-354). Frames stay on the GPU; ``frame``
+returns a float64 numpy copy.
+"""
+import time
+
+import numpy as np
+import torch
+
+from ._device import default_device, to_host
+from .tools import OopaoError, warning
+
+
+class Detector:
+    def __init__(self, nRes: int = None, integrationTime: float = None, bits: int = None,
+                 output_precision: int = None, FWC: int = None, gain: int = 1, sensor: str = 'CCD',
+                 QE: float = 1., binning: int = 1, psf_sampling: float = 2., darkCurrent: float = 0.,
+                 readoutNoise: float = 0, photonNoise: bool = False, backgroundNoise: bool = False,
+                 backgroundFlux: float = None, backgroundMap: float = None, log_scale: bool = False,
+                 device=None):
+        self.device = default_device() if device is None else torch.device(device)
+        self.resolution = nRes
+        self.integrationTime = integrationTime
+        self.bits = bits
+        self.output_precision = output_precision
+        self.FWC = FWC
+        self.gain = gain
+        self.sensor = sensor
+        self.psf_sampling = psf_sampling
+        if self.sensor not in ['EMCCD', 'CCD', 'CMOS']:
+            raise OopaoError("Sensor must be 'EMCCD', 'CCD', or 'CMOS'")
+        self.QE = QE
+        self.binning = binning
+        self.darkCurrent = darkCurrent
+        self.readoutNoise = readoutNoise
+        self.photonNoise = photonNoise
+        self.backgroundNoise = backgroundNoise
+        self.backgroundFlux = backgroundFlux
+        self.backgroundMap = backgroundMap
+        self.log_scale = log_scale
+        self._frame = None if nRes is None else torch.zeros((nRes, nRes), dtype=torch.float32,
+                                                              device=self.device)
+        self.saturation = 0
+        self.tag = 'detector'
+        self.buffer_frame = []
+        self._integrated_time = 0
+        self.fov_arcsec = None
+        self.pixel_size_rad = None
+        self.pixel_size_arcsec = None
+        self.quantification_noise = 0
+        self.photon_noise = 0
+        self.dark_shot_noise = 0
+        # time-seeded generators (Detector.py:137-145); a seed can be set for reproducible noise
+        self._gen = torch.Generator(device=self.device)
+        self._gen.manual_seed(int(time.time()))
+
+    def seed_noise(self, seed):
+        self._gen.manual_seed(int(seed))
+
+    @property
+    def frame(self):
+        return to_host(self._frame)
+
+    @frame.setter
+    def frame(self, val):
+        self._frame = None if val is None else torch.as_tensor(np.asarray(val), dtype=torch.float32,
+                                                                device=self.device)
+
+    def integrate(self, frame):
+        """Detector.integrate + readout (Detector.py:273-354) for a device frame (or a batch)."""
+        self.perfect_frame = frame
+        f = frame
+        if self.photonNoise != 0:
+            f = torch.poisson(torch.clamp(f, min=0), generator=self._gen)
+        if self.backgroundNoise is True:
+            if self.backgroundFlux is None:
+                raise OopaoError('The background map backgroundFlux is not properly set.')
+            bg = torch.as_tensor(np.asarray(self.backgroundFlux), dtype=torch.float32, device=self.device)
+            f = f + torch.poisson(bg.expand_as(f).contiguous(), generator=self._gen)
+        if self.QE != 1:
+            f = f * self.QE
+        self.buffer_frame.append(f)
+        if self.integrationTime is None or self._integrated_time >= self.integrationTime:
+            self.readout()
+
+    def readout(self):
+        f = self.buffer_frame[0]
+        for g in self.buffer_frame[1:]:
+            f = f + g
+        if self.FWC is not None:
+            f = torch.clamp(f, 0, self.FWC)
+        if self.readoutNoise != 0:
+            noise = torch.randn(f.shape, generator=self._gen, device=self.device) * self.readoutNoise
+            f = f + torch.round(noise)
+        if self.sensor in ('CCD', 'CMOS') and self.gain != 1:
+            f = f * self.gain
+        if self.backgroundMap is not None:
+            f = f - torch.as_tensor(np.asarray(self.backgroundMap), dtype=torch.float32, device=self.device)
+        self._frame = f
+        self.buffer_frame = []
+        self._integrated_time = 0
+
+    def print_properties(self):
+        print(self)
+
+    def __repr__(self):
+        return f'Detector(resolution={self.resolution}, QE={self.QE}, gain={self.gain}, ' \
+               f'photonNoise={self.photonNoise}, readoutNoise={self.readoutNoise})'

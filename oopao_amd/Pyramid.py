@@ -1,0 +1,612 @@
+"""Pyramid: modulated pyramid WFS drop-in (Pyramid.py:25-1230).
+
+Same constructor, attributes, setters and relay protocol as the reference.
+Every frame is computed on the GPU by the HIP library (oopao_amd/csrc/pwfs.hip):
+the modulation loop, the FFT chain, the |E|^2 accumulation and the detector
+binning run as two fused kernels plus a reduction, and the slopesMaps signal
+by one more kernel. Numbers are float32/complex64; public arrays are float64
+numpy copies in the reference's layout.
+
+There is no CPU fallback: unsupported geometries raise OopaoError.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from ._device import to_dev, to_host
+from .Detector import Detector
+from .pwfs_engine import PWFSEngine, SlopesEngine
+from .tools import OopaoError, warning
+
+
+class Pyramid:
+    def __init__(self, nSubap: float, telescope, modulation: float, lightRatio: float,
+                 postProcessing: str = 'slopesMaps', psfCentering: bool = True, n_pix_separation: float = 2.,
+                 calibModulation: float = 50., n_pix_edge: float = None, extraModulationFactor: int = 0,
+                 binning: int = 1, nTheta_user_defined: int = None, userValidSignal: bool = None,
+                 old_mask: bool = False, rooftop: float = 0, theta_rotation: float = 0, delta_theta: float = 0.,
+                 user_modulation_path: list = None, pupilSeparationRatio: float = None, edgePixel: int = None,
+                 zeroPadding: int = None, nchunk: int = None, pts_per_task: int = None,
+                 max_batch_bytes: float = 8e9):
+        self.gpu_available = False          # batching stays enabled in InteractionMatrix
+        self.nJobs = 1
+        self.precision = np.float32
+        self.precision_complex = np.complex64
+        self.telescope = telescope
+        self.device = telescope.device
+        if (self.telescope.resolution / nSubap) % 2 != 0 and self.telescope.resolution / nSubap != 1:
+            raise OopaoError('The resolution should be an even number and be a multiple of 2**i where i>=2')
+        if self.telescope.src is None:
+            raise OopaoError('The telescope was not coupled to any source object! Make sure to couple it with '
+                             'an src object using src*tel')
+        self.src = self.telescope.src
+        self.wavelength_calibration = self.src.wavelength
+        self.delta_theta = delta_theta
+        self.nTheta_user_defined = nTheta_user_defined
+        self.extraModulationFactor = extraModulationFactor
+        self.nSubap = nSubap
+        self.edgePixel = n_pix_edge
+        self.centerPixel = 0
+        self.postProcessing = postProcessing
+        self.userValidSignal = userValidSignal
+        self.psfCentering = psfCentering
+        self.binning = binning
+        self.old_mask = old_mask
+        if old_mask:
+            raise OopaoError('old_mask=True is not supported by oopao_amd')
+        self.user_modulation_path = user_modulation_path
+        self.modulation_length = 2 * np.pi
+        self.pupilSeparationRatio = pupilSeparationRatio
+        self.weight_vector = None
+        if edgePixel is not None:
+            raise OopaoError('The use of the edgePixel property has been deprecated. Consider using n_pix_edge instead')
+        if pupilSeparationRatio is not None:
+            raise OopaoError('The use of the pupilSeparationRatio property has been deprecated. Consider using '
+                             'n_pix_separation instead')
+        self.n_pix_separation = n_pix_separation
+        self.sx = [0, 0, 0, 0]
+        self.sy = [0, 0, 0, 0]
+        if n_pix_edge is None:
+            self.n_pix_edge = self.n_pix_separation // 2
+        else:
+            self.n_pix_edge = n_pix_edge
+            if n_pix_edge != self.n_pix_separation // 2:
+                warning('The recommanded value for n_pix_edge is ' + str(self.n_pix_separation // 2)
+                        + ' instead of ' + str(n_pix_edge))
+        self.joblib_setting = 'threads'
+        self.rooftop = rooftop
+        self.theta_rotation = theta_rotation
+        if zeroPadding is not None:
+            raise OopaoError('The use of the zeroPadding property has been deprecated')
+        n = self.telescope.resolution
+        # geometry (Pyramid.py:276-289)
+        self.resolution = int((self.nSubap * 2 + self.n_pix_separation + self.n_pix_edge * 2) * n / self.nSubap)
+        self.zeroPaddingFactor = self.resolution / n
+        self.zeroPadding = (self.resolution - n) // 2
+        self.tag = 'pyramid'
+        self.cam = Detector(round(nSubap * self.zeroPaddingFactor), device=self.device)
+        self.focal_plane_camera = Detector(int((modulation * 4 + 12) * self.zeroPaddingFactor * n /
+                                               self.telescope.initial_resolution),
+                                           psf_sampling=self.zeroPaddingFactor * n / self.telescope.initial_resolution,
+                                           device=self.device)
+        self.focal_plane_camera.is_focal_plane_camera = True
+        self._lightRatio = lightRatio
+        self.calibModulation = n / 2 - 1 if calibModulation >= n / 2 else calibModulation
+        self.isInitialized = False
+        self.isCalibrated = False
+        self._delta_Tip = 0
+        self._delta_Tilt = 0
+        self.center = self.resolution // 2
+        self._spatialFilter = None
+        self.rad2arcsec = (180 / np.pi) * 3600
+        self.fov = self.rad2arcsec * self.resolution / self.zeroPaddingFactor * (self.src.wavelength / self.telescope.D)
+        self.fov_l_d = self.resolution / self.zeroPaddingFactor
+        self.max_fov_arcsec = self.fov / 2
+        self.n_max = 1e9
+        self.max_batch_bytes = max_batch_bytes
+        # modulation tip/tilt (Pyramid.py:334-339): Tip varies along x (columns), Tilt along y (rows)
+        lin = np.linspace(-np.pi, np.pi, n) * (n / self.telescope.initial_resolution)
+        self._tip_lin = lin
+        self._tilt_lin = lin
+        # HIP engine
+        self.engine = PWFSEngine(n, self.resolution, self.cam.resolution, self.device, nchunk=nchunk,
+                                 pts_per_task=pts_per_task)
+        self.mask_computation()
+        self._setup_slopes()
+        self.slopesUnits = 1
+        self.referenceSignal = 0
+        self._referenceSignal_2D = None
+        self.referencePyramidFrame = 0
+        self._signal = None
+        self._signal_2D = None
+        self.modulation = modulation
+        print('Selection of the valid pixels...')
+        self.initialization()
+        print('Acquisition of the reference slopes and units calibration...')
+        self.modulation = modulation
+        self.wfs_calibration()
+        self.src ** self.telescope
+        self.wfs_measure()
+        print(self)
+
+    # ------------------------------------------------------------------ mask
+    def get_phase_mask(self, resolution, n_subap, n_pix_separation, n_pix_edge, psf_centering=False,
+                       sx=[0, 0, 0, 0], sy=[0, 0, 0, 0]):
+        """Pyramid phase mask (Pyramid.py:453-480)."""
+        rooftop_pixels = self.rooftop / np.sqrt(2)
+        n_tot = int((n_subap * 2 + n_pix_separation + n_pix_edge * 2) * self.telescope.resolution / self.nSubap)
+        norma = self.telescope.resolution / self.nSubap
+        lim = np.pi
+        x = np.linspace(-(1 - self.psfCentering * 1 / n_tot) * lim, (1 - self.psfCentering * 1 / n_tot) * lim, n_tot,
+                        endpoint=self.psfCentering)
+        x_, y_ = np.meshgrid(x, x)
+        xr = x_ * np.cos(self.theta_rotation) - y_ * np.sin(self.theta_rotation)
+        yr = y_ * np.cos(self.theta_rotation) + x_ * np.sin(self.theta_rotation)
+        r = (self.nSubap + self.n_pix_separation) / 2
+        P1 = xr * r + x_ * sx[0] + yr * r - y_ * sy[0] + rooftop_pixels
+        P2 = -xr * r + x_ * sx[1] + yr * r - y_ * sy[1]
+        P3 = -xr * r + x_ * sx[2] - yr * r - y_ * sy[2] + rooftop_pixels
+        P4 = xr * r + x_ * sx[3] - yr * r - y_ * sy[3]
+        return -np.max(np.stack([P1, P2, P3, P4]) * norma, axis=0)
+
+    def mask_computation(self):
+        print('Pyramid Mask initialization...')
+        self.m = self.get_phase_mask(self.resolution, self.nSubap, self.n_pix_separation, self.n_pix_edge,
+                                     self.psfCentering, self.sx, self.sy)
+        self.initial_m = self.m.copy()
+        self.mask = np.complex64(np.exp(1j * self.m))
+        self.initial_mask = np.copy(self.mask)
+        self.engine.set_mask(self.m)
+
+    def apply_shift_wfs(self, sx=None, sy=None, mis_reg=None, units='pixels'):
+        """Per-quadrant pupil shifts (Pyramid.py:393-451). Uniform shifts keep the mask separable."""
+        sx = 0 if sx is None else sx
+        sy = 0 if sy is None else sy
+        if mis_reg is not None:
+            sx = [mis_reg.dX_1, mis_reg.dX_2, mis_reg.dX_3, mis_reg.dX_4]
+            sy = [mis_reg.dY_1, mis_reg.dY_2, mis_reg.dY_3, mis_reg.dY_4]
+        factor = 1 if units == 'pixels' else 1 / (self.telescope.D / self.nSubap)
+        if np.isscalar(sx) and np.isscalar(sy):
+            shift_x = [factor * sx] * 4
+            shift_y = [factor * sy] * 4
+        else:
+            if len(sx) == 4 and len(sy) == 4:
+                shift_x = [v * factor for v in sx]
+                shift_y = [v * factor for v in sy]
+            else:
+                raise OopaoError('Wrong size for sx and/or sy, a list of 4 values is expected.')
+        if np.max(np.abs(shift_x)) > self.n_pix_edge or np.max(np.abs(shift_y)) > self.n_pix_edge:
+            warning('The Pyramid pupils have been shifted outside of the detector!' +
+                    'Wrapping of the signal is currently occuring!!')
+        self.sx = np.asarray(shift_x) / factor
+        self.sy = np.asarray(shift_y) / factor
+        self.m = self.get_phase_mask(self.resolution, self.nSubap, self.n_pix_separation, self.n_pix_edge,
+                                     self.psfCentering, shift_x, shift_y)
+        self.mask = np.complex64(np.exp(1j * self.m))
+        self.engine.set_mask(self.m)
+        self.slopesUnits = 1
+        self.referenceSignal = 0
+        self._referenceSignal_2D = None
+        self.wfs_calibration()
+
+    # ---------------------------------------------------------- slopes setup
+    def _quadrant_origins(self):
+        """Row/column origins of quadrants 1..4 (Pyramid.py:946-961)."""
+        n = self.telescope.resolution
+        e = int(np.round((self.n_pix_separation / self.nSubap) * n / (n / self.nSubap) / 2 / self.binning))
+        c = int(np.round((self.cam.resolution / self.binning) / 2))
+        w = int(np.ceil(self.nSubap / self.binning))
+        rows = [c - e - w, c - e - w, c + e, c + e]
+        cols = [c - e - w, c + e, c + e, c - e - w]
+        return rows + cols, w
+
+    def _setup_slopes(self):
+        quad, w = self._quadrant_origins()
+        self._w = w
+        self.slopes_engine = SlopesEngine(self.device, self.cam.resolution // self.binning, w, quad)
+
+    # ------------------------------------------------------------ modulation
+    @property
+    def modulation(self):
+        return self._modulation
+
+    @modulation.setter
+    def modulation(self, val):
+        """Modulation setter (Pyramid.py:1107-1154)."""
+        self._modulation = val
+        if self._modulation >= (self.telescope.resolution // 2):
+            raise OopaoError('Error the modulation radius is too large for this resolution!' +
+                             'Consider using a larger telescope resolution!')
+        if val != 0 or self.user_modulation_path is not None:
+            if self.user_modulation_path is not None:
+                self.modulation_path = list(self.user_modulation_path)
+                self.nTheta = len(self.user_modulation_path)
+            else:
+                perimeter = self.modulation_length * self._modulation
+                if self.nTheta_user_defined is None:
+                    self.nTheta = 4 * int((self.extraModulationFactor + np.ceil(perimeter / 4)))
+                else:
+                    self.nTheta = self.nTheta_user_defined
+                self.thetaModulation = np.linspace(0 + self.delta_theta, 2 * np.pi + self.delta_theta,
+                                                   self.nTheta, endpoint=False)
+                self.modulation_path = [[self.modulation * np.cos(t) + self.delta_Tip,
+                                         self.modulation * np.sin(t) + self.delta_Tilt] for t in self.thetaModulation]
+            path = np.asarray(self.modulation_path, dtype=np.float64)
+        else:
+            self.nTheta = 1
+            path = np.zeros((1, 2))
+        self._path = path
+        self.engine.set_modulation(path, self._tip_lin, self._tilt_lin, self.psfCentering)
+        if self.isCalibrated:
+            print('Updating the reference slopes')
+            self.slopesUnits = 1
+            self.referenceSignal = 0
+            self._referenceSignal_2D = None
+            self.wfs_calibration()
+            print('Done!')
+
+    @property
+    def phaseBuffModulationLowres(self):
+        """float32 tip/tilt phase per modulation point [T, n, n] (Pyramid.py:1133-1139)."""
+        tip, tilt = np.meshgrid(self._tip_lin, self._tilt_lin)
+        return np.stack([(p[0] * tip + p[1] * tilt).astype(np.float32) for p in self._path])
+
+    @property
+    def delta_Tip(self):
+        return self._delta_Tip
+
+    @delta_Tip.setter
+    def delta_Tip(self, val):
+        self._delta_Tip = val
+        if self.isCalibrated:
+            self.modulation = self.modulation
+
+    @property
+    def delta_Tilt(self):
+        return self._delta_Tilt
+
+    @delta_Tilt.setter
+    def delta_Tilt(self, val):
+        self._delta_Tilt = val
+        if self.isCalibrated:
+            self.modulation = self.modulation
+
+    # ---------------------------------------------------------- valid pixels
+    def grabQuadrant(self, n, cameraFrame=None):
+        quad, w = self._quadrant_origins()
+        frame = self.cam.frame if cameraFrame is None else np.asarray(cameraFrame)
+        r, c = quad[n - 1], quad[4 + n - 1]
+        return frame[r:r + w, c:c + w].copy()
+
+    def grabFullQuadrant(self, n, cameraFrame=None):
+        n_tot = self.cam.resolution
+        frame = self.cam.frame if cameraFrame is None else np.asarray(cameraFrame)
+        if n == 3:
+            return frame[:n_tot // 2, :n_tot // 2]
+        if n == 4:
+            return frame[:n_tot // 2, -n_tot // 2:]
+        if n == 1:
+            return frame[-n_tot // 2:, -n_tot // 2:]
+        if n == 2:
+            return frame[-n_tot // 2:, :n_tot // 2]
+
+    def initialization(self):
+        """Valid pixel selection at the calibration modulation (Pyramid.py:541-589)."""
+        self.src ** self.telescope
+        if self.userValidSignal is None:
+            if self.lightRatio == 0:
+                self.cam.frame = np.ones([self.cam.resolution, self.cam.resolution])
+            else:
+                print('The valid pixel are selected on flux considerations')
+                self.modulation = self.calibModulation
+                self.wfs_measure()
+            self.initFrame = self.cam.frame
+            if self.postProcessing[:10] == 'slopesMaps':
+                I4Q = sum(self.grabQuadrant(k) for k in (1, 2, 3, 4))
+                self.I4Q = I4Q
+                self.validI4Q = (self.I4Q >= self.lightRatio * self.I4Q.max())
+                self.validSignal = np.concatenate((self.validI4Q, self.validI4Q))
+                self.nSignal = int(np.sum(self.validSignal))
+            if self.postProcessing[:9] == 'fullFrame':
+                self.validSignal = (self.initFrame >= self.lightRatio * self.initFrame.max())
+                self.nSignal = int(np.sum(self.validSignal))
+        else:
+            print('You are using a user-defined mask for the selection of the valid pixel')
+            if self.postProcessing[:10] == 'slopesMaps':
+                self.validI4Q = np.asarray(self.userValidSignal, bool)
+                self.validSignal = np.concatenate((self.validI4Q, self.validI4Q))
+            if self.postProcessing[:9] == 'fullFrame':
+                self.validSignal = np.asarray(self.userValidSignal, bool)
+            self.nSignal = int(np.sum(self.validSignal))
+        if self.postProcessing[:10] == 'slopesMaps':
+            self.slopes_engine.set_valid(self.validI4Q)
+        else:
+            self._valid_full = torch.from_numpy(self.validSignal.astype(bool)).to(self.device)
+        self.isInitialized = True
+
+    @property
+    def lightRatio(self):
+        return self._lightRatio
+
+    @lightRatio.setter
+    def lightRatio(self, val):
+        self._lightRatio = val
+        if getattr(self, 'isInitialized', False):
+            print('Updating the map if valid pixels ...')
+            self.validI4Q = (self.I4Q >= self._lightRatio * self.I4Q.max())
+            self.validSignal = np.concatenate((self.validI4Q, self.validI4Q))
+            self.validPix = (self.initFrame >= self.lightRatio * self.initFrame.max())
+            if self.postProcessing[:10] == 'slopesMaps':
+                self.nSignal = int(np.sum(self.validSignal))
+                self.slopes_engine.set_valid(self.validI4Q)
+                if self._referenceSignal_2D is not None:
+                    self.slopes_engine.set_reference(self._referenceSignal_2D)
+            if self.postProcessing[:9] == 'fullFrame':
+                self.nSignal = int(np.sum(self.validPix))
+            print('Done!')
+
+    # ------------------------------------------------------------ calibration
+    def wfs_calibration(self, src=None):
+        """Reference signal on the flat wavefront (Pyramid.py:591-610)."""
+        tmp = (self.src._OPD.clone() if self.src._OPD is not None else None,
+               self.src._OPD_no_pupil.clone() if self.src._OPD_no_pupil is not None else None)
+        if src is None:
+            self.src ** self.telescope
+        else:
+            self.src = src
+        self.relay(self.src)
+        self._sync_reference()
+        s2d, s = self._signal_processing_dev()
+        self._referenceSignal_2D = s2d[0] if s2d.dim() == 3 else s2d
+        self.referenceSignal = to_host(s[0] if s.dim() == 2 else s)
+        if self.postProcessing[:10] == 'slopesMaps':
+            self.slopes_engine.set_reference(self._referenceSignal_2D)
+        if self.isCalibrated is False:
+            print('WFS calibrated!')
+        self.isCalibrated = True
+        self.src._OPD, self.src._OPD_no_pupil = tmp
+
+    def _sync_reference(self):
+        """Make the slopes kernel subtract the current referenceSignal_2D (zeros when reset)."""
+        if self.postProcessing[:10] != 'slopesMaps' or not hasattr(self.slopes_engine, 'valid'):
+            return
+        if self._referenceSignal_2D is None:
+            self.slopes_engine.set_reference(torch.zeros((2 * self._w, self._w), device=self.device))
+        else:
+            self.slopes_engine.set_reference(self._referenceSignal_2D)
+
+    @property
+    def slopesUnits(self):
+        return self._slopesUnits
+
+    @slopesUnits.setter
+    def slopesUnits(self, val):
+        self._slopesUnits = val
+        if hasattr(self, 'slopes_engine'):
+            self.slopes_engine.units = float(val)
+
+    @property
+    def referenceSignal_2D(self):
+        return 0 if self._referenceSignal_2D is None else to_host(self._referenceSignal_2D)
+
+    @referenceSignal_2D.setter
+    def referenceSignal_2D(self, val):
+        if np.isscalar(val):
+            self._referenceSignal_2D = None
+            self.slopes_engine.set_reference(torch.zeros((2 * self._w, self._w), device=self.device))
+        else:
+            self._referenceSignal_2D = torch.as_tensor(np.asarray(val), dtype=torch.float32, device=self.device)
+            self.slopes_engine.set_reference(self._referenceSignal_2D)
+
+    # ------------------------------------------------------------ measurement
+    def _amp(self):
+        """Field amplitude sqrt(src.intensity / nTheta) on the device (Pyramid.py:674)."""
+        return torch.sqrt(self.src._intensity / self.nTheta).contiguous()
+
+    def _batch_size(self, B):
+        per = self.engine.N * self.engine.n * 8 * max(self.nTheta, 1)
+        return max(1, min(B, int(self.max_batch_bytes // per)))
+
+    def frames_dev(self, opd):
+        """Detector frames for device OPD maps [B, n, n] (metres) of the current source."""
+        amp = self._amp()
+        scale = 2 * np.pi / self.src.wavelength
+        B = opd.shape[0]
+        bs = self._batch_size(B)
+        if bs >= B:
+            return self.engine.frames(opd.contiguous(), scale, amp)
+        out = torch.empty((B, self.engine.cam, self.engine.cam), dtype=torch.float32, device=self.device)
+        for j in range(0, B, bs):
+            self.engine.frames(opd[j:j + bs].contiguous(), scale, amp, out=out[j:j + bs])
+        return out
+
+    def wfs_measure(self, phase_in=None, integrate=True):
+        """Pyramid.wfs_measure (Pyramid.py:666-833): all modulation points, 2-D or batched 3-D."""
+        if self.isInitialized and self.isCalibrated:
+            if self.wavelength_calibration != self.src.wavelength:
+                raise OopaoError('A change in wavelength was detected in the WFS object \n' +
+                                 'Make sure that the correct source is propagated in the WFS object or '
+                                 're-calibrate with the correct source.')
+        if phase_in is not None:
+            self.src.phase = phase_in
+        if self.spatialFilter is not None:
+            raise OopaoError('spatial filters are not supported by oopao_amd')
+        opd = self.src._OPD
+        batched = opd.dim() == 3
+        frames = self.frames_dev(opd if batched else opd[None])
+        if self.binning != 1:
+            b = self.binning
+            c = frames.shape[-1]
+            frames = frames.reshape(-1, c // b, b, c // b, b).sum(dim=(2, 4))
+        self.cam._integrated_time += self.telescope.samplingTime
+        self.cam.integrate(frames if batched else frames[0])
+        if integrate and self.isInitialized and self.isCalibrated:
+            s2d, s = self._signal_processing_dev()
+            self._signal_2D, self._signal = s2d, s
+            self._batched = batched
+
+    def wfs_integrate(self):
+        if self.isInitialized and self.isCalibrated:
+            s2d, s = self._signal_processing_dev()
+            return to_host(s2d), to_host(s)
+        return None, None
+
+    def _signal_processing_dev(self):
+        frame = self.cam._frame
+        if self.postProcessing == 'slopesMaps':
+            s2d, s = self.slopes_engine.slopes(frame)
+            return (s2d, s) if frame.dim() == 3 else (s2d[0], s[0])
+        return self._signal_processing_other(frame)
+
+    def _signal_processing_other(self, frame):
+        """Other post-processing modes (Pyramid.py:856-926) as device tensor expressions."""
+        f = frame if frame.dim() == 3 else frame[None]
+        pp = self.postProcessing
+        ref = self._referenceSignal_2D if self._referenceSignal_2D is not None else 0
+        if pp.startswith('slopesMaps'):
+            quad, w = self._quadrant_origins()
+            v = torch.from_numpy(self.validI4Q.astype(np.float32)).to(self.device)
+            Iq = [f[:, quad[k]:quad[k] + w, quad[4 + k]:quad[4 + k] + w] * v for k in range(4)]
+            if pp == 'slopesMaps_incidence_flux':
+                norma = float(self.src.nPhoton * self.telescope.samplingTime * (self.telescope.D / self.nSubap) ** 2)
+                norma = torch.full((f.shape[0], 1, 1), norma, device=self.device)
+            elif pp == 'slopesMaps_camera_flux':
+                norma = f.double().mean(dim=(1, 2)).float()[:, None, None]
+            else:
+                raise OopaoError('unknown postProcessing ' + pp)
+            Sx = Iq[0] - Iq[1] + Iq[3] - Iq[2]
+            Sy = Iq[0] - Iq[3] + Iq[1] - Iq[2]
+            s2d = (torch.cat((Sx, Sy), dim=1) / norma - ref) * self.slopesUnits
+            vs = torch.from_numpy(self.validSignal.astype(bool)).to(self.device)
+            s = s2d[:, vs]
+        else:
+            if pp == 'fullFrame_camera_flux':
+                norma = f.double().mean(dim=(1, 2)).float()[:, None, None]
+            elif pp == 'fullFrame_incidence_flux':
+                norma = float(self.src.nPhoton * self.telescope.samplingTime * (self.telescope.D / self.nSubap) ** 2) / 4
+                norma = torch.full((f.shape[0], 1, 1), norma, device=self.device)
+            elif pp == 'fullFrame_sum_flux':
+                norma = f.double().sum(dim=(1, 2)).float()[:, None, None]
+            elif pp == 'fullFrame':
+                norma = f[:, self._valid_full].double().sum(dim=1).float()[:, None, None]
+            else:
+                raise OopaoError('unknown postProcessing ' + pp)
+            s2d = f / norma - ref
+            s = s2d[:, self._valid_full]
+        if frame.dim() == 2:
+            return s2d[0], s[0]
+        return s2d, s
+
+    def signalProcessing(self, cameraFrame=None):
+        """(signal_2D, signal) of the current camera frame (Pyramid.py:835-926)."""
+        s2d, s = self._signal_processing_dev()
+        return to_host(s2d), self._sig_host(s)
+
+    def _sig_host(self, s):
+        a = to_host(s)
+        if a is not None and a.ndim == 2:  # [B, nSig] -> [nSig, B]
+            a = a.T.copy()
+        return a
+
+    # --------------------------------------------------------------- outputs
+    @property
+    def signal(self):
+        return self._sig_host(self._signal)
+
+    @signal.setter
+    def signal(self, val):
+        self._signal = None if val is None else torch.as_tensor(np.asarray(val), dtype=torch.float32,
+                                                                 device=self.device)
+
+    @property
+    def pyramidSignal(self):
+        return self.signal
+
+    @property
+    def signal_2D(self):
+        return to_host(self._signal_2D)
+
+    @property
+    def pyramidSignal_2D(self):
+        return self.signal_2D
+
+    @property
+    def validSignal(self):
+        return self._validSignal
+
+    @validSignal.setter
+    def validSignal(self, val):
+        self._validSignal = val
+        self.valid_signal_2D = val
+
+    @property
+    def resolution(self):
+        return self._resolution
+
+    @resolution.setter
+    def resolution(self, val):
+        self._resolution = val
+        self.nRes = val
+
+    @property
+    def spatialFilter(self):
+        return self._spatialFilter
+
+    @spatialFilter.setter
+    def spatialFilter(self, val):
+        if val is not None:
+            raise OopaoError('spatial filters are not supported by oopao_amd')
+        self._spatialFilter = None
+
+    # ---------------------------------------------------------------- relay
+    def relay(self, src):
+        """Pyramid.relay (Pyramid.py:1156-1176)."""
+        if src.tag == 'source':
+            src_list = [src]
+        elif src.tag == 'asterism':
+            src_list = src.src
+        sig2d, sig, frames = [], [], []
+        for s in src_list:
+            s.optical_path.append([self.tag, self])
+            self.src = s
+            self.wfs_measure()
+            sig2d.append(self._signal_2D)
+            sig.append(self._signal)
+            frames.append(self.cam._frame)
+        if len(src_list) > 1:
+            self._signal_2D = torch.stack(sig2d) if sig2d[0] is not None else None
+            self._signal = torch.stack(sig) if sig[0] is not None else None
+        self.frames = frames
+
+    def __mul__(self, obj):
+        if obj.tag == 'detector':
+            if getattr(obj, 'is_focal_plane_camera', False):
+                raise OopaoError('the focal-plane camera is not computed by oopao_amd')
+            obj.integrate(self.cam._frame)
+        else:
+            raise OopaoError('Error light propagated to the wrong type of object')
+
+    @property
+    def raw_data(self):
+        raise OopaoError('the unbinned raw frame is not produced by the fused HIP path; use cam.frame')
+
+    def print_properties(self):
+        print(self)
+
+    def properties(self) -> dict:
+        self.prop = dict()
+        self.prop['pupil_diameter'] = f"{'Pupil diameter [px]':<25s}|{self.nSubap:^9d}"
+        self.prop['pupil_separation'] = f"{'Pupil separation [px]':<25s}|{self.n_pix_separation:^9.2f}"
+        self.prop['fov'] = f"{'Field of view [arcsec]':<25s}|{self.fov:^9.2f}"
+        self.prop['modulation'] = f"{'Modulation radius [l/D]':<25s}|{self.modulation:^9.1f}"
+        self.prop['psf_sampling'] = f"{'PSF sampling [px/(l/D)]':<25s}|{self.zeroPaddingFactor:^9.2f}"
+        self.prop['psf_centering'] = f"{'PSF centering':<25s}|{str(self.psfCentering):^9s}"
+        self.prop['n_valid_pixels'] = f"{'Valid pixels':<25s}|{self.nSignal:^9.0f}"
+        self.prop['post_processing'] = f"{'Post processing':<25s}|{self.postProcessing:^9s}"
+        return self.prop
+
+    def __repr__(self):
+        self.properties()
+        n_char = len(max(self.prop.values(), key=len))
+        return (f'\n{" Pyramid WFS ":-^{n_char}}\n' + ''.join(v + '\n' for v in self.prop.values())
+                + f'{"":-^{n_char}}\n')

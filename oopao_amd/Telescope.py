@@ -1,0 +1,204 @@
+"""Telescope: pupil, flux and the `tel*obj`, `tel+atm` operators (Telescope.py:19-617).
+
+The pupil is built exactly as the reference (Telescope.py:226-246) on the host,
+then mirrored on the GPU; relay() sets the source's mask and flux map on the
+device (Telescope.py:199-224).
+"""
+import copy
+
+import numpy as np
+import torch
+
+from ._device import default_device, to_host
+from .tools import OopaoError, warning
+
+
+class Telescope:
+    def __init__(self, resolution: int, diameter: float, samplingTime: float = 0.001,
+                 centralObstruction: float = 0., fov: float = 0., pupil: bool = None,
+                 pupilReflectivity: float = 1., display_optical_path: bool = False, device=None):
+        self.device = default_device() if device is None else torch.device(device)
+        self.precision = np.float64
+        self.isInitialized = False
+        self.resolution = resolution
+        self.D = diameter
+        self.initial_D = diameter
+        self.initial_resolution = resolution
+        self.pixelSize = self.D / self.resolution
+        self.centralObstruction = centralObstruction
+        self.fov = fov
+        self.rad2arcsec = (180 / np.pi) * 3600
+        self.fov_rad = fov / self.rad2arcsec
+        self.samplingTime = samplingTime
+        self.isPetalFree = False
+        self.index_pixel_petals = None
+        self.optical_path = None
+        self.user_defined_pupil = pupil
+        self.pupilReflectivity = np.float64(pupilReflectivity)
+        self.set_pupil()
+        self.src = None
+        self.tag = 'telescope'
+        self.isPaired = False
+        self.spatialFilter = None
+        self.display_optical_path = display_optical_path
+        self.coronagraph_diameter = None
+        print(self)
+        self.isInitialized = True
+        self.apply_off_axis_tip_tilt = True
+        self.initial_pupil = self.pupil.copy()
+        self.static_OPD = np.zeros([self.resolution, self.resolution])
+
+    def set_pupil(self):
+        """Circular pupil with central obstruction (Telescope.py:226-246)."""
+        if self.user_defined_pupil is None:
+            D = self.resolution + 1
+            x = np.linspace(-self.resolution / 2, self.resolution / 2, self.resolution)
+            xx, yy = np.meshgrid(x, x)
+            circle = xx ** 2 + yy ** 2
+            obs = circle >= (self.centralObstruction * D / 2) ** 2
+            pupil = (circle < (D / 2) ** 2) * obs
+        else:
+            warning('User-defined pupil, the central obstruction will not be taken into account...')
+            pupil = self.user_defined_pupil.copy()
+        refl = self.pupilReflectivity
+        self.pupil = pupil
+        self.pupilReflectivity = (self.pupil * refl).astype(np.float64)
+
+    @property
+    def pupil(self):
+        return self._pupil
+
+    @pupil.setter
+    def pupil(self, val):
+        self._pupil = np.asarray(val).astype(bool)
+        self.pixelArea = np.sum(self._pupil)
+        self.pupilLogical = np.where(self._pupil.reshape(self.resolution ** 2) > 0)
+        self.pupilReflectivity = self._pupil.astype(np.float64)
+        self._pupil_dev = torch.from_numpy(self._pupil.astype(np.float32)).to(self.device)
+        if self.isInitialized:
+            warning('A new pupil is now considered, its reflectivity is considered to be uniform. '
+                    'Assign the proper reflectivity map to tel.pupilReflectivity if required.')
+
+    @property
+    def pupilReflectivity(self):
+        return self._pupilReflectivity
+
+    @pupilReflectivity.setter
+    def pupilReflectivity(self, val):
+        self._pupilReflectivity = val
+        if np.ndim(val) == 2:
+            self._refl_dev = torch.from_numpy(np.asarray(val, np.float32)).to(self.device)
+        else:
+            self._refl_dev = None
+
+    def relay(self, src):
+        """Telescope.relay (Telescope.py:199-224) on device tensors."""
+        self.src = src
+        if src.tag == 'source':
+            self.src_list = [src]
+        elif src.tag == 'asterism':
+            self.src_list = src.src
+        for s in self.src_list:
+            s.optical_path.append([self.tag, self])
+            s.tel = self
+            s._set_mask(self.pupil, self._pupil_dev)
+            if s._OPD is None:
+                s._set_OPD_no_pupil_dev(torch.zeros((self.resolution, self.resolution),
+                                                    dtype=torch.float32, device=self.device))
+            if s._OPD.dim() == 2:
+                s._set_OPD_no_pupil_dev(s._OPD_no_pupil if s._OPD_no_pupil is not None else s._OPD)
+            else:
+                s._set_OPD_no_pupil_dev(torch.zeros((self.resolution, self.resolution),
+                                                    dtype=torch.float32, device=self.device))
+            if s._scintillation is None or s._scintillation_no_pupil is None:
+                s._scintillation_no_pupil = torch.ones((self.resolution, self.resolution),
+                                                       dtype=torch.float32, device=self.device)
+            s._scintillation = s._scintillation_no_pupil * self._pupil_dev
+            flux = self.src_flux_scale(s)
+            s._fluxMap = self._refl_dev * flux
+            s._refresh_intensity()
+
+    def src_flux_scale(self, src):
+        """nPhoton * samplingTime * pixel area (Telescope.py:223)."""
+        return float(src.nPhoton * self.samplingTime * (self.D / self.resolution) ** 2)
+
+    # ------------------------------------------------------------------ OPD
+    @property
+    def OPD(self):
+        op = self.src.OPD
+        if op is not None and op.ndim == 2:
+            self.mean_removed_OPD = (op - np.mean(op[np.where(self.pupil == 1)])) * self.pupil
+        return op
+
+    @OPD.setter
+    def OPD(self, val):
+        self.src.OPD = val
+
+    @property
+    def OPD_no_pupil(self):
+        return self.src.OPD_no_pupil
+
+    @OPD_no_pupil.setter
+    def OPD_no_pupil(self, val):
+        self.src.OPD_no_pupil = val
+
+    def resetOPD(self):
+        warning('The use of resetOPD is deprecated. Consider using the Source reset option using the ** '
+                'operator to reset the light propagation:\n src**tel')
+        self.src ** self
+
+    # ------------------------------------------------------------ operators
+    def __mul__(self, obj):
+        """Telescope.__mul__ (Telescope.py:524-546)."""
+        if type(obj) is list:
+            for o in obj:
+                self * o
+        else:
+            self.relay(self.src)
+            obj.relay(self.src)
+        return self
+
+    def __add__(self, obj):
+        if obj.tag == 'atmosphere':
+            self.isPaired = True
+            self.src * obj * self
+            self.atm = obj
+        if obj.tag == 'spatialFilter':
+            self.spatialFilter = obj
+            self * obj
+            print('Telescope and Spatial Filter combined!')
+
+    def __sub__(self, obj):
+        if obj.tag == 'atmosphere':
+            self.isPaired = False
+            self.src.reset()
+            obj.asterism = None
+        if obj.tag == 'spatialFilter':
+            self.spatialFilter = None
+            print('Telescope and Spatial Filter separated!')
+
+    def print_optical_path(self):
+        if self.optical_path is not None:
+            print(' ~~> '.join(p[0] for p in self.optical_path))
+        else:
+            print('No light propagated through the telescope')
+
+    def print_properties(self):
+        print(self)
+
+    def properties(self) -> dict:
+        self.prop = dict()
+        self.prop['diameter'] = f"{'Diameter [m]':<25s}|{self.D:^10.2f}"
+        self.prop['resolution'] = f"{'Resolution [px]':<25s}|{self.resolution:^10.0f}"
+        self.prop['pixel_size'] = f"{'Pixel size [m]':<25s}|{self.pixelSize:^10.2f}"
+        self.prop['surface'] = f"{'Surface [m²]':<25s}|{self.pixelArea * self.pixelSize ** 2:^10.2f}"
+        self.prop['obstruction'] = f"{'Central obstruction [%]':<25s}|{self.centralObstruction * 100:^10.0f}"
+        self.prop['n_pix_pupil'] = f"{'Pixels in pupil':<25s}|{self.pixelArea:^10.0f}"
+        self.prop['fov'] = f"{'Field of view [arcsec]':<25s}|{self.fov:^10.2f}"
+        return self.prop
+
+    def __repr__(self):
+        self.properties()
+        n_char = len(max(self.prop.values(), key=len))
+        return (f'\n{" Telescope ":-^{n_char}}\n' + ''.join(v + '\n' for v in self.prop.values())
+                + f'{"":-^{n_char}}\n')

@@ -29,6 +29,11 @@ SIGNATURES = {
                                   c_void_p, c_void_p]),
     'oopao_pwfs_slopes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_float,
                                   c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    'oopao_dm_gaussian_modes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_double, c_double, c_double,
+                                        c_float, c_void_p]),
+    'oopao_gemv': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_float, c_void_p]),
+    'oopao_dm_separable_opd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                                       c_void_p]),
 }
 
 

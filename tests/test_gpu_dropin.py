@@ -1,0 +1,82 @@
+"""GPU: the drop-in object chain (src**tel*dm*wfs) against the oracle at config 1.
+
+Reference: Telescope.relay (Telescope.py:199-224), DeformableMirror.coefs /
+relay (DeformableMirror.py:427-457, 651-686), Pyramid.__init__ / wfs_measure /
+signalProcessing (Pyramid.py:213-366, 541-610, 666-854).
+Tolerances: float32 GPU path vs float64 oracle, rel-L2 <= 1e-5 on slopes;
+<= 1e-6 on the DM OPD.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import rel_l2, config1_system, random_phase
+
+pytestmark = pytest.mark.gpu
+
+
+def _config1_objects():
+    from oopao_amd import Telescope, Source, DeformableMirror, Pyramid
+    tel = Telescope(80, 8, 1e-3, centralObstruction=0)
+    ngs = Source('I3', 8, display_properties=False)
+    ngs * tel
+    dm = DeformableMirror(tel, 20, 0.35, print_dm_properties=False)
+    wfs = Pyramid(20, tel, 3, 0.5, n_pix_separation=20, n_pix_edge=10, nTheta_user_defined=48)
+    return tel, ngs, dm, wfs
+
+
+def test_dm_opd_matches_oracle():
+    from oracle.dm import DMOracle
+    tel, ngs, dm, wfs = _config1_objects()
+    o = DMOracle(80, 8, 20, 0.35)
+    assert dm.nValidAct == o.nValidAct == 357
+    coefs = np.random.RandomState(2).normal(0, 20e-9, 357)
+    dm.coefs = coefs
+    ref = o.opd(o.modes(), coefs)
+    err = rel_l2(dm.OPD, ref)
+    print('separable OPD rel_l2', err)
+    assert err < 1e-6
+    dm.use_dense = True
+    dm.coefs = coefs
+    err = rel_l2(dm.OPD, ref)
+    print('dense OPD rel_l2', err)
+    assert err < 1e-6
+    # batched coefficients -> cube [n, n, B]
+    C = np.random.RandomState(3).normal(0, 20e-9, (357, 5))
+    dm.coefs = C
+    assert dm.OPD.shape == (80, 80, 5)
+    assert rel_l2(dm.OPD, o.opd(o.modes(), C)) < 1e-6
+
+
+def test_pyramid_init_matches_oracle():
+    tel, ngs, dm, wfs = _config1_objects()
+    o = config1_system()
+    assert wfs.resolution == o.geom.N == 320
+    assert wfs.cam.resolution == 80
+    assert wfs.nSignal == o.nSignal
+    assert np.array_equal(wfs.validI4Q, o.validI4Q)
+    assert rel_l2(wfs.referenceSignal_2D, o.ref2d) < 1e-5
+    # the final flat measurement in the constructor gives zero slopes
+    assert np.max(np.abs(wfs.signal)) < 1e-5
+
+
+def test_chain_signal_matches_oracle():
+    from oracle.dm import DMOracle
+    tel, ngs, dm, wfs = _config1_objects()
+    o = config1_system()
+    od = DMOracle(80, 8, 20, 0.35)
+    coefs = np.random.RandomState(2).normal(0, 20e-9, 357)
+    dm.coefs = coefs
+    ngs ** tel * dm * wfs
+    opd = od.opd(od.modes(), coefs)
+    s_ref = o.measure_opd(opd)[1]
+    err = rel_l2(wfs.signal, s_ref)
+    print('chain slopes rel_l2 vs fp64 oracle', err)
+    assert err <= 1e-5
+    # batched (3-D) phase: three screens through the same chain
+    C = np.random.RandomState(4).normal(0, 50e-9, (357, 3))
+    dm.coefs = C
+    ngs ** tel * dm * wfs
+    assert wfs.signal.shape == (wfs.nSignal, 3)
+    for j in range(3):
+        s_ref = o.measure_opd(od.opd(od.modes(), C[:, j]))[1]
+        assert rel_l2(wfs.signal[:, j], s_ref) <= 1e-5
