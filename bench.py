@@ -1,0 +1,197 @@
+"""Benchmark: modulated Pyramid WFS frames/s at 80x80 subapertures, 72 modulation points.
+
+Workload (one "step" = one PWFS frame): 8 m pupil sampled on 320 x 320 px, I3
+(800 nm) magnitude-8 source, Pyramid(nSubap=80, modulation=3 lambda/D,
+n_pix_separation=80, n_pix_edge=40, nTheta_user_defined=72) -> 1280^2 padded
+fields, 320^2 detector, slopesMaps signal. Input: a seeded synthetic OPD map
+(smooth random screen, fixed seed) resident in HBM before the timed region;
+output: the slope vector. The timed step runs the rows pass, the columns pass
+(+ chunk reduction) and the slopes kernel on one stream.
+
+Prints one JSON line. ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# config-2 geometry
+N_RES, N_SUBAP, SEP, EDGE, MOD, T_POINTS, DIAM, TS = 320, 80, 80, 40, 3, 72, 8.0, 1e-3
+VALU_PEAK_TFLOPS = 78.6    # FP32 vector FMA, non-packed (256 CU x 64 lanes x 2 x 2.4 GHz)
+HBM_PEAK_GBS = 8000.0
+
+
+def synthetic_opd(n, seed=10, rms_nm=150.0):
+    rs = np.random.RandomState(seed)
+    w = rs.randn(n, n)
+    k = np.fft.fftfreq(n)
+    f = np.sqrt(k[:, None] ** 2 + k[None, :] ** 2)
+    f[0, 0] = 1.0 / n
+    scr = np.real(np.fft.ifft2(np.fft.fft2(w) * f ** (-11 / 6)))
+    scr *= rms_nm * 1e-9 / scr.std()
+    return scr
+
+
+def cpu_baseline(validI4Q, opd, frames=3, warmup=1):
+    """Reference-faithful numpy restatement (complex128, scipy fft2 / numpy ifft2, thread pool of
+    6 or 8 workers over modulation points like the reference's joblib loop) on host cores."""
+    from oracle import pwfs as opw
+    ncpu = os.cpu_count() or 1
+    n_jobs = 8 if ncpu > 16 else 6
+    pup = opw.telescope_pupil(N_RES)
+    flux = opw.flux_map(pup, opw.n_photon('I3', 8), TS, DIAM)
+    o = opw.PyramidOracle(N_RES, N_SUBAP, MOD, 0.5, pup, flux, 0.8e-6, n_pix_separation=SEP,
+                          n_pix_edge=EDGE, nTheta_user_defined=T_POINTS, precision='fp64', n_jobs=n_jobs,
+                          validI4Q=validI4Q)
+    for _ in range(warmup):
+        o.measure_opd(opd)
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        o.measure_opd(opd)
+    dt = time.perf_counter() - t0
+    return dict(value=frames / dt, unit='frames/s', cores=n_jobs, kind='port',
+                sample=f'{frames} full config-2 frames ({T_POINTS} points, 2 x 1280^2 complex128 FFTs each) '
+                       f'after {warmup} warm-up, {n_jobs} threads; host reports {ncpu} CPUs')
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--warmup', type=int, default=10)
+    ap.add_argument('--cpu-frames', type=int, default=3)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--batch', type=int, default=8, help='screens per call for the batched form')
+    ap.add_argument('--calib-copy', action='store_true',
+                    help='also run one elementwise copy of 256 MiB (counter calibration for rocprof --pmc)')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1',
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from oopao_amd import Telescope, Source, Pyramid
+
+    dev = torch.device('cuda', 0)
+    torch.cuda.set_device(dev)
+    if args.calib_copy:
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)   # 256 MiB
+        y = x.mul(1.0)
+        torch.cuda.synchronize()
+        del x, y
+    tel = Telescope(N_RES, DIAM, TS, centralObstruction=0.0)
+    ngs = Source('I3', 8, display_properties=False)
+    ngs * tel
+    wfs = Pyramid(N_SUBAP, tel, MOD, 0.5, n_pix_separation=SEP, n_pix_edge=EDGE, nTheta_user_defined=T_POINTS)
+    eng, se = wfs.engine, wfs.slopes_engine
+    opd_host = synthetic_opd(N_RES) * tel.pupil
+    opd = torch.from_numpy(opd_host.astype(np.float32)).to(dev)[None].contiguous()
+    ngs ** tel
+    amp = wfs._amp()
+    scale = 2 * np.pi / ngs.wavelength
+    buf = eng.buffers(1)
+    frame = torch.empty((1, eng.cam, eng.cam), dtype=torch.float32, device=dev)
+    sig2d = torch.empty((1, 2 * se.w, se.w), dtype=torch.float32, device=dev)
+    sig = torch.empty((1, se.nsig), dtype=torch.float32, device=dev)
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record()
+        eng.rows(opd, scale, amp, buf)
+        if ev is not None:
+            ev[1].record()
+        eng.cols(buf, frame)
+        if ev is not None:
+            ev[2].record()
+        se.slopes(frame, sig2d, sig)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    K = args.steps
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
+    t0 = time.perf_counter()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for k in range(K):
+        step(evs[k])
+    end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = start.elapsed_time(end) / K
+    rows_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    cols_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    fps = 1000.0 / ms_step
+
+    # algorithmic work of the dominant kernel (columns pass + chunk reduction), per launch
+    n, N, T = N_RES, eng.N, eng.T
+    log2N = np.log2(N)
+    cols_flops = N * T * 2 * 5 * N * log2N            # one pruned forward + one inverse N-FFT per column & point
+    cols_bytes = T * n * N * 8 + buf['nchunk'] * eng.cam ** 2 * 4 * 2 + N * 8 + eng.cam ** 2 * 4
+    rows_flops = n * T * 2 * 5 * N * log2N
+    achieved_tf = cols_flops / (cols_ms * 1e-3) / 1e12
+    achieved_gbs = cols_bytes / (cols_ms * 1e-3) / 1e9
+
+    # batched form: B screens per call
+    B = args.batch
+    opdB = opd.expand(B, N_RES, N_RES).contiguous()
+    bufB = eng.buffers(B)
+    frameB = torch.empty((B, eng.cam, eng.cam), dtype=torch.float32, device=dev)
+    sig2dB = torch.empty((B, 2 * se.w, se.w), dtype=torch.float32, device=dev)
+    sigB = torch.empty((B, se.nsig), dtype=torch.float32, device=dev)
+    for _ in range(3):
+        eng.rows(opdB, scale, amp, bufB); eng.cols(bufB, frameB); se.slopes(frameB, sig2dB, sigB)
+    torch.cuda.synchronize()
+    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    KB = max(3, K // B)
+    s0.record()
+    for _ in range(KB):
+        eng.rows(opdB, scale, amp, bufB); eng.cols(bufB, frameB); se.slopes(frameB, sig2dB, sigB)
+    s1.record()
+    torch.cuda.synchronize()
+    batched_fps = B * KB * 1000.0 / s0.elapsed_time(s1)
+
+    out = {
+        'metric': 'PWFS frames/s at 80x80 subaps, 72 mod pts',
+        'value': round(fps, 2),
+        'unit': 'frames/s',
+        'n_gpus': 1,
+        'steps': K,
+        'warmup': args.warmup,
+        'ms_per_step': round(ms_step, 5),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'fp32/complex64',
+        'data': 'synthetic seeded OPD screen (150 nm RMS, seed 10), resident in HBM',
+        'config': {'workload': 'config-2 modulated PWFS: n=320, N=1280, T=72, 320x320 detector, slopesMaps',
+                   'n': N_RES, 'N': int(N), 'nTheta': int(T), 'nSubap': N_SUBAP, 'batch': 1,
+                   'nchunk': buf['nchunk'], 'pts_per_task': buf['ppt']},
+        'kernels_ms': {'rows_pass': round(rows_ms, 5), 'cols_pass_plus_reduce': round(cols_ms, 5)},
+        'roofline': {'bound': 'valu', 'kernel': 'k_cols_sep<1280,64> + k_reduce_chunks',
+                     'achieved': round(achieved_tf, 3), 'peak': VALU_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': round(achieved_tf / VALU_PEAK_TFLOPS, 4), 'traffic': None,
+                     'flops_per_launch': cols_flops, 'bytes_per_launch': cols_bytes,
+                     'hbm_achieved_GBs': round(achieved_gbs, 1),
+                     'hbm_frac': round(achieved_gbs / HBM_PEAK_GBS, 4)},
+        'batched': {'batch': B, 'value': round(batched_fps, 2), 'unit': 'frames/s'},
+        'wall_s_timed': round(wall, 4),
+    }
+    if not args.no_cpu:
+        valid = wfs.validI4Q.copy()
+        out['cpu_baseline'] = cpu_baseline(valid, opd_host, frames=args.cpu_frames)
+        out['speedup_vs_cpu'] = round(fps / out['cpu_baseline']['value'], 1)
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

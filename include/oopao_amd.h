@@ -62,6 +62,21 @@ int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const voi
                       int bin, int nchunk, int pts_per_task, void* workspace, size_t ws_bytes,
                       float* frames, void* stream);
 
+/* The two passes of oopao_pwfs_frames as separate calls (same arguments), for
+ * callers that own the intermediate buffers or time the passes:
+ *   rows pass : inter[B*T][N/4][n][4] complex64 (per pupil row y, point i:
+ *               IFFT_N(mx . FFT_N(pad(amp exp(i scale opd[y,:]) tx_i ty_i[y]))))
+ *   cols pass : per detector column, IFFT_N(my . FFT_N(pad(column))) for every
+ *               point, |.|^2 summed over the points of each of `nchunk` chunks
+ *               and binned 4x4 into part[B][nchunk][cam][cam], then reduced over
+ *               chunks into frames[B][cam][cam]. */
+int oopao_pwfs_intermediate_bytes(int n, int N, int T, int B, int nchunk, size_t* inter_bytes,
+                                  size_t* part_bytes);
+int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
+                    const void* mx, int n, int N, int T, int B, int pts_per_task, void* inter, void* stream);
+int oopao_pwfs_cols(const void* inter, const void* my, int n, int N, int T, int B, int nchunk, float* part,
+                    float* frames, void* stream);
+
 /* slopesMaps post-processing (Pyramid.py:835-854, grabQuadrant 946-961):
  *   frames [B][cam][cam] float32, valid [w][w] uint8 (validI4Q),
  *   quad[8] int32 (row origins of quadrants 1..4, then column origins),
@@ -72,6 +87,54 @@ int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const voi
 int oopao_pwfs_slopes(const float* frames, int B, int cam, const unsigned char* valid, int w,
                       const int* quad, const float* ref2d, float units, const int* vidx, int nsig,
                       int nvalid, float* sig2d, float* sig, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Deformable mirror (replaces DeformableMirror.modesComputation and the coefs
+ * setter's modes @ coefs: DeformableMirror.py:572-600, 651-686)
+ * ------------------------------------------------------------------------- */
+
+/* Gaussian influence functions modes[n*n][nact] float32 on the grid
+ * X = linspace(0,1,n)*n: exp(-(a dX^2 + 2 b dX dY + c dY^2)) * sign, with
+ * actuator centres x0, y0 [nact] float32 in pixels (DeformableMirror.py:572-600). */
+int oopao_dm_gaussian_modes(float* modes, int n, int nact, const float* x0, const float* y0, double a,
+                            double b, double c, float sign, void* stream);
+
+/* y[r] = alpha * sum_k M[r][k] x[k] + beta * y0[r] (y0 may be NULL); M row-major
+ * [R][K] float32. Used for OPD = modes @ coefs (DeformableMirror.py:674) and the
+ * reconstructor product (tutorials/AO_closed_loop_Pyramid_WFS.py:634). */
+int oopao_gemv(const float* M, const float* x, float* y, const float* y0, int R, int K, float alpha,
+               float beta, void* stream);
+
+/* Separable OPD of a Cartesian, zero-misregistration Gaussian DM for B
+ * coefficient grids C[B][nA][nA] (row = actuator row along Y):
+ * opd[B][n][n] = gy C gx^T with gx, gy [n][nA]; tmp [B][nA][n] scratch. Equal
+ * to modes @ coefs for that DM (DeformableMirror.py:572-600, 651-686). */
+int oopao_dm_separable_opd(const float* C, const float* gx, const float* gy, float* tmp, float* opd, int n,
+                           int nA, int B, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Atmosphere frozen-flow update (replaces Atmosphere.add_row / updateLayer and
+ * the footprint sum of fill_OPD_support: Atmosphere.py:410-421, 447-516,
+ * 589-622; the warp of tools/tools.py:202-209)
+ * ------------------------------------------------------------------------- */
+
+/* One whole-pixel step of a layer: tmp = map shifted by (sx, sy) in {-1,0,1}
+ * (tmp[r][c] = map[r-sy][c-sx]), z = tmp[inner_idx], X = A z + B xi,
+ * tmp[outer_idx] = X, mm = (min, max) of tmp. map/tmp [M*M], A [n_outer][n_inner],
+ * B [n_outer][n_outer], xi [n_outer] float64 (the layer's RandomState draw). */
+int oopao_atm_add_row(const double* map, double* tmp, int M, int sx, int sy, const int* inner_idx, int n_inner,
+                      const int* outer_idx, int n_outer, const double* A, const double* Bm, const double* xi,
+                      double* z, double* X, double* mm, void* stream);
+
+/* mm[0] = min(map[0..count)), mm[1] = max. */
+int oopao_atm_minmax(const double* map, int count, double* mm, void* stream);
+
+/* out[y][x] (+)= w * clip(cubic(map, row y + r0 - ty, col x + c0 - tx)) for an
+ * n x n window of the M x M map: Catmull-Rom cubic convolution, 0 outside the
+ * map, clipped to [mm[0], mm[1]] (scikit-image warp order 3, mode constant,
+ * clip=True). accumulate = 0 overwrites out. */
+int oopao_atm_layer_opd(const double* map, int M, const double* mm, double tx, double ty, int r0, int c0, double w,
+                        float* out, int n, int accumulate, void* stream);
 
 #ifdef __cplusplus
 }

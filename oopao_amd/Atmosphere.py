@@ -1,0 +1,389 @@
+"""Atmosphere: multi-layer frozen-flow von Karman turbulence (Atmosphere.py:26-1246).
+
+Initialisation follows the reference on the host in float64 (phase screens
+from the same RandomState seeds, Assemat extrusion matrices A, B from the von
+Karman covariances; Atmosphere.py:225-374, 707-810, 1017-1036). The per-step
+update runs on the GPU (oopao_amd/csrc/atm.hip): whole-pixel shifts and ring
+extrusion (add_row), the clipped cubic sub-pixel shift and the pupil-footprint
+sum of the layers. The extrusion noise xi is drawn from each layer's host
+RandomState in the reference's order, so a seeded run reproduces the
+reference's screens.
+
+Supported: on-axis sources at infinity, fov = 0, geometric propagation, no
+boiling. Other modes raise OopaoError.
+"""
+import ctypes
+
+import numpy as np
+import torch
+from numpy.random import RandomState
+
+from . import _lib
+from ._device import ptr, stream_ptr, to_host
+from .phaseStats import ft_sh_phase_screen, ft_phase_screen, makeCovarianceMatrix
+from .tools import OopaoError, warning
+
+
+class _Layer:
+    pass
+
+
+class Atmosphere:
+    def __init__(self, telescope, r0: float, L0: float, windSpeed: list, fractionalR0: list, windDirection: list,
+                 altitude: list, src=None, param=None, elevation: float = 90.0, mode: float = 2,
+                 angular_spectrum_propagation: bool = False, geometric_phase_backup: bool = False,
+                 unwrap_diffractive_phase=False, rytov_var: float = None, rytov_wvl: float = 500e-9,
+                 t_boiling=None):
+        if angular_spectrum_propagation:
+            raise OopaoError('angular-spectrum (scintillation) propagation is not supported by oopao_amd')
+        self.tag = 'atmosphere'
+        self.device = telescope.device
+        self.hasNotBeenInitialized = True
+        self.angular_spectrum_propagation = False
+        self.geometric_phase_backup = geometric_phase_backup
+        self.altitude = altitude
+        self.wavelength = 500 * 1e-9
+        self._elevation = max(5.0, float(elevation))
+        el = np.radians(self._elevation)
+        self.altitude_zenith = [h * np.sin(el) for h in self.altitude]
+        self.r0_def = 0.15
+        self.rad2arcsec = (180. / np.pi) * 3600
+        self.telescope = telescope
+        self.r0 = r0
+        self.fractionalR0 = fractionalR0
+        self.cn2 = (self.r0 ** (-5. / 3) / (0.423 * (2 * np.pi / self.wavelength) ** 2)) / np.max([1, np.max(self.altitude)])
+        self.L0 = L0
+        self.nLayer = len(fractionalR0)
+        self.windSpeed = windSpeed
+        self.windDirection = windDirection
+        if t_boiling is not None and any(v is not None for v in np.atleast_1d(t_boiling)):
+            raise OopaoError('boiling layers are not supported by oopao_amd')
+        self.t_boiling = [None] * self.nLayer
+        self.n_extra_pixel = 2
+        self.V0 = (np.sum(np.asarray(self.fractionalR0) * np.asarray(self.windSpeed) ** (5 / 3))) ** (3 / 5)
+        self.tau0 = 0.31 * self.r0 / self.V0
+        self.is_user_defined_opd = False
+        self.mode = mode
+        self.seeingArcsec = self.rad2arcsec * (self.wavelength / self.r0)
+        if src is None and self.telescope.src is None:
+            raise OopaoError('The Atmosphere object requires a Source. Either provide a Source directly as an '
+                             'attribute, or propagate the Source through the Telescope before creating the Atmosphere.')
+        self.src = src if src else self.telescope.src
+        self.src_list = [self.src] if self.src.tag == 'source' else self.src.src
+        self.asterism = None if self.src.tag == 'source' else self.src
+        self.param = param
+        self._OPD_dev = None
+
+    # ------------------------------------------------------------ init (host)
+    def initializeAtmosphere(self, telescope=None, compute_covariance=True):
+        """Atmosphere.initializeAtmosphere (Atmosphere.py:225-279)."""
+        if telescope is not None:
+            self.telescope = telescope
+        self.compute_covariance = compute_covariance
+        self.fov = self.telescope.fov
+        self.fov_rad = self.telescope.fov_rad
+        if self.fov_rad != 0:
+            raise OopaoError('a field of view (off-axis sources) is not supported by oopao_amd')
+        for s in self.src_list:
+            if s.coordinates[0] != 0 or np.isfinite(s.altitude):
+                raise OopaoError('only on-axis sources at infinity are supported by oopao_amd')
+        if self.hasNotBeenInitialized:
+            self.initial_r0 = self.r0
+            for i in range(self.nLayer):
+                print('Creation of layer' + str(i + 1) + '/' + str(self.nLayer) + ' ...')
+                lay = self.buildLayer(self.telescope, self.r0_def, self.L0, i_layer=i)
+                setattr(self, 'layer_' + str(i + 1), lay)
+        else:
+            print('Re-setting the atmosphere to its initial state...')
+            self.r0 = self.initial_r0
+        self.hasNotBeenInitialized = False
+        self.r0 = self.r0                       # recomputes B (Atmosphere.py:275, 1027-1035)
+        self.generateNewPhaseScreen(0)
+        self.relay(self.src)
+        print(self)
+
+    def buildLayer(self, telescope, r0, L0, i_layer):
+        """Atmosphere.buildLayer (Atmosphere.py:281-374), on-axis, fov = 0."""
+        tel = self.telescope
+        lay = _Layer()
+        lay.randomState = RandomState(42 + i_layer * 1000)
+        lay.altitude = self.altitude[i_layer]
+        lay.windSpeed = self.windSpeed[i_layer]
+        lay.direction = self.windDirection[i_layer]
+        lay.vY = lay.windSpeed * np.cos(np.deg2rad(lay.direction))
+        lay.vX = lay.windSpeed * np.sin(np.deg2rad(lay.direction))
+        lay.D_fov = tel.D + 2 * np.tan(self.fov_rad / 2) * lay.altitude
+        lay.resolution_fov = int(np.ceil((tel.resolution / tel.D) * lay.D_fov))
+        lay.resolution = lay.resolution_fov + 4
+        lay.center = lay.resolution // 2
+        lay.D = lay.resolution * tel.D / tel.resolution
+        lay.pixel_size = lay.D / lay.resolution
+        lay.n_extra_pixel = self.n_extra_pixel
+        lay.t_boiling = None
+        res, M = lay.resolution, lay.resolution + self.n_extra_pixel
+        lay.OPD_host = ft_sh_phase_screen(self.r0, self.L0, res, lay.D / res, seed=i_layer)
+        lay.seed = i_layer
+        outer = np.ones((M, M))
+        outer[1:-1, 1:-1] = 0
+        inner = np.ones((M, M)) - outer
+        inner[1 + self.n_extra_pixel:-1 - self.n_extra_pixel, 1 + self.n_extra_pixel:-1 - self.n_extra_pixel] = 0
+        lay.outerMask, lay.innerMask = outer, inner
+        x = np.linspace(0, res + 1, res + 2) * lay.D / (res - 1)
+        u, v = np.meshgrid(x, x)
+        lay.innerZ = u[inner != 0] + 1j * v[inner != 0]
+        lay.outerZ = u[outer != 0] + 1j * v[outer != 0]
+        # covariance matrices, shared by all layers of the same geometry (Atmosphere.py:707-777)
+        if not hasattr(self, 'ZZt') or self.ZZt.shape[0] != lay.innerZ.size:
+            print('Computing the covariance matrices ...')
+            self.ZZt = makeCovarianceMatrix(lay.innerZ, lay.innerZ, self.L0, self.r0_def)
+            self.ZZt_inv = np.linalg.pinv(self.ZZt)
+            self.ZXt = makeCovarianceMatrix(lay.innerZ, lay.outerZ, self.L0, self.r0_def)
+            self.XXt = makeCovarianceMatrix(lay.outerZ, lay.outerZ, self.L0, self.r0_def)
+        lay.ZZt, lay.ZXt, lay.XXt, lay.ZZt_inv = self.ZZt, self.ZXt, self.XXt, self.ZZt_inv
+        c = (self.r0_def / self.r0) ** (5 / 3)
+        lay.A = np.matmul((self.ZXt * c).T, self.ZZt_inv / c)
+        lay.B = np.linalg.cholesky(self.XXt * c - np.matmul(lay.A, self.ZXt * c))
+        lay.notDoneOnce = True
+        lay.buff = np.zeros(2)
+        lay.ratio = np.zeros(2)
+        # device state
+        dev = self.device
+        lay.M = M
+        lay.inner_idx = torch.from_numpy(np.flatnonzero(inner.reshape(-1) != 0).astype(np.int32)).to(dev)
+        lay.outer_idx = torch.from_numpy(np.flatnonzero(outer.reshape(-1) != 0).astype(np.int32)).to(dev)
+        lay.A_dev = torch.from_numpy(np.ascontiguousarray(lay.A)).to(dev)
+        lay.map = torch.zeros(M * M, dtype=torch.float64, device=dev)
+        lay.tmp = torch.zeros(M * M, dtype=torch.float64, device=dev)
+        lay.z = torch.zeros(lay.inner_idx.numel(), dtype=torch.float64, device=dev)
+        lay.X = torch.zeros(lay.outer_idx.numel(), dtype=torch.float64, device=dev)
+        lay.mm = torch.zeros(2, dtype=torch.float64, device=dev)
+        lay.xi = torch.zeros(lay.outer_idx.numel(), dtype=torch.float64, device=dev)
+        return lay
+
+    def _set_B(self, lay):
+        lay.B_dev = torch.from_numpy(np.ascontiguousarray(lay.B)).to(self.device)
+
+    def generateNewPhaseScreen(self, seed=None):
+        """Atmosphere.generateNewPhaseScreen (Atmosphere.py:779-810)."""
+        if seed is None:
+            import time
+            t = time.localtime()
+            seed = t.tm_hour * 3600 + t.tm_min * 60 + t.tm_sec
+        for i in range(self.nLayer):
+            lay = getattr(self, 'layer_' + str(i + 1))
+            if self.mode == 2:
+                phase = ft_sh_phase_screen(self.r0, self.L0, lay.resolution, lay.D / lay.resolution, seed=seed + i)
+            else:
+                phase = ft_phase_screen(self.r0, self.L0, lay.resolution, lay.D / lay.resolution, seed=seed + i)
+            lay.OPD_host = phase
+            lay.randomState = RandomState(seed + i * 1000)
+            Z = phase[lay.innerMask[1:-1, 1:-1] != 0]
+            X = np.matmul(lay.A, Z) + np.matmul(lay.B, lay.randomState.normal(size=lay.B.shape[1]))
+            ms = np.zeros((lay.M, lay.M))
+            ms[lay.outerMask != 0] = X
+            ms[lay.outerMask == 0] = phase.reshape(lay.resolution * lay.resolution)
+            lay.map.copy_(torch.from_numpy(ms.reshape(-1)))
+            _lib.call('oopao_atm_minmax', ptr(lay.map), lay.M * lay.M, ptr(lay.mm), stream_ptr(self.device))
+            lay.notDoneOnce = True
+            lay.buff = np.zeros(2)
+            lay.shift = (0.0, 0.0)
+        self._compute_opd()
+        if self.telescope.isPaired:
+            self.telescope.src ** self * self.telescope
+
+    # ----------------------------------------------------------- device update
+    def _add_row(self, lay, step):
+        """Atmosphere.add_row (Atmosphere.py:410-421) on the device."""
+        sx, sy = int(step[0]), int(step[1])
+        # blocking copy: the draw must reach the device before the host draws the next ring
+        lay.xi.copy_(torch.from_numpy(lay.randomState.normal(size=lay.outer_idx.numel())))
+        _lib.call('oopao_atm_add_row', ptr(lay.map), ptr(lay.tmp), lay.M, sx, sy, ptr(lay.inner_idx),
+                  lay.inner_idx.numel(), ptr(lay.outer_idx), lay.outer_idx.numel(), ptr(lay.A_dev), ptr(lay.B_dev),
+                  ptr(lay.xi), ptr(lay.z), ptr(lay.X), ptr(lay.mm), stream_ptr(self.device))
+        lay.map, lay.tmp = lay.tmp, lay.map
+
+    def updateLayer(self, lay, shift=None):
+        """Atmosphere.updateLayer (Atmosphere.py:447-516)."""
+        if lay.vX == 0 and lay.vY == 0 and shift is None:
+            return
+        if lay.notDoneOnce:
+            lay.notDoneOnce = False
+            lay.ratio = np.zeros(2)
+            lay.ratio[0] = lay.vX * self.telescope.samplingTime / lay.pixel_size
+            lay.ratio[1] = lay.vY * self.telescope.samplingTime / lay.pixel_size
+            lay.buff = np.zeros(2)
+        ratio = lay.ratio if shift is None else np.asarray(shift, np.float64)
+        tmp = np.abs(ratio)
+        tmp[np.isinf(tmp)] = 0
+        nScreens = tmp.astype('int')
+        step = np.zeros(2)
+        for _ in range(nScreens.min()):
+            step[0] = 1
+            step[1] = 1
+            step = step * np.sign(ratio)
+            self._add_row(lay, step)
+        for _ in range(nScreens.max() - nScreens.min()):
+            step[0] = 1
+            step[1] = 1
+            step = step * np.sign(ratio)
+            step[np.where(nScreens == nScreens.min())] = 0
+            self._add_row(lay, step)
+        sub = np.zeros(2)
+        sub[0] = (np.abs(ratio[0]) % 1) * np.sign(ratio[0])
+        sub[1] = (np.abs(ratio[1]) % 1) * np.sign(ratio[1])
+        lay.buff += sub
+        if np.abs(lay.buff[0]) >= 1 or np.abs(lay.buff[1]) >= 1:
+            step[0] = 1 * np.sign(lay.buff[0])
+            step[1] = 1 * np.sign(lay.buff[1])
+            step[np.where(np.abs(lay.buff) < 1)] = 0
+            self._add_row(lay, step)
+        lay.buff[0] = (np.abs(lay.buff[0]) % 1) * np.sign(lay.buff[0])
+        lay.buff[1] = (np.abs(lay.buff[1]) % 1) * np.sign(lay.buff[1])
+        lay.shift = (float(lay.buff[0]), float(lay.buff[1]))
+
+    def _compute_opd(self):
+        """Sum of the shifted, clipped, footprint-cropped layers in metres (Atmosphere.py:513-516, 589-622)."""
+        n = self.telescope.resolution
+        if self._OPD_dev is None or self._OPD_dev.shape[0] != n:
+            self._OPD_dev = torch.empty((n, n), dtype=torch.float32, device=self.device)
+        for i in range(self.nLayer):
+            lay = getattr(self, 'layer_' + str(i + 1))
+            r0 = lay.center - n // 2 + 1
+            w = self.wavelength / 2 / np.pi * np.sqrt(self.fractionalR0[i])
+            tx, ty = getattr(lay, 'shift', (0.0, 0.0))
+            _lib.call('oopao_atm_layer_opd', ptr(lay.map), lay.M, ptr(lay.mm), ctypes.c_double(tx),
+                      ctypes.c_double(ty), r0, r0, ctypes.c_double(w), ptr(self._OPD_dev), n, 1 if i else 0,
+                      stream_ptr(self.device))
+
+    def update(self, OPD=None):
+        """Atmosphere.update (Atmosphere.py:518-531)."""
+        if self.hasNotBeenInitialized:
+            raise OopaoError('The Atmosphere object needs to be initialised using the initialiseAtmosphere()')
+        if OPD is None:
+            self.is_user_defined_opd = False
+            for i in range(self.nLayer):
+                self.updateLayer(getattr(self, 'layer_' + str(i + 1)))
+            self._compute_opd()
+        else:
+            self.is_user_defined_opd = True
+            self.user_defined_opd = OPD
+        if self.telescope.isPaired:
+            self.telescope.src ** self * self.telescope
+
+    # ------------------------------------------------------------------ relay
+    def relay(self, src):
+        """Atmosphere.relay (Atmosphere.py:533-581), geometric propagation of an on-axis source."""
+        self.src = src
+        self.src_list = [src] if src.tag == 'source' else src.src
+        if self.is_user_defined_opd:
+            warning('User-Defined OPD are only propagated once in the Atmosphere class.')
+            opd = torch.as_tensor(np.asarray(self.user_defined_opd), dtype=torch.float32, device=self.device)
+            self.is_user_defined_opd = False
+        else:
+            opd = self._OPD_dev
+        n = self.telescope.resolution
+        for s in self.src_list:
+            if s.coordinates[0] > getattr(self, 'fov', 0) / 2:
+                raise OopaoError(f'Source zenith ({s.coordinates[0]}") outside of fov!')
+            s.through_atm = True
+            s.optical_path.append([self.tag, self])
+            s._scintillation_no_pupil = torch.ones((n, n), dtype=torch.float32, device=self.device)
+            m = s._mask_dev()
+            s._scintillation = s._scintillation_no_pupil if m is None else s._scintillation_no_pupil * m
+            s._refresh_intensity()
+            s._set_OPD_no_pupil_dev(opd.clone())
+
+    # -------------------------------------------------------------- properties
+    @property
+    def OPD(self):
+        return to_host(self._OPD_dev)
+
+    @property
+    def OPD_no_pupil(self):
+        return to_host(self._OPD_dev)
+
+    def layer_opd(self, i_layer):
+        """Full shifted screen of one layer in radians at 500 nm ([res, res], layer.OPD of the reference)."""
+        lay = getattr(self, 'layer_' + str(i_layer + 1))
+        out = torch.empty((lay.resolution, lay.resolution), dtype=torch.float32, device=self.device)
+        tx, ty = getattr(lay, 'shift', (0.0, 0.0))
+        _lib.call('oopao_atm_layer_opd', ptr(lay.map), lay.M, ptr(lay.mm), ctypes.c_double(tx), ctypes.c_double(ty),
+                  1, 1, ctypes.c_double(1.0), ptr(out), lay.resolution, 0, stream_ptr(self.device))
+        return out.double().cpu().numpy()
+
+    @property
+    def r0(self):
+        return self._r0
+
+    @r0.setter
+    def r0(self, val):
+        """r0 setter (Atmosphere.py:1017-1036): rescales the covariances and recomputes B."""
+        self._r0 = val
+        if self.hasNotBeenInitialized is False:
+            print('Updating the Atmosphere covariance matrices...')
+            self.seeingArcsec = self.rad2arcsec * (self.wavelength / val)
+            self.cn2 = (self.r0 ** (-5. / 3) / (0.423 * (2 * np.pi / self.wavelength) ** 2)) / np.max([1, np.max(self.altitude)])
+            c = (self.r0_def / self.r0) ** (5 / 3)
+            for i in range(self.nLayer):
+                lay = getattr(self, 'layer_' + str(i + 1))
+                BBt = lay.XXt * c - np.matmul(lay.A, lay.ZXt * c)
+                lay.B = np.linalg.cholesky(BBt)
+                self._set_B(lay)
+
+    @property
+    def windSpeed(self):
+        return self._windSpeed
+
+    @windSpeed.setter
+    def windSpeed(self, val):
+        self._windSpeed = val
+        if self.hasNotBeenInitialized is False:
+            if len(val) != self.nLayer:
+                raise OopaoError('Wrong value for the wind-speed! Make sure that you inpute a wind-speed for each layer')
+            self.V0 = (np.sum(np.asarray(self.fractionalR0) * np.asarray(self.windSpeed) ** (5 / 3))) ** (3 / 5)
+            self.tau0 = 0.31 * self.r0 / self.V0
+            for i in range(self.nLayer):
+                lay = getattr(self, 'layer_' + str(i + 1))
+                lay.windSpeed = val[i]
+                self._update_velocity(lay)
+
+    @property
+    def windDirection(self):
+        return self._windDirection
+
+    @windDirection.setter
+    def windDirection(self, val):
+        self._windDirection = val
+        if self.hasNotBeenInitialized is False:
+            if len(val) != self.nLayer:
+                raise OopaoError('Wrong value for the wind-speed! Make sure that you inpute a wind-direction for each layer')
+            for i in range(self.nLayer):
+                lay = getattr(self, 'layer_' + str(i + 1))
+                lay.direction = val[i]
+                self._update_velocity(lay)
+
+    def _update_velocity(self, lay):
+        lay.vY = lay.windSpeed * np.cos(np.deg2rad(lay.direction))
+        lay.vX = lay.windSpeed * np.sin(np.deg2rad(lay.direction))
+        lay.ratio[0] = lay.vX * self.telescope.samplingTime / lay.pixel_size
+        lay.ratio[1] = lay.vY * self.telescope.samplingTime / lay.pixel_size
+
+    # --------------------------------------------------------------- operators
+    def __mul__(self, obj):
+        if obj.tag in ('telescope', 'source', 'asterism'):
+            if obj.tag == 'telescope':
+                self.telescope = obj
+                self.relay(obj.src)
+            else:
+                self.relay(obj)
+            return obj
+        raise OopaoError('The Atmosphere can be multiplied only with a Telescope or a Source object!')
+
+    def print_properties(self):
+        print(self)
+
+    def __repr__(self):
+        s = '\n------------ Atmosphere ------------\n'
+        s += f"{'r0 @500nm [m]':<25s}|{self.r0:^9.3f}\n{'L0 [m]':<25s}|{self.L0:^9.1f}\n"
+        s += f"{'Layers':<25s}|{self.nLayer:^9d}\n"
+        return s

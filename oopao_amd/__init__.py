@@ -13,3 +13,4 @@ from .Detector import Detector  # noqa: F401
 from .MisRegistration import MisRegistration  # noqa: F401
 from .DeformableMirror import DeformableMirror  # noqa: F401
 from .Pyramid import Pyramid  # noqa: F401
+from .Atmosphere import Atmosphere  # noqa: F401

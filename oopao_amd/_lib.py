@@ -9,7 +9,7 @@ import os
 from .tools import OopaoError
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, 'liboopao_amd.so')
+LIB_PATH = os.environ.get('OOPAO_AMD_LIB', os.path.join(_HERE, 'liboopao_amd.so'))
 
 _lib = None
 
@@ -29,6 +29,17 @@ SIGNATURES = {
                                   c_void_p, c_void_p]),
     'oopao_pwfs_slopes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_float,
                                   c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    'oopao_pwfs_intermediate_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t),
+                                              ctypes.POINTER(c_size_t)]),
+    'oopao_pwfs_rows': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                                c_int, c_int, c_void_p, c_void_p]),
+    'oopao_pwfs_cols': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                c_void_p]),
+    'oopao_atm_add_row': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int,
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'oopao_atm_minmax': (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    'oopao_atm_layer_opd': (c_int, [c_void_p, c_int, c_void_p, c_double, c_double, c_int, c_int, c_double, c_void_p,
+                                    c_int, c_int, c_void_p]),
     'oopao_dm_gaussian_modes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_double, c_double, c_double,
                                         c_float, c_void_p]),
     'oopao_gemv': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_float, c_void_p]),

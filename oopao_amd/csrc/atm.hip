@@ -1,0 +1,165 @@
+// Frozen-flow atmosphere update on CDNA4 (Atmosphere.py:410-531, 589-622).
+//
+// Each layer keeps its extended screen mapShift [(res+2)^2] (float64) on the
+// device. Per loop step the reference:
+//  * when the accumulated sub-pixel shift crosses a pixel, shifts mapShift by
+//    one pixel (cubic warp with an integer translation = exact copy) and
+//    extrudes the new outer ring X = A Z + B xi from the inner ring Z
+//    (Assemat et al. 2006), add_row, Atmosphere.py:410-421;
+//  * shifts mapShift by the residual sub-pixel amount with Catmull-Rom cubic
+//    convolution, constant (0) boundary, result clipped to [min, max] of
+//    mapShift (scikit-image warp order 3), keeps [1:-1, 1:-1]
+//    (Atmosphere.py:513-516);
+//  * crops the pupil footprint and sums the layers scaled by
+//    lambda_500 / 2 pi * sqrt(fractionalR0) (fill_OPD_support, 589-622).
+// Kernels here implement those pieces; the random draws xi come from the
+// layer's host RandomState (same stream as the reference).
+#include "common.hpp"
+
+namespace oopao {
+
+// dst[r][c] = src[r - sy][c - sx] (0 outside), M x M maps, |sx|,|sy| <= 1 integers.
+__global__ void k_shift_int(const double* __restrict__ src, double* __restrict__ dst, int M, int sx, int sy) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= M * M) return;
+  const int r = idx / M, c = idx % M;
+  const int rs = r - sy, cs = c - sx;
+  dst[idx] = (rs >= 0 && rs < M && cs >= 0 && cs < M) ? src[rs * M + cs] : 0.0;
+}
+
+__global__ void k_gather(const double* __restrict__ map, const int* __restrict__ idx, double* __restrict__ out,
+                         int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = map[idx[k]];
+}
+
+__global__ void k_scatter(double* __restrict__ map, const int* __restrict__ idx, const double* __restrict__ v,
+                          int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) map[idx[k]] = v[k];
+}
+
+// X[r] = sum_k A[r][k] z[k] + sum_k B[r][k] xi[k]  (float64, one wavefront per row)
+__global__ void __launch_bounds__(256) k_extrude(const double* __restrict__ A, const double* __restrict__ z,
+                                                 const double* __restrict__ Bm, const double* __restrict__ xi,
+                                                 double* __restrict__ X, int R, int KA, int KB) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + wave;
+  if (r >= R) return;
+  const double* a = A + (size_t)r * KA;
+  const double* b = Bm + (size_t)r * KB;
+  double acc = 0.0;
+  for (int k = lane; k < KA; k += 64) acc = fma(a[k], z[k], acc);
+  for (int k = lane; k < KB; k += 64) acc = fma(b[k], xi[k], acc);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) X[r] = acc;
+}
+
+// mm[0] = min(map), mm[1] = max(map); one workgroup.
+__global__ void __launch_bounds__(1024) k_minmax(const double* __restrict__ map, int n, double* __restrict__ mm) {
+  __shared__ double lo[1024], hi[1024];
+  double a = INFINITY, b = -INFINITY;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = map[i];
+    a = fmin(a, v);
+    b = fmax(b, v);
+  }
+  lo[threadIdx.x] = a;
+  hi[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + s]);
+      hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    mm[0] = lo[0];
+    mm[1] = hi[0];
+  }
+}
+
+__device__ __forceinline__ double cubic(double x, double f0, double f1, double f2, double f3) {
+  return f1 + 0.5 * x * (f2 - f0 + x * (2.0 * f0 - 5.0 * f1 + 4.0 * f2 - f3 + x * (3.0 * (f1 - f2) + f3 - f0)));
+}
+
+// out[y][x] (+)= w * clip(cubic(map, r = y + r0 - ty, c = x + c0 - tx)), n x n output,
+// map M x M, clip range mm[0..1] with the constant-mode cval (0) preserved when
+// 0 lies outside [min, max].
+__global__ void k_layer_opd(const double* __restrict__ map, int M, const double* __restrict__ mm, double tx,
+                            double ty, int r0, int c0, double w, float* __restrict__ out, int n, int accumulate) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int y = idx / n, x = idx % n;
+  const double rr = (double)(y + r0) - ty;
+  const double cc = (double)(x + c0) - tx;
+  const double fr = floor(rr), fc = floor(cc);
+  const int ir = (int)fr, ic = (int)fc;
+  const double xr = rr - fr, xc = cc - fc;
+  double rows[4];
+#pragma unroll
+  for (int pr = 0; pr < 4; ++pr) {
+    const int r = ir - 1 + pr;
+    double f[4];
+#pragma unroll
+    for (int pc = 0; pc < 4; ++pc) {
+      const int c = ic - 1 + pc;
+      f[pc] = (r >= 0 && r < M && c >= 0 && c < M) ? map[r * M + c] : 0.0;
+    }
+    rows[pr] = cubic(xc, f[0], f[1], f[2], f[3]);
+  }
+  double v = cubic(xr, rows[0], rows[1], rows[2], rows[3]);
+  const double lo = mm[0], hi = mm[1];
+  const bool preserve = !(lo <= 0.0 && 0.0 <= hi);
+  if (!(preserve && v == 0.0)) v = fmin(fmax(v, lo), hi);
+  const float contrib = (float)(v * w);
+  out[idx] = accumulate ? out[idx] + contrib : contrib;
+}
+
+}  // namespace oopao
+
+using namespace oopao;
+
+extern "C" {
+
+int oopao_atm_add_row(const double* map, double* tmp, int M, int sx, int sy, const int* inner_idx, int n_inner,
+                      const int* outer_idx, int n_outer, const double* A, const double* Bm, const double* xi,
+                      double* z, double* X, double* mm, void* stream) {
+  OOPAO_CHECK_ARG(map && tmp && inner_idx && outer_idx && A && Bm && xi && z && X && mm, "null pointer argument");
+  OOPAO_CHECK_ARG(M > 2 && n_inner > 0 && n_outer > 0, "invalid sizes");
+  OOPAO_CHECK_ARG(sx >= -1 && sx <= 1 && sy >= -1 && sy <= 1, "integer shift must be -1, 0 or 1");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_shift_int, dim3((M * M + 255) / 256), dim3(256), 0, st, map, tmp, M, sx, sy);
+  OOPAO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_gather, dim3((n_inner + 255) / 256), dim3(256), 0, st, tmp, inner_idx, z, n_inner);
+  OOPAO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_extrude, dim3((n_outer + 3) / 4), dim3(256), 0, st, A, z, Bm, xi, X, n_outer, n_inner,
+                     n_outer);
+  OOPAO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scatter, dim3((n_outer + 255) / 256), dim3(256), 0, st, tmp, outer_idx, X, n_outer);
+  OOPAO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_minmax, dim3(1), dim3(1024), 0, st, tmp, M * M, mm);
+  OOPAO_LAUNCH_CHECK();
+  return OK;
+}
+
+int oopao_atm_minmax(const double* map, int count, double* mm, void* stream) {
+  OOPAO_CHECK_ARG(map && mm && count > 0, "invalid arguments");
+  hipLaunchKernelGGL(k_minmax, dim3(1), dim3(1024), 0, reinterpret_cast<hipStream_t>(stream), map, count, mm);
+  OOPAO_LAUNCH_CHECK();
+  return OK;
+}
+
+int oopao_atm_layer_opd(const double* map, int M, const double* mm, double tx, double ty, int r0, int c0, double w,
+                        float* out, int n, int accumulate, void* stream) {
+  OOPAO_CHECK_ARG(map && mm && out, "null pointer argument");
+  OOPAO_CHECK_ARG(n > 0 && M > 0, "invalid sizes");
+  hipLaunchKernelGGL(k_layer_opd, dim3((n * n + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                     map, M, mm, tx, ty, r0, c0, w, out, n, accumulate);
+  OOPAO_LAUNCH_CHECK();
+  return OK;
+}
+
+}  // extern "C"

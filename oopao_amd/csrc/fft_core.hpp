@@ -160,18 +160,31 @@ __host__ __device__ constexpr int plan_stages(int rem, int E) {
   return rem == 1 ? 0 : 1 + plan_stages(rem / pick_radix(rem, E), E);
 }
 
-// Exchange-buffer index padding: one float2 every 32 breaks the power-of-two
-// strides of the early Stockham stages across LDS banks.
+// Optional exchange-buffer index padding (one float2 every 32). Off by default:
+// on gfx950 the unpadded layout lets every LDS address fold into the
+// instruction's immediate offset, which saved registers (occupancy 2 -> 4
+// waves/SIMD for the 1280-point columns pass) and ran 10-30% faster.
+#ifndef OOPAO_LDS_PAD
+#define OOPAO_LDS_PAD 0
+#endif
+#if OOPAO_LDS_PAD
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
 __host__ __device__ constexpr int lds_len(int N) { return N + N / 32 + 1; }
+#else
+__device__ __forceinline__ int lds_pad(int i) { return i; }
+__host__ __device__ constexpr int lds_len(int N) { return N; }
+#endif
 
 // Opaque copy of a read-only table pointer. Loads through the returned pointer
 // cannot be hoisted above this point, which keeps loop-invariant twiddle and
 // mask loads next to their use instead of pinning registers across loops.
+// (The opaque value is an integer offset, so the pointer keeps its global
+// address space and the loads stay global_load rather than flat_load.)
 template <typename T>
 __device__ __forceinline__ const T* launder(const T* p) {
-  asm volatile("" : "+s"(p)::"memory");
-  return p;
+  int zero = 0;
+  asm volatile("" : "+s"(zero)::"memory");
+  return p + zero;
 }
 
 // One Stockham stage (radix R, already-sorted span NS) on the strided layout.

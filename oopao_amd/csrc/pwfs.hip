@@ -24,6 +24,10 @@
 #include "common.hpp"
 #include "fft_core.hpp"
 
+#ifndef OOPAO_TPT_1280
+#define OOPAO_TPT_1280 64
+#endif
+
 namespace oopao {
 
 // Lane count per transform for each supported padded length N (fast path: the
@@ -31,21 +35,34 @@ namespace oopao {
 // radix divides N/TPT and the pruned forward + inverse use few stages.
 __host__ __device__ constexpr int fast_tpt(int N) {
   return N == 64 ? 16 : N == 128 ? 16 : N == 256 ? 32 : N == 320 ? 16 : N == 512 ? 32
-       : N == 640 ? 16 : N == 1024 ? 64 : N == 1280 ? 64 : N == 1536 ? 64 : N == 2048 ? 64 : 0;
+       : N == 640 ? 16 : N == 1024 ? 64 : N == 1280 ? OOPAO_TPT_1280 : N == 1536 ? 64 : N == 2048 ? 64 : 0;
 }
 
 #define OOPAO_FAST_SIZES(X) X(64) X(128) X(256) X(320) X(512) X(640) X(1024) X(1280) X(1536) X(2048)
 
 constexpr int kWaves = 4;  // wavefronts per workgroup
 
+// Pupil-plane field u[b][y][x] = amp[y][x] exp(i scale opd[b][y][x]), once per
+// screen (the reference re-forms it per modulation point, Pyramid.py:619-620).
+__global__ void k_field(const float* __restrict__ opd, float scale, const float* __restrict__ amp,
+                        float2* __restrict__ u, int n, int B) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)B * n * n;
+  if (idx >= total) return;
+  const size_t p = idx % ((size_t)n * n);
+  float s, c;
+  sincosf(opd[idx] * scale, &s, &c);
+  const float a = amp[p];
+  u[idx] = make_float2(a * c, a * s);
+}
+
 // ---------------------------------------------------------------------------
-// Rows pass. One lane group (TPT lanes) per pupil row y; the row's complex
-// amplitude u = amp exp(i scale opd) is built once and reused for a chunk of
-// modulation points.
+// Rows pass. One lane group (TPT lanes) per pupil row y; the row's field is
+// loaded once and reused for a chunk of modulation points.
 // ---------------------------------------------------------------------------
 template <int N, int TPT>
 __global__ void __launch_bounds__(64 * kWaves)
-k_rows_sep(const float* __restrict__ opd, float scale, const float* __restrict__ amp,
+k_rows_sep(const float2* __restrict__ field,
            const float2* __restrict__ tx, const float2* __restrict__ ty,
            const float2* __restrict__ mx, const float2* __restrict__ tw,
            float2* __restrict__ ws, int B, int T, int pts_per_task) {
@@ -71,16 +88,9 @@ k_rows_sep(const float* __restrict__ opd, float scale, const float* __restrict__
   float2* xl = lds[wave * G + h];
 
   float2 u[E];
-  const float* op = opd + ((size_t)b * n + y) * n;
-  const float* ap = amp + (size_t)y * n;
+  const float2* fp = field + ((size_t)b * n + y) * n;
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int x = (t + TPT * m) >> 2;
-    float s, c;
-    sincosf(op[x] * scale, &s, &c);
-    const float a = ap[x];
-    u[m] = make_float2(a * c, a * s);
-  }
+  for (int m = 0; m < E; ++m) u[m] = fp[(t + TPT * m) >> 2];
   const int i0 = pc * pts_per_task;
   const int i1 = min(T, i0 + pts_per_task);
   for (int i = i0; i < i1; ++i) {
@@ -187,14 +197,14 @@ __global__ void k_reduce_chunks(const float* __restrict__ part, float* __restric
 // slopesMaps post-processing (Pyramid.py:838-854), one workgroup per frame.
 // quad[0..3] = row origins, quad[4..7] = column origins of quadrants 1..4.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_slopes(const float* __restrict__ frames, int cam, const unsigned char* __restrict__ valid, int w,
          const int* __restrict__ quad, const float* __restrict__ ref2d, float units,
          const int* __restrict__ vidx, int nsig, int nvalid, float* __restrict__ sig2d,
          float* __restrict__ sig) {
   const int b = blockIdx.x;
   const float* f = frames + (size_t)b * cam * cam;
-  __shared__ double red[256];
+  __shared__ double red[1024];
   __shared__ float norma_s;
   const int r1 = quad[0], r2 = quad[1], r3 = quad[2], r4 = quad[3];
   const int c1 = quad[4], c2 = quad[5], c3 = quad[6], c4 = quad[7];
@@ -243,46 +253,58 @@ k_slopes(const float* __restrict__ frames, int cam, const unsigned char* __restr
 // ---------------------------------------------------------------------------
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// intermediate [B*T][N/4][n][4] followed by the pupil field [B][n][n]
 static size_t ws_inter_bytes(int n, int N, int T, int B) {
-  return align256((size_t)B * T * (size_t)N * n * sizeof(float2));
+  return align256((size_t)B * T * (size_t)N * n * sizeof(float2) + (size_t)B * n * n * sizeof(float2));
 }
 static size_t ws_part_bytes(int N, int B, int nchunk) {
   return align256((size_t)B * nchunk * (size_t)(N / 4) * (N / 4) * sizeof(float));
 }
 
 template <int N>
-static int launch_fast(const float* opd, float scale, const float* amp, const float2* tx,
-                       const float2* ty, const float2* mx, const float2* my, int T, int B,
-                       int nchunk, int pts_per_task, char* ws, float* frames, hipStream_t st) {
+static int launch_rows(const float* opd, float scale, const float* amp, const float2* tx, const float2* ty,
+                       const float2* mx, int T, int B, int pts_per_task, float2* inter, hipStream_t st) {
   constexpr int TPT = fast_tpt(N);
   constexpr int n = N / 4;
   constexpr int G = 64 / TPT;
   const float2* tw = nullptr;
   int rc = twiddles(N, &tw);
   if (rc) return rc;
-  float2* inter = reinterpret_cast<float2*>(ws);
-  float* part = reinterpret_cast<float*>(ws + ws_inter_bytes(n, N, T, B));
+  constexpr int RPB = kWaves * G;
+  const int nrb = (n + RPB - 1) / RPB;
+  const int npc = (T + pts_per_task - 1) / pts_per_task;
+  float2* field = inter + (size_t)B * T * (size_t)N * n;
   {
-    constexpr int RPB = kWaves * G;
-    const int nrb = (n + RPB - 1) / RPB;
-    const int npc = (T + pts_per_task - 1) / pts_per_task;
-    dim3 grid((unsigned)(nrb * npc * B));
-    hipLaunchKernelGGL((k_rows_sep<N, TPT>), grid, dim3(64 * kWaves), 0, st, opd, scale, amp, tx, ty,
-                       mx, tw, inter, B, T, pts_per_task);
+    const size_t total = (size_t)B * n * n;
+    hipLaunchKernelGGL(k_field, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, opd, scale, amp, field,
+                       n, B);
     OOPAO_LAUNCH_CHECK();
   }
+  dim3 grid((unsigned)(nrb * npc * B));
+  hipLaunchKernelGGL((k_rows_sep<N, TPT>), grid, dim3(64 * kWaves), 0, st, field, tx, ty, mx, tw,
+                     inter, B, T, pts_per_task);
+  OOPAO_LAUNCH_CHECK();
+  return OK;
+}
+
+template <int N>
+static int launch_cols(const float2* inter, const float2* my, int T, int B, int nchunk, float* part,
+                       float* frames, hipStream_t st) {
+  constexpr int TPT = fast_tpt(N);
+  const float2* tw = nullptr;
+  int rc = twiddles(N, &tw);
+  if (rc) return rc;
   {
     const long tasks = (long)B * nchunk * (N / 4);
     dim3 grid((unsigned)((tasks + kWaves - 1) / kWaves));
-    hipLaunchKernelGGL((k_cols_sep<N, TPT>), grid, dim3(64 * kWaves), 0, st, inter, my, tw, part, B,
-                       T, nchunk);
+    hipLaunchKernelGGL((k_cols_sep<N, TPT>), grid, dim3(64 * kWaves), 0, st, inter, my, tw, part, B, T, nchunk);
     OOPAO_LAUNCH_CHECK();
   }
   {
     const int cam = N / 4;
     const size_t total = (size_t)B * cam * cam;
-    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
-                       part, frames, B, nchunk, cam);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, part, frames,
+                       B, nchunk, cam);
     OOPAO_LAUNCH_CHECK();
   }
   return OK;
@@ -308,33 +330,40 @@ int oopao_pwfs_workspace_bytes(int n, int N, int T, int B, int nchunk, size_t* b
 
 int oopao_pwfs_supported(int n, int N, int bin) { return fast_supported(n, N, bin) ? 1 : 0; }
 
-int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
-                      const void* mx, const void* my, int n, int N, int T, int B, int bin, int nchunk,
-                      int pts_per_task, void* workspace, size_t ws_bytes, float* frames,
-                      void* stream) {
-  OOPAO_CHECK_ARG(opd && amp && tx && ty && mx && my && workspace && frames, "null pointer argument");
-  OOPAO_CHECK_ARG(T > 0 && B > 0 && nchunk > 0 && nchunk <= T && pts_per_task > 0,
-                  "invalid T=%d B=%d nchunk=%d pts_per_task=%d", T, B, nchunk, pts_per_task);
+static int check_geometry(int n, int N, int bin) {
   if (!fast_supported(n, N, bin)) {
-    set_error("unsupported PWFS geometry n=%d N=%d bin=%d (need N = 4 n, 4x4 binning, N in the "
-              "compiled set)", n, N, bin);
+    set_error("unsupported PWFS geometry n=%d N=%d bin=%d (need N = 4 n, 4x4 binning, N in the compiled set)",
+              n, N, bin);
     return ERR_UNSUPPORTED;
   }
-  size_t need = ws_inter_bytes(n, N, T, B) + ws_part_bytes(N, B, nchunk);
-  if (ws_bytes < need) {
-    set_error("workspace too small: %zu < %zu bytes", ws_bytes, need);
-    return ERR_WORKSPACE;
-  }
+  return OK;
+}
+
+int oopao_pwfs_intermediate_bytes(int n, int N, int T, int B, int nchunk, size_t* inter_bytes,
+                                  size_t* part_bytes) {
+  OOPAO_CHECK_ARG(inter_bytes && part_bytes, "null pointer argument");
+  OOPAO_CHECK_ARG(n > 0 && N > 0 && T > 0 && B > 0 && nchunk > 0 && nchunk <= T,
+                  "invalid sizes n=%d N=%d T=%d B=%d nchunk=%d", n, N, T, B, nchunk);
+  *inter_bytes = ws_inter_bytes(n, N, T, B);
+  *part_bytes = ws_part_bytes(N, B, nchunk);
+  return OK;
+}
+
+int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
+                    const void* mx, int n, int N, int T, int B, int pts_per_task, void* inter, void* stream) {
+  OOPAO_CHECK_ARG(opd && amp && tx && ty && mx && inter, "null pointer argument");
+  OOPAO_CHECK_ARG(T > 0 && B > 0 && pts_per_task > 0, "invalid T=%d B=%d pts_per_task=%d", T, B, pts_per_task);
+  int rc = check_geometry(n, N, 4);
+  if (rc) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const float2* txp = reinterpret_cast<const float2*>(tx);
   const float2* typ = reinterpret_cast<const float2*>(ty);
   const float2* mxp = reinterpret_cast<const float2*>(mx);
-  const float2* myp = reinterpret_cast<const float2*>(my);
-  char* w = reinterpret_cast<char*>(workspace);
+  float2* ip = reinterpret_cast<float2*>(inter);
   switch (N) {
 #define OOPAO_CASE(NN) \
   case NN:             \
-    return launch_fast<NN>(opd, scale, amp, txp, typ, mxp, myp, T, B, nchunk, pts_per_task, w, frames, st);
+    return launch_rows<NN>(opd, scale, amp, txp, typ, mxp, T, B, pts_per_task, ip, st);
     OOPAO_FAST_SIZES(OOPAO_CASE)
 #undef OOPAO_CASE
     default:
@@ -344,6 +373,49 @@ int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const voi
   return ERR_UNSUPPORTED;
 }
 
+int oopao_pwfs_cols(const void* inter, const void* my, int n, int N, int T, int B, int nchunk, float* part,
+                    float* frames, void* stream) {
+  OOPAO_CHECK_ARG(inter && my && part && frames, "null pointer argument");
+  OOPAO_CHECK_ARG(T > 0 && B > 0 && nchunk > 0 && nchunk <= T, "invalid T=%d B=%d nchunk=%d", T, B, nchunk);
+  int rc = check_geometry(n, N, 4);
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const float2* ip = reinterpret_cast<const float2*>(inter);
+  const float2* myp = reinterpret_cast<const float2*>(my);
+  switch (N) {
+#define OOPAO_CASE(NN) \
+  case NN:             \
+    return launch_cols<NN>(ip, myp, T, B, nchunk, part, frames, st);
+    OOPAO_FAST_SIZES(OOPAO_CASE)
+#undef OOPAO_CASE
+    default:
+      break;
+  }
+  set_error("unsupported padded length N=%d", N);
+  return ERR_UNSUPPORTED;
+}
+
+int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
+                      const void* mx, const void* my, int n, int N, int T, int B, int bin, int nchunk,
+                      int pts_per_task, void* workspace, size_t ws_bytes, float* frames, void* stream) {
+  OOPAO_CHECK_ARG(opd && amp && tx && ty && mx && my && workspace && frames, "null pointer argument");
+  OOPAO_CHECK_ARG(T > 0 && B > 0 && nchunk > 0 && nchunk <= T && pts_per_task > 0,
+                  "invalid T=%d B=%d nchunk=%d pts_per_task=%d", T, B, nchunk, pts_per_task);
+  int rc = check_geometry(n, N, bin);
+  if (rc) return rc;
+  size_t need = ws_inter_bytes(n, N, T, B) + ws_part_bytes(N, B, nchunk);
+  if (ws_bytes < need) {
+    set_error("workspace too small: %zu < %zu bytes", ws_bytes, need);
+    return ERR_WORKSPACE;
+  }
+  char* w = reinterpret_cast<char*>(workspace);
+  void* inter = w;
+  float* part = reinterpret_cast<float*>(w + ws_inter_bytes(n, N, T, B));
+  rc = oopao_pwfs_rows(opd, scale, amp, tx, ty, mx, n, N, T, B, pts_per_task, inter, stream);
+  if (rc) return rc;
+  return oopao_pwfs_cols(inter, my, n, N, T, B, nchunk, part, frames, stream);
+}
+
 int oopao_pwfs_slopes(const float* frames, int B, int cam, const unsigned char* valid, int w,
                       const int* quad, const float* ref2d, float units, const int* vidx, int nsig,
                       int nvalid, float* sig2d, float* sig, void* stream) {
@@ -351,7 +423,7 @@ int oopao_pwfs_slopes(const float* frames, int B, int cam, const unsigned char* 
   OOPAO_CHECK_ARG(B > 0 && cam > 0 && w > 0 && nsig >= 0 && nvalid > 0,
                   "invalid sizes B=%d cam=%d w=%d nsig=%d nvalid=%d", B, cam, w, nsig, nvalid);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_slopes, dim3(B), dim3(256), 0, st, frames, cam, valid, w, quad, ref2d, units,
+  hipLaunchKernelGGL(k_slopes, dim3(B), dim3(1024), 0, st, frames, cam, valid, w, quad, ref2d, units,
                      vidx, nsig, nvalid, sig2d, sig);
   OOPAO_LAUNCH_CHECK();
   return OK;

@@ -1,0 +1,92 @@
+"""Von Karman phase screens and covariances (phaseStats.py:19-271), host-side init code.
+
+Phase screens are drawn from numpy's legacy RandomState exactly as the
+reference does (same seeds, same draw order), so a seeded screen is the
+reference's screen up to FFT rounding. Used at initialisation only; the
+per-frame update runs on the GPU (oopao_amd/csrc/atm.hip).
+"""
+import math
+
+import numpy as np
+import scipy.special
+from numpy.random import RandomState
+
+
+def gamma(x):
+    return math.gamma(x)
+
+
+def covariance_constants(L0, r0):
+    """(cst, var) of the von Karman phase covariance (phaseStats.py:39-58, 72-80)."""
+    L0r0 = (L0 / r0) ** (5. / 3)
+    cst = (24. * gamma(6. / 5) / 5) ** (5. / 6) * (gamma(11. / 6) / ((2. ** (5. / 6)) * np.pi ** (8. / 3))) * L0r0
+    var = (24. * gamma(6. / 5) / 5) ** (5. / 6) * (gamma(11. / 6) * gamma(5. / 6) / (2 * np.pi ** (8. / 3))) * L0r0
+    return cst, var
+
+
+def makeCovarianceMatrix(rho1, rho2, L0, r0_def):
+    """Phase covariance between complex coordinate sets rho1, rho2 (phaseStats.py:72-132)."""
+    rho = np.abs(rho1[:, None] - rho2[None, :])
+    cst, var = covariance_constants(L0, r0_def)
+    out = np.full(rho.shape, var)
+    index = np.where(rho != 0)
+    u = 2 * np.pi * rho[index] / L0
+    out[index] = cst * u ** (5. / 6) * scipy.special.kv(5. / 6, u)
+    return out
+
+
+def ift2(G):
+    """fftshift(fft2(fftshift(G))) (phaseStats.py:135-150)."""
+    return np.fft.fftshift(np.fft.fft2(np.fft.fftshift(G)))
+
+
+def ft_phase_screen(r0, L0, N, delta, l0=1e-10, seed=None):
+    """FFT von Karman screen in radians at 500 nm (phaseStats.py:153-198)."""
+    delta = float(delta)
+    if seed is None:
+        import random
+        import time
+        seed = int(random.SystemRandom(time.time()).random() * 100000)
+    rs = RandomState(seed)
+    del_f = 1. / (N * delta)
+    fx = np.arange(-N / 2., N / 2.) * del_f
+    fx, fy = np.meshgrid(fx, fx)
+    f = np.sqrt(fx ** 2 + fy ** 2)
+    fm = 5.92 / l0 / (2 * np.pi)
+    f0 = 1. / L0
+    psd = 0.023 * r0 ** (-5. / 3.) * np.exp(-1 * ((f / fm) ** 2)) / (((f ** 2) + (f0 ** 2)) ** (11. / 6))
+    psd[int(N / 2), int(N / 2)] = 0
+    cn = (rs.normal(size=(N, N)) + 1j * rs.normal(size=(N, N))) * np.sqrt(psd) * del_f
+    return ift2(cn).real
+
+
+def ft_sh_phase_screen(r0, L0, resolution, pixel_size, l0=1e-10, seed=None):
+    """FFT screen plus 3 levels of 2 x 2 subharmonics (phaseStats.py:201-271)."""
+    pixel_size = float(pixel_size)
+    if seed is None:
+        import random
+        import time
+        seed = int(random.SystemRandom(time.time()).random() * 100000)
+    rs = RandomState(seed)
+    D = resolution * pixel_size
+    phs_hi = ft_phase_screen(r0, L0, resolution, pixel_size, seed=seed)
+    coords = np.arange(-resolution / 2, resolution / 2) * pixel_size
+    x, y = np.meshgrid(coords, coords)
+    phs_lo = np.zeros(phs_hi.shape)
+    for p in range(1, 4):
+        del_f = 1 / (3 ** p * D)
+        fx = np.arange(-1, 2) * del_f
+        fx, fy = np.meshgrid(fx, fx)
+        f = np.sqrt(fx ** 2 + fy ** 2)
+        fm = 5.92 / l0 / (2 * np.pi)
+        f0 = 1. / L0
+        psd = 0.023 * r0 ** (-5. / 3) * np.exp(-1 * (f / fm) ** 2) / ((f ** 2 + f0 ** 2) ** (11. / 6))
+        psd[1, 1] = 0
+        cn = (rs.normal(size=(3, 3)) + 1j * rs.normal(size=(3, 3))) * np.sqrt(psd) * del_f
+        SH = np.zeros((resolution, resolution), dtype='complex')
+        for i in range(0, 2):
+            for j in range(0, 2):
+                SH += cn[i, j] * np.exp(1j * 2 * np.pi * (fx[i, j] * x + fy[i, j] * y))
+        phs_lo = phs_lo + SH
+    phs_lo = phs_lo.real - phs_lo.real.mean()
+    return phs_lo + phs_hi

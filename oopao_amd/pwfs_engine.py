@@ -97,8 +97,10 @@ class PWFSEngine:
 
     # -------------------------------------------------------------- execution
     def _knobs(self, B):
-        nchunk = self.nchunk_user or max(1, min(self.T, 8))
-        ppt = self.pts_per_task_user or max(1, min(self.T, 8))
+        """Point-chunk sizes: 6 points per columns-pass chunk and per rows-pass task (measured best at
+        n=320, N=1280, T=72; any value gives the same frames up to float summation order)."""
+        nchunk = self.nchunk_user or max(1, -(-self.T // 6))
+        ppt = self.pts_per_task_user or max(1, min(self.T, 6))
         return min(nchunk, self.T), ppt
 
     def _workspace(self, B, nchunk):
@@ -126,6 +128,26 @@ class PWFSEngine:
                   _ptr(self.ty), _ptr(self.mx), _ptr(self.my), self.n, self.N, self.T, B, self.bin,
                   nchunk, ppt, _ptr(ws), ctypes.c_size_t(nb), _ptr(out), _stream_ptr(self.device))
         return out
+
+
+    # split passes (same result as frames(); used to time the passes separately)
+    def buffers(self, B):
+        nchunk, ppt = self._knobs(B)
+        ib, pb = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        _lib.call('oopao_pwfs_intermediate_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(ib),
+                  ctypes.byref(pb))
+        inter = torch.empty(ib.value, dtype=torch.uint8, device=self.device)
+        part = torch.empty(pb.value // 4, dtype=torch.float32, device=self.device)
+        return dict(B=B, nchunk=nchunk, ppt=ppt, inter=inter, part=part)
+
+    def rows(self, opd, scale, amp, buf):
+        _lib.call('oopao_pwfs_rows', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx), _ptr(self.ty),
+                  _ptr(self.mx), self.n, self.N, self.T, buf['B'], buf['ppt'], _ptr(buf['inter']),
+                  _stream_ptr(self.device))
+
+    def cols(self, buf, out):
+        _lib.call('oopao_pwfs_cols', _ptr(buf['inter']), _ptr(self.my), self.n, self.N, self.T, buf['B'],
+                  buf['nchunk'], _ptr(buf['part']), _ptr(out), _stream_ptr(self.device))
 
 
 class SlopesEngine:

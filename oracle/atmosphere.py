@@ -1,0 +1,125 @@
+"""numpy restatement of OOPAO's frozen-flow atmosphere update (CPU oracle).
+
+Test infrastructure only (see ``oracle/__init__.py``).
+
+* warp_shift: the sub-pixel translation the reference delegates to
+  scikit-image (tools/tools.py:202-209: warp(image, SimilarityTransform(
+  translation=shift).inverse, order=3), mode 'constant', cval 0, clip=True),
+  restated as Catmull-Rom cubic convolution (Keys, 1981, a = -1/2) on the 4x4
+  neighbourhood with out-of-bounds samples equal to cval, followed by the
+  clip of the result to [min, max] of the input. Pinned by
+  tests/golden/warp_skimage.npz (outputs of scikit-image 0.18.3 itself).
+* LayerOracle: updateLayer / add_row (Atmosphere.py:410-421, 447-516) and the
+  pupil-footprint crop of fill_OPD_support (589-622) for an on-axis source at
+  infinity, in float64, consuming the layer's RandomState exactly like the
+  reference (one normal(size=nOuter) draw per extruded ring).
+"""
+import numpy as np
+
+
+def _cubic(x, f0, f1, f2, f3):
+    return f1 + 0.5 * x * (f2 - f0 + x * (2.0 * f0 - 5.0 * f1 + 4.0 * f2 - f3 + x * (3.0 * (f1 - f2) + f3 - f0)))
+
+
+def warp_shift(img, tx, ty, cval=0.0, clip=True):
+    """out[r, c] = cubic interpolation of img at (r - ty, c - tx)."""
+    img = np.asarray(img, dtype=np.float64)
+    R, C = img.shape
+    rr, cc = np.meshgrid(np.arange(R, dtype=np.float64) - ty, np.arange(C, dtype=np.float64) - tx, indexing='ij')
+    r0 = np.floor(rr).astype(np.int64)
+    c0 = np.floor(cc).astype(np.int64)
+    xr = rr - r0
+    xc = cc - c0
+
+    def px(r, c):
+        ok = (r >= 0) & (r < R) & (c >= 0) & (c < C)
+        v = np.full(r.shape, cval, dtype=np.float64)
+        v[ok] = img[r[ok], c[ok]]
+        return v
+
+    rows = []
+    for pr in range(-1, 3):
+        f = [px(r0 + pr, c0 + pc) for pc in range(-1, 3)]
+        rows.append(_cubic(xc, *f))
+    out = _cubic(xr, *rows)
+    if clip:
+        lo, hi = img.min(), img.max()
+        preserve = not (lo <= cval <= hi)
+        if preserve:
+            mask = out == cval
+        out = np.clip(out, lo, hi)
+        if preserve:
+            out[mask] = cval
+    return out
+
+
+class LayerOracle:
+    """One frozen-flow layer: mapShift [(res+2)^2] float64, A, B, RandomState (Atmosphere.py:281-374, 410-516)."""
+
+    def __init__(self, mapShift, A, B, inner_mask, outer_mask, random_state, ratio):
+        self.mapShift = np.array(mapShift, dtype=np.float64)
+        self.A = np.asarray(A, np.float64)
+        self.B = np.asarray(B, np.float64)
+        self.inner = np.asarray(inner_mask) != 0        # [(res+2)^2] ring masks
+        self.outer = np.asarray(outer_mask) != 0
+        self.rs = random_state
+        self.ratio = np.asarray(ratio, np.float64)
+        self.buff = np.zeros(2)
+        self.res = self.mapShift.shape[0] - 2
+        self.OPD = self.mapShift[1:-1, 1:-1].copy()
+
+    def add_row(self, step):
+        """Atmosphere.add_row (Atmosphere.py:410-421)."""
+        tmp = warp_shift(self.mapShift, step[0], step[1])
+        one = tmp[1:-1, 1:-1]
+        Z = one[self.inner[1:-1, 1:-1]]
+        X = self.A @ Z + self.B @ self.rs.normal(size=self.B.shape[1])
+        self.mapShift[self.outer] = X
+        self.mapShift[~self.outer] = one.reshape(self.res * self.res)
+        return one
+
+    def update(self):
+        """Atmosphere.updateLayer (Atmosphere.py:447-516) for a layer with non-zero wind."""
+        ratio = self.ratio
+        tmp = np.abs(ratio)
+        tmp[np.isinf(tmp)] = 0
+        nScreens = tmp.astype('int')
+        step = np.zeros(2)
+        sub = np.zeros(2)
+        for _ in range(nScreens.min()):
+            step[0] = 1
+            step[1] = 1
+            step = step * np.sign(ratio)
+            self.OPD = self.add_row(step)
+        for _ in range(nScreens.max() - nScreens.min()):
+            step[0] = 1
+            step[1] = 1
+            step = step * np.sign(ratio)
+            step[np.where(nScreens == nScreens.min())] = 0
+            self.OPD = self.add_row(step)
+        sub[0] = (np.abs(ratio[0]) % 1) * np.sign(ratio[0])
+        sub[1] = (np.abs(ratio[1]) % 1) * np.sign(ratio[1])
+        self.buff += sub
+        if np.abs(self.buff[0]) >= 1 or np.abs(self.buff[1]) >= 1:
+            step[0] = 1 * np.sign(self.buff[0])
+            step[1] = 1 * np.sign(self.buff[1])
+            step[np.where(np.abs(self.buff) < 1)] = 0
+            self.OPD = self.add_row(step)
+        self.buff[0] = (np.abs(self.buff[0]) % 1) * np.sign(self.buff[0])
+        self.buff[1] = (np.abs(self.buff[1]) % 1) * np.sign(self.buff[1])
+        self.OPD = warp_shift(self.mapShift, self.buff[0], self.buff[1])[1:-1, 1:-1]
+        return self.OPD
+
+
+def footprint_crop(layer_opd, n):
+    """On-axis pupil footprint of a layer map (Atmosphere.py:309-315, 614-616)."""
+    c = layer_opd.shape[0] // 2
+    return layer_opd[c - n // 2:c + n // 2, c - n // 2:c + n // 2]
+
+
+def atmosphere_opd(layers, fractionalR0, n, wavelength=500e-9):
+    """Sum of layer OPDs in metres on the pupil (fill_OPD_support, Atmosphere.py:589-622)."""
+    out = np.zeros((n, n))
+    for lay, f in zip(layers, fractionalR0):
+        out += footprint_crop(lay.OPD, n) * wavelength / 2 / np.pi * np.sqrt(f)
+    return out

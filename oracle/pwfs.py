@@ -233,7 +233,8 @@ class PyramidOracle:
     def __init__(self, n, nSubap, modulation, lightRatio, pupil, amp_flux, wavelength,
                  n_pix_separation=2.0, n_pix_edge=None, calibModulation=50.0,
                  nTheta_user_defined=None, psfCentering=True, precision='fp64', n_jobs=1,
-                 rooftop=0.0, theta_rotation=0.0, delta_theta=0.0, user_modulation_path=None):
+                 rooftop=0.0, theta_rotation=0.0, delta_theta=0.0, user_modulation_path=None,
+                 validI4Q=None):
         self.geom = Geometry(n, nSubap, n_pix_separation, n_pix_edge, calibModulation)
         self.precision = precision
         self.psfCentering = psfCentering
@@ -249,6 +250,14 @@ class PyramidOracle:
         self.m, self.mask = phase_mask(self.geom, psfCentering, rooftop, theta_rotation)
         self.lightRatio = lightRatio
         self._pool = None
+        if validI4Q is not None:
+            # timing use: take the valid pixels as given, reference slopes zero
+            self.validI4Q = np.asarray(validI4Q, bool)
+            self.validSignal = np.concatenate((self.validI4Q, self.validI4Q))
+            self.nSignal = int(self.validSignal.sum())
+            self.set_modulation(modulation)
+            self.ref2d = 0
+            return
         # initialization(): valid pixels at the calibration modulation
         self.set_modulation(self.geom.calibModulation)
         flat = np.zeros((n, n))

@@ -1,0 +1,73 @@
+"""GPU: frozen-flow atmosphere update against the float64 oracle.
+
+Reference: Atmosphere.updateLayer / add_row / fill_OPD_support
+(Atmosphere.py:410-516, 589-622) and the scikit-image cubic warp
+(tools/tools.py:202-209; the oracle's warp is pinned to scikit-image's own
+outputs by tests/golden/warp_skimage.npz). Both sides start from the same
+mapShift and the same RandomState stream. Tolerance: the device keeps the
+screens in float64 and writes the summed OPD in float32: rel-L2 <= 1e-6.
+"""
+import numpy as np
+import pytest
+from numpy.random import RandomState
+
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _atm(n, Ts, wind, direction, seed):
+    from oopao_amd import Telescope, Source, Atmosphere
+    tel = Telescope(n, 8, Ts)
+    ngs = Source('I3', 8, display_properties=False)
+    ngs * tel
+    atm = Atmosphere(tel, 0.15, 25, windSpeed=wind, fractionalR0=[0.7, 0.3], windDirection=direction,
+                     altitude=[0, 10000])
+    atm.initializeAtmosphere(tel)
+    atm.generateNewPhaseScreen(seed)
+    return tel, ngs, atm
+
+
+def _oracle_layers(atm, seed):
+    from oracle.atmosphere import LayerOracle
+    out = []
+    for i in range(atm.nLayer):
+        lay = getattr(atm, 'layer_%d' % (i + 1))
+        ms = lay.map.double().cpu().numpy().reshape(lay.M, lay.M)
+        rs = RandomState(seed + i * 1000)
+        rs.normal(size=lay.B.shape[1])          # the draw made by generateNewPhaseScreen
+        ratio = [lay.vX * atm.telescope.samplingTime / lay.pixel_size,
+                 lay.vY * atm.telescope.samplingTime / lay.pixel_size]
+        out.append(LayerOracle(ms, lay.A, lay.B, lay.innerMask, lay.outerMask, rs, ratio))
+    return out
+
+
+@pytest.mark.parametrize('wind,direction,Ts', [([10, 20], [0, 90], 0.02),      # 0.8 px and 1.6 px per step
+                                               ([15, 25], [45, 200], 0.01),    # diagonal, negative components
+                                               ([10, 20], [0, 90], 0.001)])    # config-2 speeds: 0.04 / 0.08 px
+def test_update_matches_oracle(wind, direction, Ts):
+    from oracle.atmosphere import atmosphere_opd
+    seed = 10
+    tel, ngs, atm = _atm(32, Ts, wind, direction, seed)
+    layers = _oracle_layers(atm, seed)
+    ref0 = atmosphere_opd(layers, atm.fractionalR0, 32)
+    assert rel_l2(atm.OPD, ref0) < 1e-6
+    worst = 0.0
+    for step in range(30):
+        atm.update()
+        for L in layers:
+            L.update()
+        err = rel_l2(atm.OPD, atmosphere_opd(layers, atm.fractionalR0, 32))
+        worst = max(worst, err)
+    print('worst rel_l2 over 30 steps', worst)
+    assert worst < 1e-6
+
+
+def test_paired_telescope_propagates_atmosphere():
+    tel, ngs, atm = _atm(32, 0.01, [10, 20], [0, 90], 3)
+    tel + atm
+    atm.update()
+    ref = atm.OPD * tel.pupil
+    assert rel_l2(tel.OPD, ref) < 1e-7
+    ngs ** atm * tel
+    assert rel_l2(ngs.OPD, ref) < 1e-7

@@ -1,0 +1,147 @@
+"""CPU: pins of the PWFS oracle (oracle/pwfs.py) by routes independent of its code.
+
+The reference ships no tests or golden vectors for this path, so each oracle
+function is checked against: a closed form evaluated another way, a brute-force
+DFT on a tiny grid, Parseval's theorem, or invariants the reference's own
+construction implies (zero slopes at the calibration wavefront,
+Pyramid.py:591-610).
+"""
+import numpy as np
+import pytest
+
+from oracle import pwfs as opw
+from tests.helpers import rel_l2, small_system, random_phase
+
+
+def test_photometry_and_flux():
+    # Source.py:362 (I3 = 800 nm, zero point 0.58e12/368), Source.py:125
+    assert opw.PHOTOMETRY['I3'][0] == 0.8e-6
+    assert np.isclose(opw.n_photon('I3', 0), 0.58e12 / 368)
+    assert np.isclose(opw.n_photon('I3', 5), 0.58e12 / 368 / 100)
+    pup = opw.telescope_pupil(80)
+    # circular pupil area close to pi (n+1)^2/4 and symmetric
+    assert abs(pup.sum() - np.pi * 81 ** 2 / 4) / pup.sum() < 0.02
+    assert np.array_equal(pup, pup[::-1]) and np.array_equal(pup, pup.T)
+
+
+def test_geometry_config1_and_config2():
+    g = opw.Geometry(80, 20, 20, 10)
+    assert (g.N, g.cam, g.center) == (320, 80, 160)
+    g2 = opw.Geometry(320, 80, 80, 40)
+    assert (g2.N, g2.cam) == (1280, 320)
+    # calibModulation clamp (Pyramid.py:292-296)
+    assert g.calibModulation == 39
+    with pytest.raises(ValueError):
+        opw.Geometry(80, 16, 2)     # n / nSubap = 5 is odd
+
+
+def test_modulation_path_counts_and_circle():
+    p = opw.modulation_path(3)
+    assert p.shape[0] == 4 * int(np.ceil(2 * np.pi * 3 / 4)) == 20
+    p = opw.modulation_path(3, 48)
+    assert p.shape == (48, 2)
+    assert np.allclose(np.hypot(p[:, 0], p[:, 1]), 3.0)
+    assert np.isclose(p[0, 0], 3.0) and np.isclose(p[0, 1], 0.0)
+
+
+def test_mask_closed_form():
+    # default pyramid (no rooftop/rotation/shift): max of the 4 facets is r(|x| + |y|)
+    g = opw.Geometry(80, 20, 20, 10)
+    m, mask = opw.phase_mask(g)
+    N = g.N
+    k = np.arange(N)
+    x = -np.pi + np.pi / N + 2 * np.pi * k / N
+    c = (g.n / g.nSubap) * (g.nSubap + g.sep) / 2
+    closed = -c * (np.abs(x)[None, :] + np.abs(x)[:, None])
+    assert np.max(np.abs(m - closed)) < 1e-9
+    assert mask.dtype == np.complex64
+    assert np.max(np.abs(np.abs(mask) - 1)) < 1e-6
+
+
+def test_phasor_closed_form():
+    N = 32
+    ph = opw.phasor(N)
+    w = np.exp(-1j * np.pi * (N + 1) / N)
+    # phasor[y, x] = w**(x + y): ratios along both axes are constant
+    assert np.allclose(ph[:, 1:] / ph[:, :-1], w)
+    assert np.allclose(ph[1:, :] / ph[:-1, :], w)
+    assert np.isclose(ph[0, 0], 1)
+
+
+def _dft_matrix(N, sign):
+    k = np.arange(N)
+    return np.exp(sign * 2j * np.pi * np.outer(k, k) / N)
+
+
+def test_pyramid_transform_vs_bruteforce_dft():
+    o = small_system(n=8, nSubap=2, modulation=1, T=4)
+    g = o.geom
+    N, n, c = g.N, g.n, g.center
+    amp = np.sqrt(o.flux / o.nTheta)
+    ph = random_phase(n, 2, 1.0, o.pupil) + o.tt[1]
+    got, _ = opw.pyramid_transform(amp, ph, o.mask, o.phasor, g)
+    s = np.zeros((N, N), complex)
+    s[c - n // 2:c + n // 2, c - n // 2:c + n // 2] = amp * np.exp(1j * ph)
+    F = _dft_matrix(N, -1)
+    Fi = _dft_matrix(N, +1) / N
+    E = Fi @ ((F @ (s * o.phasor) @ F) * o.mask) @ Fi
+    assert rel_l2(got, np.abs(E) ** 2) < 1e-12
+
+
+def test_flux_conservation_parseval():
+    # |mask| = 1 and Parseval: sum of the modulated frame = sum of the pupil intensity
+    o = small_system()
+    ph = random_phase(o.geom.n, 4, 2.0, o.pupil)
+    raw = o.raw(ph)
+    assert abs(raw.sum() / o.flux.sum() - 1) < 1e-6
+    assert raw.min() >= 0
+
+
+def test_zero_slopes_at_calibration_and_flux_invariance():
+    o = small_system()
+    s2d, s = o.measure(np.zeros((o.geom.n, o.geom.n)))
+    assert np.max(np.abs(s)) == 0.0
+    ph = random_phase(o.geom.n, 5, 1.0, o.pupil)
+    s_a = o.measure(ph)[1]
+    o2 = small_system()
+    o2.flux = o2.flux * 3.7            # slopes are normalised by the mean valid flux
+    assert rel_l2(o2.measure(ph)[1], s_a) < 1e-12
+
+
+def test_small_tip_sign_and_symmetry():
+    o = small_system()
+    n = o.geom.n
+    x = np.linspace(-1, 1, n)
+    tip = 0.05 * np.tile(x, (n, 1)) * o.pupil           # phase ramp along x (columns)
+    s2d = o.measure(tip)[0]
+    w = o.geom.nSubap
+    sx, sy = s2d[:w], s2d[w:]
+    v = o.validI4Q
+    assert abs(sx[v].mean()) > 10 * abs(sy[v].mean())
+    s2d_neg = o.measure(-tip)[0]
+    # small-signal linearity: opposite tips give opposite slopes
+    assert rel_l2(s2d_neg[:w][v], -sx[v]) < 0.05
+
+
+def test_quadrant_geometry_config1():
+    g = opw.Geometry(80, 20, 20, 10)
+    frame = np.arange(80 * 80, dtype=float).reshape(80, 80)
+    # e = 10, c = 40, w = 20 (Pyramid.py:948-950)
+    assert np.array_equal(opw.grab_quadrant(frame, 1, g), frame[10:30, 10:30])
+    assert np.array_equal(opw.grab_quadrant(frame, 2, g), frame[10:30, 50:70])
+    assert np.array_equal(opw.grab_quadrant(frame, 3, g), frame[50:70, 50:70])
+    assert np.array_equal(opw.grab_quadrant(frame, 4, g), frame[50:70, 10:30])
+
+
+def test_set_binning_sum():
+    a = np.arange(64.).reshape(8, 8)
+    b = opw.set_binning(a, 4)
+    assert b.shape == (2, 2)
+    assert b[0, 0] == a[:4, :4].sum() and b[1, 0] == a[4:, :4].sum()
+
+
+def test_fp32_mode_close_to_fp64():
+    o64 = small_system(precision='fp64')
+    o32 = small_system(precision='fp32')
+    ph = random_phase(o64.geom.n, 6, 1.0, o64.pupil)
+    assert rel_l2(o32.frame(ph), o64.frame(ph)) < 1e-5
