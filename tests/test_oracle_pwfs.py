@@ -19,8 +19,8 @@ def test_photometry_and_flux():
     assert np.isclose(opw.n_photon('I3', 0), 0.58e12 / 368)
     assert np.isclose(opw.n_photon('I3', 5), 0.58e12 / 368 / 100)
     pup = opw.telescope_pupil(80)
-    # circular pupil area close to pi (n+1)^2/4 and symmetric
-    assert abs(pup.sum() - np.pi * 81 ** 2 / 4) / pup.sum() < 0.02
+    # radius (n+1)/2 on a grid of spacing n/(n-1) (Telescope.py:229-233): area pi/4 ((n+1)(n-1)/n)^2 px
+    assert abs(pup.sum() - np.pi / 4 * (81 * 79 / 80) ** 2) / pup.sum() < 0.01
     assert np.array_equal(pup, pup[::-1]) and np.array_equal(pup, pup.T)
 
 
