@@ -1,0 +1,79 @@
+"""Integrator closed loop with all state on the GPU.
+
+Same sequence as tutorials/AO_closed_loop_Pyramid_WFS.py:601-638 (and
+closed_loop/run_cl.py:176-217): per iteration
+    atm.update();  ngs**atm*tel*dm*wfs
+    frame_delay == 1: s = wfs.signal
+    dm.coefs = dm.coefs - gain * reconstructor @ s
+    frame_delay == 2: s = wfs.signal
+with the OPD sum, PWFS frame, slopes, reconstructor GEMV and integrator update
+kept on the device (no host round trip per iteration). Records the
+turbulence and residual wavefront error in the pupil (nm RMS) per iteration.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from ._device import ptr, stream_ptr
+
+
+def _pupil_std_nm(opd, pupil_mask, npix):
+    sel = opd[pupil_mask]
+    return sel.double().std(unbiased=False) * 1e9
+
+
+def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100, frame_delay=2,
+                    record_slopes=False):
+    dev = wfs.device
+    R = reconstructor if isinstance(reconstructor, torch.Tensor) else torch.from_numpy(
+        np.ascontiguousarray(reconstructor, dtype=np.float32))
+    R = R.to(dev, torch.float32).contiguous()
+    nAct, nSig = R.shape
+    if nAct != dm.nValidAct or nSig != wfs.nSignal:
+        raise ValueError('reconstructor shape %s does not match (nAct=%d, nSignal=%d)'
+                         % (tuple(R.shape), dm.nValidAct, wfs.nSignal))
+    pupil = tel._pupil_dev
+    pmask = pupil > 0
+    npix = int(pmask.sum())
+    coefs = torch.as_tensor(np.asarray(dm.coefs, np.float32), device=dev).clone()
+    sig_buf = torch.zeros(nSig, dtype=torch.float32, device=dev)
+    new_coefs = torch.empty_like(coefs)
+    wfe_atm = torch.zeros(n_iter, dtype=torch.float64, device=dev)
+    wfe_res = torch.zeros(n_iter, dtype=torch.float64, device=dev)
+    slopes = torch.zeros((n_iter, nSig), dtype=torch.float32, device=dev) if record_slopes else None
+    tel.isPaired = False
+    ngs ** tel
+    amp = wfs._amp()
+    scale = 2 * np.pi / ngs.wavelength
+    eng, se = wfs.engine, wfs.slopes_engine
+    buf = eng.buffers(1)
+    frame = torch.empty((1, eng.cam, eng.cam), dtype=torch.float32, device=dev)
+    s2d = torch.empty((1, 2 * se.w, se.w), dtype=torch.float32, device=dev)
+    s = torch.empty((1, se.nsig), dtype=torch.float32, device=dev)
+    for i in range(n_iter):
+        atm.update()
+        turb = atm._OPD_dev
+        wfe_atm[i] = _pupil_std_nm(turb * pupil, pmask, npix)
+        dm_opd = dm.opd_from_coefs(coefs)
+        opd = ((turb + dm_opd) * pupil).contiguous()
+        wfe_res[i] = _pupil_std_nm(opd, pmask, npix)
+        eng.rows(opd[None], scale, amp, buf)
+        eng.cols(buf, frame)
+        se.slopes(frame, s2d, s)
+        if record_slopes:
+            slopes[i] = s[0]
+        if frame_delay == 1:
+            sig_buf.copy_(s[0])
+        # coefs <- coefs - gain * R @ sig_buf   (R [nAct, nSig] row-major GEMV)
+        _lib.call('oopao_gemv', ptr(R), ptr(sig_buf), ptr(new_coefs), ptr(coefs), nAct, nSig, float(-gain), 1.0,
+                  stream_ptr(dev))
+        coefs, new_coefs = new_coefs, coefs
+        if frame_delay == 2:
+            sig_buf.copy_(s[0])
+    torch.cuda.synchronize(dev)
+    dm.coefs = coefs.double().cpu().numpy()
+    out = dict(wfe_atmosphere=wfe_atm.cpu().numpy(), wfe_residual=wfe_res.cpu().numpy(),
+               coefs=coefs.double().cpu().numpy())
+    if record_slopes:
+        out['slopes'] = slopes.double().cpu().numpy()
+    return out

@@ -1,0 +1,124 @@
+"""Interaction matrix, CalibrationVault and the closed loop.
+
+CPU: CalibrationVault (calibration/CalibrationVault.py:45-70) checked against
+the Moore-Penrose identities and numpy's float64 pinv.
+GPU: InteractionMatrix (calibration/InteractionMatrix.py:13-169) against the
+oracle's columns; the device closed loop against a CPU oracle loop
+(tutorials/AO_closed_loop_Pyramid_WFS.py:601-638) driven by the same
+reconstructor and the same atmosphere stream.
+"""
+import numpy as np
+import pytest
+from numpy.random import RandomState
+
+from tests.helpers import rel_l2
+
+
+def test_calibration_vault_pinv_identities():
+    from oopao_amd.calibration.CalibrationVault import CalibrationVault
+    rs = np.random.RandomState(0)
+    D = rs.randn(60, 25) @ np.diag(np.logspace(0, -3, 25))
+    cv = CalibrationVault(D, device='cpu')
+    M = cv.M
+    assert np.allclose(D @ M @ D, D, atol=1e-10)
+    assert np.allclose(M @ D @ M, M, atol=1e-6)
+    assert np.allclose(M, np.linalg.pinv(D), rtol=1e-8, atol=1e-8)
+    s = np.linalg.svd(D, compute_uv=False)
+    assert np.allclose(cv.eigenValues, s, rtol=1e-10)
+    assert np.isclose(cv.cond, s[0] / s[-1], rtol=1e-10)
+    cv.nTrunc = 5
+    assert cv.Mtrunc.shape == (25, 60)
+    assert np.isclose(cv.cond, cv.eigenValues[0] / cv.eigenValues[-6])
+
+
+def _small_loop_system(seed=10):
+    from oopao_amd import Telescope, Source, DeformableMirror, Pyramid, Atmosphere
+    tel = Telescope(32, 8, 0.005)
+    ngs = Source('I3', 8, display_properties=False)
+    ngs * tel
+    atm = Atmosphere(tel, 0.15, 25, windSpeed=[10, 20], fractionalR0=[0.7, 0.3], windDirection=[0, 90],
+                     altitude=[0, 10000])
+    atm.initializeAtmosphere(tel)
+    atm.generateNewPhaseScreen(seed)
+    dm = DeformableMirror(tel, 8, 0.35, print_dm_properties=False)
+    wfs = Pyramid(8, tel, 2, 0.1, n_pix_separation=8, n_pix_edge=4, nTheta_user_defined=16)
+    return tel, ngs, atm, dm, wfs
+
+
+def _oracle_pwfs(wfs, tel, ngs):
+    from oracle.pwfs import PyramidOracle, flux_map
+    flux = flux_map(tel.pupil, ngs.nPhoton, tel.samplingTime, tel.D)
+    return PyramidOracle(tel.resolution, wfs.nSubap, wfs.modulation, wfs.lightRatio, tel.pupil, flux, ngs.wavelength,
+                         n_pix_separation=wfs.n_pix_separation, n_pix_edge=wfs.n_pix_edge,
+                         nTheta_user_defined=wfs.nTheta)
+
+
+@pytest.mark.gpu
+def test_interaction_matrix_matches_oracle():
+    from oopao_amd import InteractionMatrix
+    from oracle.dm import DMOracle
+    tel, ngs, atm, dm, wfs = _small_loop_system()
+    o = _oracle_pwfs(wfs, tel, ngs)
+    assert np.array_equal(o.validI4Q, wfs.validI4Q)
+    od = DMOracle(32, 8, 8, 0.35)
+    modes = od.modes()
+    M2C = np.eye(dm.nValidAct)[:, :24]
+    stroke = 20e-9
+    calib = InteractionMatrix(ngs, tel, dm, wfs, M2C, stroke, nMeasurements=10, single_pass=True)
+    D = calib.D
+    assert D.shape == (wfs.nSignal, 24)
+    Dref = np.stack([o.measure_opd(od.opd(modes, M2C[:, k] * stroke))[1] / stroke for k in range(24)], axis=1)
+    err = rel_l2(D, Dref)
+    print('IM rel_l2 vs fp64 oracle', err)
+    assert err < 1e-4
+    calib_pp = InteractionMatrix(ngs, tel, dm, wfs, M2C, stroke, nMeasurements=24, single_pass=False)
+    Dpp = np.stack([0.5 * (o.measure_opd(od.opd(modes, M2C[:, k] * stroke))[1]
+                           - o.measure_opd(od.opd(modes, -M2C[:, k] * stroke))[1]) / stroke for k in range(24)], axis=1)
+    assert rel_l2(calib_pp.D, Dpp) < 1e-4
+
+
+@pytest.mark.gpu
+def test_closed_loop_matches_oracle_loop():
+    from oopao_amd import InteractionMatrix
+    from oopao_amd.closed_loop import run_closed_loop
+    from oracle.dm import DMOracle
+    from oracle.atmosphere import LayerOracle, atmosphere_opd
+    seed = 10
+    tel, ngs, atm, dm, wfs = _small_loop_system(seed)
+    calib = InteractionMatrix(ngs, tel, dm, wfs, np.eye(dm.nValidAct), 20e-9, nMeasurements=100)
+    calib.nTrunc = 10
+    R = calib.Mtrunc
+    # oracle atmosphere from the same initial state and RandomState streams
+    layers = []
+    for i in range(atm.nLayer):
+        lay = getattr(atm, 'layer_%d' % (i + 1))
+        ms = lay.map.double().cpu().numpy().reshape(lay.M, lay.M)
+        rs = RandomState(seed + i * 1000)
+        rs.normal(size=lay.B.shape[1])
+        ratio = [lay.vX * tel.samplingTime / lay.pixel_size, lay.vY * tel.samplingTime / lay.pixel_size]
+        layers.append(LayerOracle(ms, lay.A, lay.B, lay.innerMask, lay.outerMask, rs, ratio))
+    n_iter, gain = 25, 0.4
+    out = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=gain, n_iter=n_iter, record_slopes=True)
+    o = _oracle_pwfs(wfs, tel, ngs)
+    od = DMOracle(32, 8, 8, 0.35)
+    modes = od.modes()
+    coefs = np.zeros(dm.nValidAct)
+    sig = np.zeros(wfs.nSignal)
+    wfe = []
+    s_trace = []
+    for it in range(n_iter):
+        for L in layers:
+            L.update()
+        opd = (atmosphere_opd(layers, [0.7, 0.3], 32) + od.opd(modes, coefs)) * tel.pupil
+        wfe.append(np.std(opd[tel.pupil > 0]) * 1e9)
+        s = o.measure_opd(opd)[1]
+        s_trace.append(s)
+        coefs = coefs - gain * R @ sig
+        sig = s
+    wfe = np.array(wfe)
+    print('residual wfe (nm) device', out['wfe_residual'][-5:], 'oracle', wfe[-5:])
+    assert np.max(np.abs(out['wfe_residual'] - wfe) / wfe) < 1e-4
+    err = max(rel_l2(out['slopes'][k], s_trace[k]) for k in range(n_iter))
+    print('worst per-iteration slope rel_l2', err)
+    assert err < 1e-4
+    assert out['wfe_residual'][-1] < 0.5 * out['wfe_atmosphere'][-1]      # the loop corrects
