@@ -60,6 +60,52 @@ def cpu_baseline(validI4Q, opd, frames=3, warmup=1):
                        f'after {warmup} warm-up, {n_jobs} threads; host reports {ncpu} CPUs')
 
 
+def closed_loop_leg(tel, ngs, wfs, dev, n_iter=100):
+    """Closed-loop iterations/s at config 2: 2-layer atmosphere update, 81x81 DM, PWFS, slopes,
+    reconstructor GEMV [nAct x nSignal] (seeded random, same shape as M2C @ pinv(D)) and integrator,
+    all on the device (oopao_amd.closed_loop)."""
+    import torch
+    from oopao_amd import Atmosphere, DeformableMirror
+    from oopao_amd.closed_loop import run_closed_loop
+    atm = Atmosphere(tel, 0.15, 25, windSpeed=[10, 20], fractionalR0=[0.7, 0.3], windDirection=[0, 90],
+                     altitude=[0, 10000])
+    atm.initializeAtmosphere(tel)
+    atm.generateNewPhaseScreen(10)
+    dm = DeformableMirror(tel, N_SUBAP, 0.35, print_dm_properties=False)
+    rs = np.random.RandomState(0)
+    R = (rs.randn(dm.nValidAct, wfs.nSignal) * 1e-9).astype(np.float32)
+    run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=5)          # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=n_iter)
+    dt = time.perf_counter() - t0
+    return {'value': round(n_iter / dt, 2), 'unit': 'iterations/s', 'iterations': n_iter, 'nAct': int(dm.nValidAct),
+            'nSignal': int(wfs.nSignal), 'dm_route': 'dense' if dm.use_dense else 'separable',
+            'note': 'reconstructor is seeded random (timing only); wall clock incl. host RNG draws'}
+
+
+def im_leg(dev, n_modes=1000):
+    """Interaction-matrix modes/s: push-pull of n_modes seeded random DM modes (+-10 nm) through the
+    config-1 PWFS (80 px pupil, 20x20 subapertures, 48 points), batched, FP64 SVD included."""
+    import torch
+    from oopao_amd import Telescope, Source, DeformableMirror, Pyramid, InteractionMatrix
+    tel = Telescope(80, 8.0, 1e-3)
+    ngs = Source('I3', 8, display_properties=False)
+    ngs * tel
+    dm = DeformableMirror(tel, 20, 0.35, print_dm_properties=False)
+    wfs = Pyramid(20, tel, 3, 0.5, n_pix_separation=20, n_pix_edge=10, nTheta_user_defined=48)
+    M2C = np.random.RandomState(1).randn(dm.nValidAct, n_modes)
+    M2C /= np.std(M2C, axis=0)
+    InteractionMatrix(ngs, tel, dm, wfs, M2C[:, :50], 10e-9, nMeasurements=50, single_pass=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    calib = InteractionMatrix(ngs, tel, dm, wfs, M2C, 10e-9, nMeasurements=250, single_pass=False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {'value': round(n_modes / dt, 1), 'unit': 'modes/s (push-pull, incl. FP64 SVD)', 'modes': n_modes,
+            'nSignal': int(wfs.nSignal), 'cond': float(calib.cond)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -68,6 +114,7 @@ def main():
     ap.add_argument('--cpu-frames', type=int, default=3)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--batch', type=int, default=8, help='screens per call for the batched form')
+    ap.add_argument('--no-extra', action='store_true', help='skip the closed-loop and interaction-matrix legs')
     ap.add_argument('--calib-copy', action='store_true',
                     help='also run one elementwise copy of 256 MiB (counter calibration for rocprof --pmc)')
     args = ap.parse_args()
@@ -160,6 +207,11 @@ def main():
     torch.cuda.synchronize()
     batched_fps = B * KB * 1000.0 / s0.elapsed_time(s1)
 
+    extra = {}
+    if not args.no_extra:
+        extra['closed_loop'] = closed_loop_leg(tel, ngs, wfs, dev)
+        extra['interaction_matrix'] = im_leg(dev)
+
     out = {
         'metric': 'PWFS frames/s at 80x80 subaps, 72 mod pts',
         'value': round(fps, 2),
@@ -186,6 +238,7 @@ def main():
         'batched': {'batch': B, 'value': round(batched_fps, 2), 'unit': 'frames/s'},
         'wall_s_timed': round(wall, 4),
     }
+    out.update(extra)
     if not args.no_cpu:
         valid = wfs.validI4Q.copy()
         out['cpu_baseline'] = cpu_baseline(valid, opd_host, frames=args.cpu_frames)

@@ -187,6 +187,23 @@ __device__ __forceinline__ const T* launder(const T* p) {
   return p + zero;
 }
 
+// Value of lane (lane & ~3) + c for every lane of each quad (DPP quad_perm
+// broadcast; c must be uniform across the wavefront).
+__device__ __forceinline__ float dpp_quad_bcast(float x, int c) {
+  const int v = __float_as_int(x);
+  int r;
+  switch (c) {
+    case 0: r = __builtin_amdgcn_mov_dpp(v, 0x00, 0xF, 0xF, false); break;
+    case 1: r = __builtin_amdgcn_mov_dpp(v, 0x55, 0xF, 0xF, false); break;
+    case 2: r = __builtin_amdgcn_mov_dpp(v, 0xAA, 0xF, 0xF, false); break;
+    default: r = __builtin_amdgcn_mov_dpp(v, 0xFF, 0xF, 0xF, false); break;
+  }
+  return __int_as_float(r);
+}
+__device__ __forceinline__ float2 quad_bcast(float2 x, int c) {
+  return make_float2(dpp_quad_bcast(x.x, c), dpp_quad_bcast(x.y, c));
+}
+
 // One Stockham stage (radix R, already-sorted span NS) on the strided layout.
 // When NS divides TPT the twiddle index k = j mod NS equals t mod NS for all of
 // the lane's butterflies, so R-1 twiddles are loaded once per stage.

@@ -118,9 +118,15 @@ k_rows_sep(const float2* __restrict__ field,
 }
 
 // ---------------------------------------------------------------------------
-// Columns pass. One wavefront per (screen b, point chunk, detector column g):
-// the 4 pupil-plane columns 4g..4g+3 are transformed for every point of the
-// chunk, |E|^2 accumulated in registers, then binned 4 rows x 4 columns.
+// Columns pass. Per (screen b, point chunk, detector column g) the 4 pupil-
+// plane columns 4g..4g+3 are transformed for every point of the chunk, |E|^2
+// accumulated in registers, then binned 4 rows x 4 columns.
+//  * 64-lane transforms (one per wavefront): one workgroup per (b, chunk, g),
+//    wavefront w takes column 4g+w. The four wavefronts walk the same points
+//    in step, so each 32-byte row segment of the slab is fetched once into
+//    L1/L2 and shared; the column sums are combined through LDS.
+//  * narrower transforms: one wavefront per (b, chunk, g), its 64/TPT lane
+//    groups taking the columns.
 // Output: part[b][chunk][g][r] (detector column-major partial sums).
 // ---------------------------------------------------------------------------
 template <int N, int TPT>
@@ -131,17 +137,19 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   constexpr int E = N / TPT;
   constexpr int G = 64 / TPT;
   constexpr int NG = N / 4;
+  constexpr bool kWaveCols = (TPT == 64);
   __shared__ float2 lds[kWaves * G][lds_len(N)];
+  __shared__ float red[kWaveCols ? kWaves * NG : 1];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int h = lane / TPT;
   const int t = lane % TPT;
-  int task = blockIdx.x * kWaves + wave;
+  int task = kWaveCols ? (int)blockIdx.x : (int)blockIdx.x * kWaves + wave;
   const int g = task % NG;
   task /= NG;
   const int chunk = task % nchunk;
   const int b = task / nchunk;
-  if (b >= B) return;  // whole wavefront uniform
+  if (b >= B) return;  // uniform over the wavefront (and over the workgroup when kWaveCols)
   const int i0 = (int)(((long)chunk * T) / nchunk);
   const int i1 = (int)(((long)(chunk + 1) * T) / nchunk);
   float2* xl = lds[wave * G + h];
@@ -152,8 +160,8 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   for (int i = i0; i < i1; ++i) {
     const float2* tile = ws + (((size_t)b * T + i) * NG + g) * (size_t)n * 4;
 #pragma unroll 1
-    for (int c0 = 0; c0 < 4; c0 += G) {
-      const int c = c0 + h;
+    for (int c0 = 0; c0 < (kWaveCols ? 1 : 4); c0 += G) {
+      const int c = kWaveCols ? wave : c0 + h;
       float2 v[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + c];
@@ -165,17 +173,29 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
       for (int m = 0; m < E; ++m) acc[m] += cabs2(v[m]);
     }
   }
-  // rows t..t+3 (a lane quad) form one detector row; lane groups hold the
-  // other pupil columns of the same detector column.
+  // rows t..t+3 (a lane quad) form one detector row
   float* dst = part + (((size_t)b * nchunk + chunk) * NG + g) * (size_t)NG;
+  if constexpr (kWaveCols) {
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    float s = acc[m];
-    s += __shfl_xor(s, 1);
-    s += __shfl_xor(s, 2);
+    for (int m = 0; m < E; ++m) {
+      float s = acc[m];
+      s += __shfl_xor(s, 1);
+      s += __shfl_xor(s, 2);
+      if ((t & 3) == 0) red[wave * NG + ((t + 64 * m) >> 2)] = s;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < NG; r += 64 * kWaves)
+      dst[r] = (red[r] + red[NG + r]) + (red[2 * NG + r] + red[3 * NG + r]);
+  } else {
 #pragma unroll
-    for (int off = TPT; off < 64; off <<= 1) s += __shfl_xor(s, off);
-    if (h == 0 && (t & 3) == 0) dst[(t + TPT * m) >> 2] = s;
+    for (int m = 0; m < E; ++m) {
+      float s = acc[m];
+      s += __shfl_xor(s, 1);
+      s += __shfl_xor(s, 2);
+#pragma unroll
+      for (int off = TPT; off < 64; off <<= 1) s += __shfl_xor(s, off);
+      if (h == 0 && (t & 3) == 0) dst[(t + TPT * m) >> 2] = s;
+    }
   }
 }
 
@@ -296,7 +316,7 @@ static int launch_cols(const float2* inter, const float2* my, int T, int B, int 
   if (rc) return rc;
   {
     const long tasks = (long)B * nchunk * (N / 4);
-    dim3 grid((unsigned)((tasks + kWaves - 1) / kWaves));
+    dim3 grid((unsigned)(TPT == 64 ? tasks : (tasks + kWaves - 1) / kWaves));
     hipLaunchKernelGGL((k_cols_sep<N, TPT>), grid, dim3(64 * kWaves), 0, st, inter, my, tw, part, B, T, nchunk);
     OOPAO_LAUNCH_CHECK();
   }
