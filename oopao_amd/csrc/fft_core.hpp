@@ -143,13 +143,33 @@ struct Dft<8, DIR> {
 // Compile-time radix plans. Every radix must divide E so that each lane owns
 // whole butterflies. Larger radices first: fewer LDS exchanges.
 // ---------------------------------------------------------------------------
+// Odd radices go first: a radix-5 (or 3) stage at a small span writes with a
+// lane stride coprime to the 16 eight-byte LDS bank groups, and the power-of-
+// two stages that follow then run at spans >= 4/5 where their writes are
+// (nearly) conflict-free; power-of-two stages at spans 1 and 4 are 4-way
+// bank-conflicted.
+// (Measured on gfx950 at N = 1280: odd-first plans run 2x slower because spans
+// 5, 20, 80, 320 do not divide the 64-lane count, so every twiddle becomes a
+// per-butterfly load; kept as an option, off by default.)
+#ifndef OOPAO_ODD_FIRST
+#define OOPAO_ODD_FIRST 0
+#endif
 __host__ __device__ constexpr int pick_radix(int rem, int E) {
+#if OOPAO_ODD_FIRST
+  return (rem % 5 == 0 && E % 5 == 0) ? 5
+       : (rem % 3 == 0 && E % 3 == 0) ? 3
+       : (rem % 8 == 0 && E % 8 == 0) ? 8
+       : (rem % 4 == 0 && E % 4 == 0) ? 4
+       : (rem % 2 == 0 && E % 2 == 0) ? 2
+       : 0;
+#else
   return (rem % 8 == 0 && E % 8 == 0) ? 8
        : (rem % 4 == 0 && E % 4 == 0) ? 4
        : (rem % 5 == 0 && E % 5 == 0) ? 5
        : (rem % 3 == 0 && E % 3 == 0) ? 3
        : (rem % 2 == 0 && E % 2 == 0) ? 2
        : 0;
+#endif
 }
 
 __host__ __device__ constexpr bool plan_ok(int rem, int E) {
@@ -174,6 +194,20 @@ __host__ __device__ constexpr int lds_len(int N) { return N + N / 32 + 1; }
 __device__ __forceinline__ int lds_pad(int i) { return i; }
 __host__ __device__ constexpr int lds_len(int N) { return N; }
 #endif
+
+// XOR swizzle of the exchange-buffer index for 64-lane transforms: the low 4
+// bits (the 16 eight-byte bank groups of one 128-byte LDS row) are XORed with
+// 5 * (row mod 4). Reads of the strided layout t + 64m stay conflict-free and
+// the span-1 and span-4 radix-4 writes (4-way conflicted unswizzled) become
+// conflict-free; the XOR term depends only on the lane, so per-butterfly
+// offsets (multiples of 64) remain immediate offsets.
+#ifndef OOPAO_LDS_SWZ
+#define OOPAO_LDS_SWZ 1
+#endif
+__device__ __forceinline__ int lds_swz(int a) {
+  const int rw = (a >> 4) & 3;
+  return a ^ (rw | (rw << 2));
+}
 
 // Opaque copy of a read-only table pointer. Loads through the returned pointer
 // cannot be hoisted above this point, which keeps loop-invariant twiddle and
@@ -207,7 +241,7 @@ __device__ __forceinline__ float2 quad_bcast(float2 x, int c) {
 // One Stockham stage (radix R, already-sorted span NS) on the strided layout.
 // When NS divides TPT the twiddle index k = j mod NS equals t mod NS for all of
 // the lane's butterflies, so R-1 twiddles are loaded once per stage.
-template <int N, int TPT, int R, int NS, int DIR>
+template <int N, int TPT, int R, int NS, int DIR, bool SWZ = false>
 __device__ __forceinline__ void stockham_stage(float2* v, float2* lds, const float2* __restrict__ tw, int t) {
   constexpr int E = N / TPT;
   constexpr int NB = E / R;
@@ -215,6 +249,15 @@ __device__ __forceinline__ void stockham_stage(float2* v, float2* lds, const flo
   constexpr int TS = N / (NS * R);
   static_assert(E % R == 0, "radix must divide the per-lane element count");
   const float2* twp = launder(tw);
+  // XOR-swizzled exchange slots for 64-lane transforms (see lds_swz); the
+  // lane-dependent part is computed once per stage, butterflies add constants.
+  constexpr bool kSwz = SWZ && (OOPAO_LDS_SWZ != 0) && (TPT == 64) && ((64 % NS == 0) || (NS % 64 == 0));
+  int p0[R];
+  if constexpr (kSwz) {
+    const int base0 = (t / NS) * (NS * R) + (t % NS);
+#pragma unroll
+    for (int r = 0; r < R; ++r) p0[r] = lds_swz(base0 + r * NS);
+  }
   float2 wu[R];
   if (kUniform && NS > 1) {
     const int k = t % NS;
@@ -246,45 +289,59 @@ __device__ __forceinline__ void stockham_stage(float2* v, float2* lds, const flo
       }
     }
     Dft<R, DIR>::run(x);
-    const int base = (j / NS) * (NS * R) + k;
+    if constexpr (kSwz) {
+      // swizzled slot of butterfly b = slot of butterfly 0 + a multiple of 64
+      const int off = ((64 * b) / NS) * (NS * R) + (64 * b) % NS;   // folds to a constant
 #pragma unroll
-    for (int r = 0; r < R; ++r) lds[lds_pad(base + r * NS)] = x[r];
+      for (int r = 0; r < R; ++r) lds[p0[r] + off] = x[r];
+    } else {
+      const int base = (j / NS) * (NS * R) + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) lds[lds_pad(base + r * NS)] = x[r];
+    }
   }
   wave_lds_sync();
+  if constexpr (kSwz) {
+    const int q = lds_swz(t);
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = lds[lds_pad(t + m * TPT)];
+    for (int m = 0; m < E; ++m) v[m] = lds[q + 64 * m];
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = lds[lds_pad(t + m * TPT)];
+  }
   wave_lds_sync();
 }
 
 // Remaining stages from span NS (REM = N / NS still to transform).
-template <int N, int TPT, int DIR, int NS, int REM>
+template <int N, int TPT, int DIR, int NS, int REM, bool SWZ = false>
 struct Stages {
   static constexpr int E = N / TPT;
   static constexpr int R = pick_radix(REM, E);
   static_assert(R != 0, "length not supported by the radix set for this lane count");
   __device__ __forceinline__ static void run(float2* v, float2* lds, const float2* __restrict__ tw, int t) {
-    stockham_stage<N, TPT, R, NS, DIR>(v, lds, tw, t);
-    Stages<N, TPT, DIR, NS * R, REM / R>::run(v, lds, tw, t);
+    stockham_stage<N, TPT, R, NS, DIR, SWZ>(v, lds, tw, t);
+    Stages<N, TPT, DIR, NS * R, REM / R, SWZ>::run(v, lds, tw, t);
   }
 };
-template <int N, int TPT, int DIR, int NS>
-struct Stages<N, TPT, DIR, NS, 1> {
+template <int N, int TPT, int DIR, int NS, bool SWZ>
+struct Stages<N, TPT, DIR, NS, 1, SWZ> {
   __device__ __forceinline__ static void run(float2*, float2*, const float2* __restrict__, int) {}
 };
 
 // Full transform of the strided-layout registers (natural order in and out).
-template <int N, int TPT, int DIR>
+// SWZ selects the XOR-swizzled exchange (64-lane transforms only).
+template <int N, int TPT, int DIR, bool SWZ = false>
 __device__ __forceinline__ void fft_full(float2* v, float2* lds, const float2* __restrict__ tw, int t) {
-  Stages<N, TPT, DIR, 1, N>::run(v, lds, tw, t);
+  Stages<N, TPT, DIR, 1, N, SWZ>::run(v, lds, tw, t);
 }
 
 // Forward transform of an input that is non-zero only on [0, N/P): the first
 // radix-P stage of such an input replicates x[q/P] into position q, so the
 // caller loads v[m] = x[(t + m*TPT) / P] and the remaining stages start at
 // span P.
-template <int N, int TPT, int P>
+template <int N, int TPT, int P, bool SWZ = false>
 __device__ __forceinline__ void fft_fwd_pruned(float2* v, float2* lds, const float2* __restrict__ tw, int t) {
-  Stages<N, TPT, -1, P, N / P>::run(v, lds, tw, t);
+  Stages<N, TPT, -1, P, N / P, SWZ>::run(v, lds, tw, t);
 }
 
 }  // namespace oopao

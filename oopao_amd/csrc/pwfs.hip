@@ -165,10 +165,12 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
       float2 v[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + c];
-      fft_fwd_pruned<N, TPT, 4>(v, xl, tw, t);
+      // XOR-swizzled LDS exchange (conflict-free small-span stages): measured
+      // faster in this pass, slower in the rows pass (extra ds instructions).
+      fft_fwd_pruned<N, TPT, 4, true>(v, xl, tw, t);
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = cmul(v[m], launder(my)[t + TPT * m]);
-      fft_full<N, TPT, +1>(v, xl, tw, t);
+      fft_full<N, TPT, +1, true>(v, xl, tw, t);
 #pragma unroll
       for (int m = 0; m < E; ++m) acc[m] += cabs2(v[m]);
     }

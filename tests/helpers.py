@@ -29,6 +29,15 @@ def config1_system(precision='fp64', n_jobs=1):
                              nTheta_user_defined=48, precision=precision, n_jobs=n_jobs)
 
 
+def config2_system(precision='fp64', n_jobs=16):
+    """Config 2: 320 px pupil, 80 x 80 subapertures, 72 points at 3 lambda/D (N = 1280)."""
+    n = 320
+    pup = opw.telescope_pupil(n)
+    flux = opw.flux_map(pup, opw.n_photon('I3', 8), 1e-3, 8.0)
+    return opw.PyramidOracle(n, 80, 3, 0.5, pup, flux, 0.8e-6, n_pix_separation=80, n_pix_edge=40,
+                             nTheta_user_defined=72, precision=precision, n_jobs=n_jobs)
+
+
 def random_phase(n, seed, rms=1.0, pupil=None):
     rs = np.random.RandomState(seed)
     ph = rs.randn(n, n)

@@ -7,7 +7,7 @@ with the fp32 oracle at rel-L2 <= 1e-5.
 import numpy as np
 import pytest
 
-from tests.helpers import rel_l2, small_system, config1_system, random_phase
+from tests.helpers import rel_l2, small_system, config1_system, config2_system, random_phase
 
 pytestmark = pytest.mark.gpu
 
@@ -34,7 +34,7 @@ def _frames(eng, amp, phases):
     return out.cpu().numpy()
 
 
-@pytest.mark.parametrize('builder', [small_system, config1_system])
+@pytest.mark.parametrize('builder', [small_system, config1_system, config2_system])
 def test_frames_match_oracle(builder):
     o = builder()
     eng, amp = _engine_for(o)
@@ -48,7 +48,7 @@ def test_frames_match_oracle(builder):
         assert abs(got[b].sum() - ref.sum()) / ref.sum() < 1e-5
 
 
-@pytest.mark.parametrize('builder', [small_system, config1_system])
+@pytest.mark.parametrize('builder', [small_system, config1_system, config2_system])
 def test_slopes_match_oracle(builder):
     import torch
     from oopao_amd.pwfs_engine import SlopesEngine
