@@ -77,6 +77,17 @@ int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void*
 int oopao_pwfs_cols(const void* inter, const void* my, int n, int N, int T, int B, int nchunk, float* part,
                     float* frames, void* stream);
 
+/* Modulation focal-plane camera (Pyramid.py:638-642 modulation_camera_em,
+ * 1183-1190 intensity crop): for B screens,
+ *   out[b][crop][crop] = sum_i |fft2(pad(E_b tx_i ty_i))|^2 / N^2
+ * on the crop x crop window starting at N/2 - crop/2 (crop = the focal
+ * camera's resolution). Same field inputs as oopao_pwfs_frames; workspace
+ * size from oopao_pwfs_focal_bytes. */
+int oopao_pwfs_focal_bytes(int n, int N, int T, int B, int nchunk, int crop, size_t* bytes);
+int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void* tx, const void* ty, int n,
+                     int N, int T, int B, int nchunk, int pts_per_task, int crop, void* workspace,
+                     size_t ws_bytes, float* out, void* stream);
+
 /* slopesMaps post-processing (Pyramid.py:835-854, grabQuadrant 946-961):
  *   frames [B][cam][cam] float32, valid [w][w] uint8 (validI4Q),
  *   quad[8] int32 (row origins of quadrants 1..4, then column origins),

@@ -58,9 +58,10 @@ __global__ void k_field(const float* __restrict__ opd, float scale, const float*
 
 // ---------------------------------------------------------------------------
 // Rows pass. One lane group (TPT lanes) per pupil row y; the row's field is
-// loaded once and reused for a chunk of modulation points.
+// loaded once and reused for a chunk of modulation points. FWD_ONLY stops
+// after the forward transform (focal-plane field rows, no mask).
 // ---------------------------------------------------------------------------
-template <int N, int TPT>
+template <int N, int TPT, bool FWD_ONLY = false>
 __global__ void __launch_bounds__(64 * kWaves)
 k_rows_sep(const float2* __restrict__ field,
            const float2* __restrict__ tx, const float2* __restrict__ ty,
@@ -100,12 +101,12 @@ k_rows_sep(const float2* __restrict__ field,
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(cmul(u[m], txi[(t + TPT * m) >> 2]), tyv);
     fft_fwd_pruned<N, TPT, 4>(v, xl, tw, t);
-    {
+    if constexpr (!FWD_ONLY) {
       const float2* mxp = launder(mx);
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = cmul(v[m], mxp[t + TPT * m]);
+      fft_full<N, TPT, +1>(v, xl, tw, t);
     }
-    fft_full<N, TPT, +1>(v, xl, tw, t);
     if (active) {
       float2* dst = ws + ((size_t)b * T + i) * (size_t)(N / 4) * n * 4;
 #pragma unroll
@@ -201,6 +202,73 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Focal-plane camera (Pyramid.py:638-642, 1183-1190): sum over points of
+// |fft2(E_i phasor)|^2 / N^2 on the crop x crop window centred at N/2. The
+// rows pass in forward-only mode leaves the row-transformed fields in the
+// tiled intermediate; here each lane group transforms one column inside the
+// window for a chunk of points and keeps the window's rows. (|.|^2 removes the
+// zero-padding offset phase, so no mask or shift is applied.)
+// part[b][chunk][crop][crop]
+// ---------------------------------------------------------------------------
+template <int N, int TPT>
+__global__ void __launch_bounds__(64 * kWaves)
+k_cols_focal(const float2* __restrict__ ws, const float2* __restrict__ tw, float* __restrict__ part, int B, int T,
+             int nchunk, int crop) {
+  constexpr int n = N / 4;
+  constexpr int E = N / TPT;
+  constexpr int G = 64 / TPT;
+  constexpr int NG = N / 4;
+  __shared__ float2 lds[kWaves * G][lds_len(N)];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int h = lane / TPT;
+  const int t = lane % TPT;
+  const long total = (long)B * nchunk * crop;
+  long task = ((long)blockIdx.x * kWaves + wave) * G + h;
+  const bool active = task < total;
+  if (!active) task = total - 1;
+  const int xc = (int)(task % crop);
+  const int chunk = (int)((task / crop) % nchunk);
+  const int b = (int)(task / ((long)crop * nchunk));
+  const int x0 = N / 2 - crop / 2;
+  const int X = x0 + xc;
+  const int i0 = (int)(((long)chunk * T) / nchunk);
+  const int i1 = (int)(((long)(chunk + 1) * T) / nchunk);
+  float2* xl = lds[wave * G + h];
+  float acc[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) acc[m] = 0.f;
+  for (int i = i0; i < i1; ++i) {
+    const float2* tile = ws + (((size_t)b * T + i) * NG + (X >> 2)) * (size_t)n * 4;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + (X & 3)];
+    fft_fwd_pruned<N, TPT, 4>(v, xl, tw, t);
+#pragma unroll
+    for (int m = 0; m < E; ++m) acc[m] += cabs2(v[m]);
+  }
+  if (!active) return;
+  const float inv = 1.0f / ((float)N * (float)N);
+  float* dst = part + ((size_t)b * nchunk + chunk) * (size_t)crop * crop;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int Y = t + TPT * m - x0;
+    if (Y >= 0 && Y < crop) dst[(size_t)Y * crop + xc] = acc[m] * inv;
+  }
+}
+
+// out[b][p] = sum_k part[b][k][p], p < count  (fixed summation order)
+__global__ void k_sum_chunks(const float* __restrict__ part, float* __restrict__ out, int B, int nchunk, int count) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)B * count) return;
+  const int p = (int)(idx % count);
+  const int b = (int)(idx / count);
+  float s = 0.f;
+  for (int k = 0; k < nchunk; ++k) s += part[((size_t)b * nchunk + k) * count + p];
+  out[idx] = s;
+}
+
 // frame[b][r][g] = sum_k part[b][k][g][r]  (fixed summation order)
 __global__ void k_reduce_chunks(const float* __restrict__ part, float* __restrict__ frame, int B,
                                 int nchunk, int cam) {
@@ -283,7 +351,7 @@ static size_t ws_part_bytes(int N, int B, int nchunk) {
   return align256((size_t)B * nchunk * (size_t)(N / 4) * (N / 4) * sizeof(float));
 }
 
-template <int N>
+template <int N, bool FWD_ONLY = false>
 static int launch_rows(const float* opd, float scale, const float* amp, const float2* tx, const float2* ty,
                        const float2* mx, int T, int B, int pts_per_task, float2* inter, hipStream_t st) {
   constexpr int TPT = fast_tpt(N);
@@ -303,7 +371,7 @@ static int launch_rows(const float* opd, float scale, const float* amp, const fl
     OOPAO_LAUNCH_CHECK();
   }
   dim3 grid((unsigned)(nrb * npc * B));
-  hipLaunchKernelGGL((k_rows_sep<N, TPT>), grid, dim3(64 * kWaves), 0, st, field, tx, ty, mx, tw,
+  hipLaunchKernelGGL((k_rows_sep<N, TPT, FWD_ONLY>), grid, dim3(64 * kWaves), 0, st, field, tx, ty, mx, tw,
                      inter, B, T, pts_per_task);
   OOPAO_LAUNCH_CHECK();
   return OK;
@@ -329,6 +397,26 @@ static int launch_cols(const float2* inter, const float2* my, int T, int B, int 
                        B, nchunk, cam);
     OOPAO_LAUNCH_CHECK();
   }
+  return OK;
+}
+
+template <int N>
+static int launch_focal(const float2* inter, int T, int B, int nchunk, int crop, float* part, float* out,
+                        hipStream_t st) {
+  constexpr int TPT = fast_tpt(N);
+  constexpr int G = 64 / TPT;
+  const float2* tw = nullptr;
+  int rc = twiddles(N, &tw);
+  if (rc) return rc;
+  const long groups = (long)B * nchunk * crop;
+  const long waves = (groups + G - 1) / G;
+  hipLaunchKernelGGL((k_cols_focal<N, TPT>), dim3((unsigned)((waves + kWaves - 1) / kWaves)), dim3(64 * kWaves), 0,
+                     st, inter, tw, part, B, T, nchunk, crop);
+  OOPAO_LAUNCH_CHECK();
+  const size_t total = (size_t)B * crop * crop;
+  hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, part, out, B, nchunk,
+                     crop * crop);
+  OOPAO_LAUNCH_CHECK();
   return OK;
 }
 
@@ -436,6 +524,48 @@ int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const voi
   rc = oopao_pwfs_rows(opd, scale, amp, tx, ty, mx, n, N, T, B, pts_per_task, inter, stream);
   if (rc) return rc;
   return oopao_pwfs_cols(inter, my, n, N, T, B, nchunk, part, frames, stream);
+}
+
+int oopao_pwfs_focal_bytes(int n, int N, int T, int B, int nchunk, int crop, size_t* bytes) {
+  OOPAO_CHECK_ARG(bytes != nullptr, "bytes is null");
+  OOPAO_CHECK_ARG(n > 0 && N > 0 && T > 0 && B > 0 && nchunk > 0 && nchunk <= T && crop > 0 && crop <= N,
+                  "invalid sizes n=%d N=%d T=%d B=%d nchunk=%d crop=%d", n, N, T, B, nchunk, crop);
+  *bytes = ws_inter_bytes(n, N, T, B) + align256((size_t)B * nchunk * crop * crop * sizeof(float));
+  return OK;
+}
+
+int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void* tx, const void* ty, int n, int N,
+                     int T, int B, int nchunk, int pts_per_task, int crop, void* workspace, size_t ws_bytes,
+                     float* out, void* stream) {
+  OOPAO_CHECK_ARG(opd && amp && tx && ty && workspace && out, "null pointer argument");
+  OOPAO_CHECK_ARG(T > 0 && B > 0 && nchunk > 0 && nchunk <= T && pts_per_task > 0 && crop > 0 && crop <= N,
+                  "invalid T=%d B=%d nchunk=%d pts_per_task=%d crop=%d", T, B, nchunk, pts_per_task, crop);
+  int rc = check_geometry(n, N, 4);
+  if (rc) return rc;
+  size_t need = 0;
+  oopao_pwfs_focal_bytes(n, N, T, B, nchunk, crop, &need);
+  if (ws_bytes < need) {
+    set_error("workspace too small: %zu < %zu bytes", ws_bytes, need);
+    return ERR_WORKSPACE;
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const float2* txp = reinterpret_cast<const float2*>(tx);
+  const float2* typ = reinterpret_cast<const float2*>(ty);
+  float2* inter = reinterpret_cast<float2*>(workspace);
+  float* part = reinterpret_cast<float*>(reinterpret_cast<char*>(workspace) + ws_inter_bytes(n, N, T, B));
+  switch (N) {
+#define OOPAO_CASE(NN)                                                                                     \
+  case NN:                                                                                                 \
+    rc = launch_rows<NN, true>(opd, scale, amp, txp, typ, txp, T, B, pts_per_task, inter, st);             \
+    if (rc) return rc;                                                                                     \
+    return launch_focal<NN>(inter, T, B, nchunk, crop, part, out, st);
+    OOPAO_FAST_SIZES(OOPAO_CASE)
+#undef OOPAO_CASE
+    default:
+      break;
+  }
+  set_error("unsupported padded length N=%d", N);
+  return ERR_UNSUPPORTED;
 }
 
 int oopao_pwfs_slopes(const float* frames, int B, int cam, const unsigned char* valid, int w,

@@ -129,6 +129,30 @@ class PWFSEngine:
                   nchunk, ppt, _ptr(ws), ctypes.c_size_t(nb), _ptr(out), _stream_ptr(self.device))
         return out
 
+    def focal(self, opd, scale, amp, crop):
+        """Modulation focal-plane intensity [B, crop, crop]: sum over points of |fft2(E_i)|^2 / N^2 on
+        the window starting at N/2 - crop/2 (Pyramid.py:638-642, 1183-1190)."""
+        if self.T == 0:
+            raise OopaoError('modulation tables not set')
+        opd = opd.reshape(-1, self.n, self.n)
+        if opd.dtype != torch.float32 or not opd.is_contiguous() or opd.device != self.device:
+            raise OopaoError('opd must be a contiguous float32 tensor on %s' % self.device)
+        if amp.dtype != torch.float32 or tuple(amp.shape) != (self.n, self.n) or not amp.is_contiguous():
+            raise OopaoError('amp must be a contiguous float32 [n, n] tensor')
+        crop = int(crop)
+        if not 0 < crop <= self.N:
+            raise OopaoError('focal window %d outside 1..%d' % (crop, self.N))
+        B = opd.shape[0]
+        nchunk, ppt = self._knobs(B)
+        nb = ctypes.c_size_t(0)
+        _lib.call('oopao_pwfs_focal_bytes', self.n, self.N, self.T, B, nchunk, crop, ctypes.byref(nb))
+        if self._ws is None or self._ws.numel() < nb.value:
+            self._ws = torch.empty(nb.value, dtype=torch.uint8, device=self.device)
+        out = torch.empty((B, crop, crop), dtype=torch.float32, device=self.device)
+        _lib.call('oopao_pwfs_focal', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx), _ptr(self.ty),
+                  self.n, self.N, self.T, B, nchunk, ppt, crop, _ptr(self._ws), ctypes.c_size_t(nb.value),
+                  _ptr(out), _stream_ptr(self.device))
+        return out
 
     # split passes (same result as frames(); used to time the passes separately)
     def buffers(self, B):

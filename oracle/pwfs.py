@@ -311,6 +311,24 @@ class PyramidOracle:
             maps = [one(i) for i in range(self.nTheta)]
         return np.sum(np.asarray(maps), axis=0)
 
+    def focal(self, phase, resolution):
+        """Modulation focal-plane camera frame (Pyramid.py:642, 1183-1190): sum over the points of
+        |fft2(support phasor) / N|^2, centre-cropped to ``resolution`` (whole field if larger than N)."""
+        amp = self._amp()
+        rdt = np.float64 if self.precision == 'fp64' else np.float32
+        ph = np.asarray(phase, dtype=rdt)
+        tts = [0] if self.tt is None else [self.tt[i] for i in range(self.nTheta)]
+        inten = 0
+        for tt in tts:
+            ft = pyramid_transform(amp, ph + tt, self.mask, self.phasor, self.geom, self.precision,
+                                   self.psfCentering)[1]
+            inten = inten + np.abs(ft) ** 2
+        N = inten.shape[0]
+        if resolution > N:
+            return inten
+        return inten[N // 2 - resolution // 2:N // 2 + resolution // 2,
+                     N // 2 - resolution // 2:N // 2 + resolution // 2]
+
     def frame(self, phase):
         """Detector frame: sum-binning, QE = gain = 1, float64 (Pyramid.py:1193-1203, Detector.py:273-354)."""
         raw = self.raw(phase)

@@ -80,3 +80,22 @@ def test_chain_signal_matches_oracle():
     for j in range(3):
         s_ref = o.measure_opd(od.opd(od.modes(), C[:, j]))[1]
         assert rel_l2(wfs.signal[:, j], s_ref) <= 1e-5
+
+
+def test_focal_plane_camera_matches_oracle():
+    # wfs * wfs.focal_plane_camera (Pyramid.py:1178-1203): resolution (3*4+12)*zpf = 96 px
+    from oracle.dm import DMOracle
+    tel, ngs, dm, wfs = _config1_objects()
+    o = config1_system()
+    od = DMOracle(80, 8, 20, 0.35)
+    coefs = np.random.RandomState(5).normal(0, 30e-9, 357)
+    dm.coefs = coefs
+    ngs ** tel * dm * wfs
+    wfs * wfs.focal_plane_camera
+    fr = wfs.focal_plane_camera.frame
+    assert fr.shape == (96, 96)
+    opd = od.opd(od.modes(), coefs)
+    ref = o.focal(opd * o.pupil * 2 * np.pi / o.wavelength, 96)
+    err = rel_l2(fr, ref)
+    print('focal camera rel_l2 vs fp64 oracle', err)
+    assert err <= 1e-5

@@ -79,3 +79,22 @@ def test_slopes_match_oracle(builder):
     err32 = rel_l2(s.cpu().numpy()[0], o32.measure(ph)[1])
     print('slopes rel_l2 vs fp32 oracle', err32)
     assert err32 <= 1e-5
+
+
+@pytest.mark.parametrize('builder,crop', [(small_system, 20), (small_system, 64), (config1_system, 60),
+                                          (config2_system, 96)])
+def test_focal_camera_matches_oracle(builder, crop):
+    # modulation focal-plane camera (Pyramid.py:642, 1183-1190), rel-L2 <= 1e-5 vs fp64 oracle
+    import torch
+    o = builder()
+    eng, amp = _engine_for(o)
+    ph = np.stack([random_phase(o.geom.n, 7, 1.0, o.pupil), random_phase(o.geom.n, 8, 0.5, o.pupil)])
+    got = eng.focal(torch.from_numpy(ph.astype(np.float32)).to(amp.device).contiguous(), 1.0, amp, crop)
+    torch.cuda.synchronize()
+    got = got.cpu().numpy()
+    assert got.shape == (2, crop, crop)
+    for b in range(2):
+        ref = o.focal(ph[b], crop)
+        err = rel_l2(got[b], ref)
+        print('focal', b, 'crop', crop, 'rel_l2 vs fp64 oracle', err)
+        assert err <= 1e-5

@@ -145,3 +145,28 @@ def test_fp32_mode_close_to_fp64():
     o32 = small_system(precision='fp32')
     ph = random_phase(o64.geom.n, 6, 1.0, o64.pupil)
     assert rel_l2(o32.frame(ph), o64.frame(ph)) < 1e-5
+
+
+def test_focal_camera_parseval_and_centring():
+    # Pyramid.py:642, 1183-1190. Parseval: the whole focal field holds the pupil flux; the
+    # PSF-centring phasor puts the zero frequency at (N-1)/2, so a real field (flat phase) gives an
+    # image point-symmetric about that point, and any even centred crop inherits the symmetry.
+    o = small_system()
+    N = o.geom.N
+    ph = random_phase(o.geom.n, 6, 1.0, o.pupil)
+    full = o.focal(ph, N + 1)
+    assert full.shape == (N, N)
+    assert abs(full.sum() / o.flux.sum() - 1) < 1e-9
+    flat = o.focal(np.zeros_like(ph), N + 1)
+    assert rel_l2(flat, flat[::-1, ::-1]) < 1e-12
+    # the four central pixels are the maxima of the modulated ring's symmetric image
+    crop = o.focal(np.zeros_like(ph), 20)
+    assert crop.shape == (20, 20)
+    assert rel_l2(crop, crop[::-1, ::-1]) < 1e-12
+    assert rel_l2(crop, flat[N // 2 - 10:N // 2 + 10, N // 2 - 10:N // 2 + 10]) == 0
+    # a single unmodulated point: the PSF peak sits on the four pixels around (N-1)/2
+    o0 = small_system(modulation=0, T=None)
+    f0 = o0.focal(np.zeros_like(ph), N + 1)
+    peak = np.unravel_index(np.argmax(f0), f0.shape)
+    assert peak[0] in (N // 2 - 1, N // 2) and peak[1] in (N // 2 - 1, N // 2)
+    assert abs(f0[N // 2 - 1, N // 2 - 1] / f0[N // 2, N // 2] - 1) < 1e-9
