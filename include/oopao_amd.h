@@ -88,6 +88,28 @@ int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void
                      int N, int T, int B, int nchunk, int pts_per_task, int crop, void* workspace,
                      size_t ws_bytes, float* out, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * General PWFS path: any padded length N <= 2048, any N x N mask (rooftop,
+ * rotation, per-quadrant shifts, spatial filter: Pyramid.py:368-480, 1051-1085),
+ * any integer detector binning, and the unbinned raw frame (raw_data,
+ * Pyramid.py:751-757). Same per-point field tables tx, ty as the fast path
+ * (the pupil sits at offset N/2 - n/2 of the padded support, Pyramid.py:623-625).
+ *   maskT [N][N] complex64: the mask TRANSPOSED (maskT[kx][ky] = mask[ky][kx])
+ *         and divided by N^2 (both inverse normalisations)
+ *   frames [B][N/bin][N/bin] (may be null), raw [B][N][N] (may be null; not
+ *         both) float32.
+ * ------------------------------------------------------------------------- */
+int oopao_pwfs_generic_supported(int n, int N, int bin);
+int oopao_pwfs_generic_bytes(int n, int N, int T, int B, int nchunk, size_t* bytes);
+int oopao_pwfs_generic_frames(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
+                              const void* maskT, int n, int N, int T, int B, int bin, int nchunk,
+                              void* workspace, size_t ws_bytes, float* frames, float* raw, void* stream);
+/* Focal-plane camera on the general path (same output as oopao_pwfs_focal). */
+int oopao_pwfs_generic_focal_bytes(int n, int N, int T, int B, int nchunk, int crop, size_t* bytes);
+int oopao_pwfs_generic_focal(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
+                             int n, int N, int T, int B, int nchunk, int crop, void* workspace, size_t ws_bytes,
+                             float* out, void* stream);
+
 /* slopesMaps post-processing (Pyramid.py:835-854, grabQuadrant 946-961):
  *   frames [B][cam][cam] float32, valid [w][w] uint8 (validI4Q),
  *   quad[8] int32 (row origins of quadrants 1..4, then column origins),

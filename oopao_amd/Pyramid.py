@@ -52,12 +52,10 @@ class Pyramid:
         self.psfCentering = psfCentering
         self.binning = binning
         self.old_mask = old_mask
-        if old_mask:
-            raise OopaoError('old_mask=True is not supported by oopao_amd')
         self.user_modulation_path = user_modulation_path
         self.modulation_length = 2 * np.pi
         self.pupilSeparationRatio = pupilSeparationRatio
-        self.weight_vector = None
+        self._weight_vector = None
         if edgePixel is not None:
             raise OopaoError('The use of the edgePixel property has been deprecated. Consider using n_pix_edge instead')
         if pupilSeparationRatio is not None:
@@ -113,6 +111,8 @@ class Pyramid:
                                  pts_per_task=pts_per_task)
         # (opd, wavelength, amplitude) of the last measurement, for the focal-plane camera
         self._last_opd = None
+        self._last_weighted = False
+        self._raw_cache = None
         self.mask_computation()
         self._setup_slopes()
         self.slopesUnits = 1
@@ -151,23 +151,72 @@ class Pyramid:
         P4 = xr * r + x_ * sx[3] - yr * r - y_ * sy[3]
         return -np.max(np.stack([P1, P2, P3, P4]) * norma, axis=0)
 
+    def get_phase_mask_old(self, resolution, n_subap, n_pix_separation, n_pix_edge, psf_centering=False,
+                           sx=[0, 0, 0, 0], sy=[0, 0, 0, 0]):
+        """Previous pyramid mask: one zero-mean tip/tilt ramp per quadrant (Pyramid.py:482-539)."""
+        n_tot = int((n_subap * 2 + n_pix_separation + n_pix_edge * 2) * self.telescope.resolution / self.nSubap)
+        norma = (self.telescope.resolution / self.nSubap) / 4
+        h = n_tot // 2
+        L = n_subap + n_pix_separation
+        m = np.zeros([n_tot, n_tot])
+
+        def ramps(nx, ny, limx, limy, endx, endy):
+            tip, tilt = np.meshgrid(np.linspace(-limx, limx, nx, endpoint=endx),
+                                    np.linspace(-limy, limy, ny, endpoint=endy))
+            return tip - tip.mean(), tilt - tilt.mean()
+        if psf_centering:
+            tip, tilt = ramps(h, h, np.pi, np.pi, False, False)
+            m[:h, :h] = (tip * (L + sx[0]) + tilt * (L - sy[0])) * norma
+            m[:h, -h:] = (-tip * (L - sx[1]) + tilt * (L - sy[1])) * norma
+            m[-h:, -h:] = (-tip * (L - sx[2]) - tilt * (L + sy[2])) * norma
+            m[-h:, :h] = (tip * (L + sx[3]) - tilt * (L + sy[3])) * norma
+        else:
+            lp = np.pi
+            lm = np.pi - 2 * np.pi / n_tot
+            tip1, tilt1 = ramps(h + 1, h + 1, lp, lp, True, True)
+            tip2, tilt2 = ramps(h + 1, h - 1, lp, lm, True, False)
+            tip3, tilt3 = ramps(h - 1, h - 1, lm, lm, False, False)
+            tip4, tilt4 = ramps(h - 1, h + 1, lm, lp, False, True)
+            m[:h + 1, :h + 1] = (tip1 * (L + sx[0]) + tilt1 * (L - sy[0])) * norma
+            m[:h + 1, -h + 1:] = (-tip4 * (L - sx[1]) + tilt4 * (L - sy[1])) * norma
+            m[-h + 1:, -h + 1:] = (-tip3 * (L - sx[2]) - tilt3 * (L + sy[2])) * norma
+            m[-h + 1:, :h + 1] = (tip2 * (L + sx[3]) - tilt2 * (L + sy[3])) * norma
+        return -m
+
+    def _phase_mask(self, sx, sy):
+        f = self.get_phase_mask_old if self.old_mask else self.get_phase_mask
+        return f(self.resolution, self.nSubap, self.n_pix_separation, self.n_pix_edge, self.psfCentering, sx, sy)
+
+    def _apply_mask(self):
+        """Send the current mask to the HIP engine. With a spatial filter the mask holds it
+        (mask = initial_mask * filter, Pyramid.py:1072) and, without PSF centring, the focal field is
+        multiplied by it once more before the mask (Pyramid.py:633-636)."""
+        sf = self._spatialFilter
+        if sf is None:
+            self.engine.set_mask(self.m, self.mask)
+        else:
+            factor = None if self.psfCentering else np.asarray(sf)
+            self.engine.set_mask(self.m, self.mask, factor=factor, general=True)
+
     def mask_computation(self):
+        """Pyramid.mask_computation (Pyramid.py:368-391)."""
         print('Pyramid Mask initialization...')
-        self.m = self.get_phase_mask(self.resolution, self.nSubap, self.n_pix_separation, self.n_pix_edge,
-                                     self.psfCentering, self.sx, self.sy)
+        self.m = self._phase_mask(self.sx, self.sy)
         self.initial_m = self.m.copy()
         self.mask = np.complex64(np.exp(1j * self.m))
         self.initial_mask = np.copy(self.mask)
-        self.engine.set_mask(self.m)
+        self._apply_mask()
 
     def apply_shift_wfs(self, sx=None, sy=None, mis_reg=None, units='pixels'):
-        """Per-quadrant pupil shifts (Pyramid.py:393-451). Uniform shifts keep the mask separable."""
+        """Per-quadrant pupil shifts (Pyramid.py:393-451). Uniform shifts keep the mask separable;
+        others run the general HIP path."""
         sx = 0 if sx is None else sx
         sy = 0 if sy is None else sy
         if mis_reg is not None:
             sx = [mis_reg.dX_1, mis_reg.dX_2, mis_reg.dX_3, mis_reg.dX_4]
             sy = [mis_reg.dY_1, mis_reg.dY_2, mis_reg.dY_3, mis_reg.dY_4]
-        factor = 1 if units == 'pixels' else 1 / (self.telescope.D / self.nSubap)
+        f = 2 if self.old_mask else 1     # the previous mask's shift convention (Pyramid.py:401-405)
+        factor = f if units == 'pixels' else f / (self.telescope.D / self.nSubap)
         if np.isscalar(sx) and np.isscalar(sy):
             shift_x = [factor * sx] * 4
             shift_y = [factor * sy] * 4
@@ -182,10 +231,9 @@ class Pyramid:
                     'Wrapping of the signal is currently occuring!!')
         self.sx = np.asarray(shift_x) / factor
         self.sy = np.asarray(shift_y) / factor
-        self.m = self.get_phase_mask(self.resolution, self.nSubap, self.n_pix_separation, self.n_pix_edge,
-                                     self.psfCentering, shift_x, shift_y)
+        self.m = self._phase_mask(shift_x, shift_y)
         self.mask = np.complex64(np.exp(1j * self.m))
-        self.engine.set_mask(self.m)
+        self._apply_mask()
         self.slopesUnits = 1
         self.referenceSignal = 0
         self._referenceSignal_2D = None
@@ -357,6 +405,7 @@ class Pyramid:
         else:
             self.src = src
         self.relay(self.src)
+        self._ref_frame = (self._last_opd, self._last_weighted)
         self._sync_reference()
         s2d, s = self._signal_processing_dev()
         self._referenceSignal_2D = s2d[0] if s2d.dim() == 3 else s2d
@@ -406,20 +455,21 @@ class Pyramid:
         return torch.sqrt(self.src._intensity / self.nTheta).contiguous()
 
     def _batch_size(self, B):
-        per = self.engine.N * self.engine.n * 8 * max(self.nTheta, 1)
+        per = max(1, self.engine.bytes_per_screen())
         return max(1, min(B, int(self.max_batch_bytes // per)))
 
-    def frames_dev(self, opd):
-        """Detector frames for device OPD maps [B, n, n] (metres) of the current source."""
+    def frames_dev(self, opd, weighted=False):
+        """Detector frames for device OPD maps [B, n, n] (metres) of the current source. ``weighted``
+        applies weight_vector to the modulation points (the 2-D measurement, Pyramid.py:750-755)."""
         amp = self._amp()
         scale = 2 * np.pi / self.src.wavelength
         B = opd.shape[0]
         bs = self._batch_size(B)
         if bs >= B:
-            return self.engine.frames(opd.contiguous(), scale, amp)
+            return self.engine.frames(opd.contiguous(), scale, amp, weighted=weighted)
         out = torch.empty((B, self.engine.cam, self.engine.cam), dtype=torch.float32, device=self.device)
         for j in range(0, B, bs):
-            self.engine.frames(opd[j:j + bs].contiguous(), scale, amp, out=out[j:j + bs])
+            self.engine.frames(opd[j:j + bs].contiguous(), scale, amp, out=out[j:j + bs], weighted=weighted)
         return out
 
     def wfs_measure(self, phase_in=None, integrate=True):
@@ -431,12 +481,13 @@ class Pyramid:
                                  're-calibrate with the correct source.')
         if phase_in is not None:
             self.src.phase = phase_in
-        if self.spatialFilter is not None:
-            raise OopaoError('spatial filters are not supported by oopao_amd')
         opd = self.src._OPD
         batched = opd.dim() == 3
-        frames = self.frames_dev(opd if batched else opd[None])
+        weighted = (not batched) and self._weight_vector is not None
+        frames = self.frames_dev(opd if batched else opd[None], weighted=weighted)
         self._last_opd = (opd if batched else opd[None], self.src.wavelength, self._amp())
+        self._last_weighted = weighted
+        self._raw_cache = None
         if self.binning != 1:
             b = self.binning
             c = frames.shape[-1]
@@ -557,9 +608,63 @@ class Pyramid:
 
     @spatialFilter.setter
     def spatialFilter(self, val):
-        if val is not None:
-            raise OopaoError('spatial filters are not supported by oopao_amd')
-        self._spatialFilter = None
+        """Spatial filter setter (Pyramid.py:1055-1085): the mask becomes initial_mask * filter."""
+        self._spatialFilter = val
+        if not self.isInitialized:
+            return
+        if val is None:
+            print('No spatial filter considered')
+            self.mask = self.initial_mask
+        elif np.shape(val) == self.mask.shape:
+            print('A spatial filter is now considered')
+            self.mask = self.initial_mask * np.asarray(val)
+        else:
+            warning('Wrong shape for the spatial filter. No spatial filter attached to the mask')
+            self.mask = self.initial_mask
+        self._apply_mask()
+        if self.isCalibrated:
+            print('Updating the reference slopes and Wavelength Calibration for the new modulation...')
+            self.slopesUnits = 1
+            self.referenceSignal = 0
+            self._referenceSignal_2D = None
+            self.wfs_calibration()
+            print('Done!')
+
+    def _spatial_filter_fields(self):
+        """Focal field of the unmodulated source and its filtered pupil-plane image (the diagnostics
+        em_field_spatial_filter / pupil_plane_spatial_filter of Pyramid.py:676-684), on the device."""
+        if self._spatialFilter is None or self._last_opd is None:
+            return None, None
+        opd, wavelength, amp = self._last_opd
+        n, N = self.engine.n, self.engine.N
+        c0 = N // 2 - n // 2
+        field = amp * torch.exp(1j * opd[0] * (2 * np.pi / wavelength))
+        support = torch.zeros((N, N), dtype=torch.complex64, device=self.device)
+        support[c0:c0 + n, c0:c0 + n] = field
+        xx = torch.arange(N, device=self.device, dtype=torch.float64)
+        ph = torch.exp(-1j * np.pi * (N + 1) / N * (xx[None, :] + xx[:, None])).to(torch.complex64)
+        em = torch.fft.fft2(support * ph)
+        sf = torch.as_tensor(np.asarray(self._spatialFilter), device=self.device).to(torch.complex64)
+        return em, torch.fft.ifft2(em * sf)
+
+    @property
+    def em_field_spatial_filter(self):
+        return to_host(self._spatial_filter_fields()[0])
+
+    @property
+    def pupil_plane_spatial_filter(self):
+        return to_host(self._spatial_filter_fields()[1])
+
+    @property
+    def weight_vector(self):
+        return self._weight_vector
+
+    @weight_vector.setter
+    def weight_vector(self, val):
+        """Per-point weights of the 2-D measurement: raw = sum_i w_i I_i / nTheta (Pyramid.py:750-755)."""
+        self._weight_vector = None if val is None else np.asarray(val, np.float64).reshape(-1)
+        if hasattr(self, 'engine'):
+            self.engine.set_weights(self._weight_vector)
 
     # ---------------------------------------------------------------- relay
     def relay(self, src):
@@ -618,9 +723,36 @@ class Pyramid:
         else:
             raise OopaoError('Error light propagated to the wrong type of object')
 
+    def _raw_dev(self, args, weighted):
+        opd, wavelength, amp = args
+        return self.engine.raw(opd[-1:].contiguous(), 2 * np.pi / wavelength, amp, weighted=weighted)[0]
+
     @property
     def raw_data(self):
-        raise OopaoError('the unbinned raw frame is not produced by the fused HIP path; use cam.frame')
+        """Unbinned modulated frame [N, N] of the last measurement (of its last screen when batched),
+        Pyramid.py:751-757, 820-821; computed on demand by the general HIP path."""
+        if self._last_opd is None:
+            return None
+        if self._raw_cache is None:
+            self._raw_cache = self._raw_dev(self._last_opd, self._last_weighted)
+        return to_host(self._raw_cache)
+
+    @property
+    def pyramidFrame(self):
+        return self.raw_data
+
+    @property
+    def referencePyramidFrame(self):
+        """Unbinned frame of the calibration wavefront (Pyramid.py:604), computed on first access."""
+        v = self._ref_frame
+        if isinstance(v, tuple):
+            self._ref_frame = to_host(self._raw_dev(*v))
+            v = self._ref_frame
+        return v
+
+    @referencePyramidFrame.setter
+    def referencePyramidFrame(self, val):
+        self._ref_frame = val
 
     def print_properties(self):
         print(self)

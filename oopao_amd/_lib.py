@@ -38,6 +38,15 @@ SIGNATURES = {
     'oopao_pwfs_focal_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_focal': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
                                  c_int, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p]),
+    'oopao_pwfs_generic_supported': (c_int, [c_int, c_int, c_int]),
+    'oopao_pwfs_generic_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
+    'oopao_pwfs_generic_frames': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                          c_int, c_int, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p,
+                                          c_void_p]),
+    'oopao_pwfs_generic_focal_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, c_int,
+                                               ctypes.POINTER(c_size_t)]),
+    'oopao_pwfs_generic_focal': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                                         c_int, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p]),
     'oopao_atm_add_row':(c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'oopao_atm_minmax': (c_int, [c_void_p, c_int, c_void_p, c_void_p]),

@@ -12,6 +12,12 @@ float64 (rounded once to complex64):
   Pyramid.py:453-480) gives mx[l] = exp(i b[l]) e^{-2 pi i l c0/N} / N and
   my[k] = exp(i a[k]) e^{-2 pi i k c0/N} / N, folding the zero-padding offset c0
   and the 1/N of each inverse transform.
+
+Sensors outside that fast path (a non-separable mask: rooftop, rotation,
+per-quadrant shifts, spatial filter; a padded length other than 4n; a binning
+other than 4 x 4) run the general three-pass path of the library with the full
+N x N complex64 mask (divided by N^2, stored column-major) and the pupil placed
+at its offset in the padded support; tx and ty are the same tables.
 """
 import ctypes
 
@@ -48,7 +54,8 @@ def modulation_phasors(path, tip_lin, tilt_lin, N, n, psfCentering=True):
     if psfCentering:
         ph = np.exp(-1j * np.pi * (N + 1) / N * X)
     else:
-        ph = (-1.0) ** X
+        # fftshift of the spectrum = a phase ramp of N//2 cycles on the support ((-1)^X for even N)
+        ph = np.exp(2j * np.pi * X * (N // 2) / N)
     tx = np.exp(1j * path[:, 0:1] * tip_lin[None, :]) * ph[None, :]
     ty = np.exp(1j * path[:, 1:2] * tilt_lin[None, :]) * ph[None, :]
     return tx, ty
@@ -69,31 +76,79 @@ class PWFSEngine:
         self.device = torch.device('cuda', torch.cuda.current_device()) if device is None else torch.device(device)
         self.n, self.N, self.cam = n, N, cam
         self.bin = N // cam if cam and N % cam == 0 else 0
-        if not _lib.lib().oopao_pwfs_supported(n, N, self.bin):
+        L = _lib.lib()
+        self.fast_geometry = bool(L.oopao_pwfs_supported(n, N, self.bin))
+        if not self.fast_geometry and not L.oopao_pwfs_generic_supported(n, N, self.bin):
             raise OopaoError('PWFS geometry n=%d N=%d binning=%d is not supported by the HIP path '
-                             '(needs N = 4 n, 4 x 4 detector binning and a compiled N)'
-                             % (n, N, self.bin))
+                             '(needs n <= N <= 2048 and a detector size dividing N)' % (n, N, self.bin))
+        self.mode = 'fast' if self.fast_geometry else 'generic'
         self.nchunk_user = nchunk
         self.pts_per_task_user = pts_per_task
         self._ws = None
         self.T = 0
+        self.maskT = None
+        self._fullT = None
 
     # ------------------------------------------------------------------ tables
     def set_modulation(self, path, tip_lin, tilt_lin, psfCentering=True):
         tx, ty = modulation_phasors(np.asarray(path, np.float64), tip_lin, tilt_lin, self.N, self.n,
                                     psfCentering)
         self.T = tx.shape[0]
+        self._tx_host = tx
         self.tx = torch.from_numpy(tx.astype(np.complex64)).to(self.device)
         self.ty = torch.from_numpy(ty.astype(np.complex64)).to(self.device)
+        self._make_weighted()
 
-    def set_mask(self, m):
-        f = separable_mask_factors(np.asarray(m, np.float64))
-        if f is None:
-            raise OopaoError('the pyramid mask is not separable (rooftop, rotation or per-quadrant '
-                             'shifts); only separable masks are supported by the HIP path')
-        my, mx = mask_vectors(f[0], f[1], self.N, self.n)
-        self.my = torch.from_numpy(my.astype(np.complex64)).to(self.device)
-        self.mx = torch.from_numpy(mx.astype(np.complex64)).to(self.device)
+    def set_weights(self, w):
+        """Per-point weights: frames computed with weighted=True hold sum_i w_i I_i / T
+        (Pyramid.py:750-755), by scaling tx_i with sqrt(w_i / T)."""
+        self._weights = None if w is None else np.asarray(w, np.float64).reshape(-1)
+        self._make_weighted()
+
+    def _make_weighted(self):
+        w = getattr(self, '_weights', None)
+        self.txw = None
+        if w is None or self.T == 0:
+            return
+        if w.size != self.T:
+            return      # reported when a weighted frame is asked for
+        if np.any(w < 0):
+            raise OopaoError('negative modulation weights are not supported')
+        txw = self._tx_host * np.sqrt(w / self.T)[:, None]
+        self.txw = torch.from_numpy(txw.astype(np.complex64)).to(self.device)
+
+    def _tx(self, weighted):
+        w = getattr(self, '_weights', None)
+        if not weighted or w is None:
+            return self.tx
+        if self.txw is None:
+            raise OopaoError('weight_vector has %d entries for %d modulation points' % (w.size, self.T))
+        return self.txw
+
+    def set_mask(self, m, mask=None, factor=None, general=False):
+        """Pyramid mask from its phase m [N, N] (and the complex64 mask exp(i m) the reference
+        stores, Pyramid.py:388). ``factor`` multiplies the mask in the focal plane (a spatial filter);
+        ``general`` forces the full mask (when ``mask`` is not exp(i m)). A separable phase on a fast-path
+        geometry selects the two-pass kernels; anything else the general path with the full mask."""
+        m = np.asarray(m, np.float64)
+        self._fullT = None
+        f = separable_mask_factors(m) if (self.fast_geometry and factor is None and not general) else None
+        if f is not None:
+            my, mx = mask_vectors(f[0], f[1], self.N, self.n)
+            self.my = torch.from_numpy(my.astype(np.complex64)).to(self.device)
+            self.mx = torch.from_numpy(mx.astype(np.complex64)).to(self.device)
+            self.mode = 'fast'
+            self.maskT = None
+            return
+        full = np.complex64(np.exp(1j * m)) if mask is None else np.asarray(mask)
+        full = full.astype(np.complex128)
+        if factor is not None:
+            full = full * np.asarray(factor)
+        full = full / float(self.N) ** 2
+        self.maskT = torch.from_numpy(np.ascontiguousarray(full.T).astype(np.complex64)).to(self.device)
+        self.mode = 'generic'
+        if self.bin == 0:
+            raise OopaoError('detector size %d does not divide N=%d' % (self.cam, self.N))
 
     # -------------------------------------------------------------- execution
     def _knobs(self, B):
@@ -103,15 +158,7 @@ class PWFSEngine:
         ppt = self.pts_per_task_user or max(1, min(self.T, 6))
         return min(nchunk, self.T), ppt
 
-    def _workspace(self, B, nchunk):
-        nb = ctypes.c_size_t(0)
-        _lib.call('oopao_pwfs_workspace_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
-        if self._ws is None or self._ws.numel() < nb.value:
-            self._ws = torch.empty(nb.value, dtype=torch.uint8, device=self.device)
-        return self._ws, nb.value
-
-    def frames(self, opd, scale, amp, out=None):
-        """Detector frames [B, cam, cam] float32 for OPD/phase maps opd [B, n, n] (phase = opd*scale)."""
+    def _check(self, opd, amp):
         if self.T == 0:
             raise OopaoError('modulation tables not set')
         opd = opd.reshape(-1, self.n, self.n)
@@ -119,39 +166,115 @@ class PWFSEngine:
             raise OopaoError('opd must be a contiguous float32 tensor on %s' % self.device)
         if amp.dtype != torch.float32 or tuple(amp.shape) != (self.n, self.n) or not amp.is_contiguous():
             raise OopaoError('amp must be a contiguous float32 [n, n] tensor')
+        return opd
+
+    def _scratch(self, nbytes):
+        if self._ws is None or self._ws.numel() < nbytes:
+            self._ws = None
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    def _generic_nchunk(self, B):
+        """Point chunks of the general path: enough (screen, chunk, row) tasks to fill the GPU."""
+        if self.nchunk_user:
+            return min(self.nchunk_user, self.T)
+        return max(1, min(self.T, -(-8192 // (B * self.N))))
+
+    def bytes_per_screen(self):
+        """Workspace bytes per phase screen of the current mode (for batch sizing)."""
+        nb = ctypes.c_size_t(0)
+        if self.mode == 'fast':
+            nchunk, _ = self._knobs(1)
+            _lib.call('oopao_pwfs_workspace_bytes', self.n, self.N, self.T, 1, nchunk, ctypes.byref(nb))
+        else:
+            _lib.call('oopao_pwfs_generic_bytes', self.n, self.N, self.T, 1, self._generic_nchunk(1),
+                      ctypes.byref(nb))
+        return nb.value
+
+    def frames(self, opd, scale, amp, out=None, raw=None, weighted=False):
+        """Detector frames [B, cam, cam] float32 for OPD/phase maps opd [B, n, n] (phase = opd*scale).
+        ``raw`` (a float32 [B, N, N] tensor) also receives the unbinned frames (general path);
+        ``weighted`` uses the per-point weights of set_weights."""
+        opd = self._check(opd, amp)
         B = opd.shape[0]
-        nchunk, ppt = self._knobs(B)
-        ws, nb = self._workspace(B, nchunk)
         if out is None:
             out = torch.empty((B, self.cam, self.cam), dtype=torch.float32, device=self.device)
-        _lib.call('oopao_pwfs_frames', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx),
+        tx = self._tx(weighted)
+        if self.mode == 'generic' or raw is not None:
+            return self._frames_generic(opd, scale, amp, out, raw, tx)
+        nchunk, ppt = self._knobs(B)
+        nb = ctypes.c_size_t(0)
+        _lib.call('oopao_pwfs_workspace_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
+        ws = self._scratch(nb.value)
+        _lib.call('oopao_pwfs_frames', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(tx),
                   _ptr(self.ty), _ptr(self.mx), _ptr(self.my), self.n, self.N, self.T, B, self.bin,
-                  nchunk, ppt, _ptr(ws), ctypes.c_size_t(nb), _ptr(out), _stream_ptr(self.device))
+                  nchunk, ppt, _ptr(ws), ctypes.c_size_t(nb.value), _ptr(out), _stream_ptr(self.device))
         return out
+
+    def _full_maskT(self):
+        if self.maskT is None:
+            if self._fullT is not None:
+                return self._fullT
+            # the fast path's separable mask as a full matrix (for the unbinned frame)
+            mx = self.mx.cpu().numpy().astype(np.complex128)
+            my = self.my.cpu().numpy().astype(np.complex128)
+            c0 = self.N // 2 - self.n // 2
+            k = np.arange(self.N)
+            unshift = np.exp(2j * np.pi * k * c0 / self.N)
+            full = np.outer(my * unshift, mx * unshift)
+            self._fullT = torch.from_numpy(np.ascontiguousarray(full.T).astype(np.complex64)).to(self.device)
+            return self._fullT
+        return self.maskT
+
+    def _frames_generic(self, opd, scale, amp, out, raw, tx):
+        B = opd.shape[0]
+        if raw is not None and (raw.dtype != torch.float32 or tuple(raw.shape) != (B, self.N, self.N)
+                                or not raw.is_contiguous()):
+            raise OopaoError('raw must be a contiguous float32 [B, N, N] tensor')
+        nchunk = self._generic_nchunk(B)
+        nb = ctypes.c_size_t(0)
+        _lib.call('oopao_pwfs_generic_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
+        ws = self._scratch(nb.value)
+        maskT = self._full_maskT()
+        _lib.call('oopao_pwfs_generic_frames', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(tx),
+                  _ptr(self.ty), _ptr(maskT), self.n, self.N, self.T, B, self.bin, nchunk, _ptr(ws),
+                  ctypes.c_size_t(nb.value), _ptr(out), ctypes.c_void_p(0 if raw is None else raw.data_ptr()),
+                  _stream_ptr(self.device))
+        return out
+
+    def raw(self, opd, scale, amp, weighted=False):
+        """Unbinned modulated frames [B, N, N] (the reference's raw_data, Pyramid.py:751-757)."""
+        opd = self._check(opd, amp)
+        B = opd.shape[0]
+        raw = torch.empty((B, self.N, self.N), dtype=torch.float32, device=self.device)
+        out = torch.empty((B, self.cam, self.cam), dtype=torch.float32, device=self.device)
+        self._frames_generic(opd, scale, amp, out, raw, self._tx(weighted))
+        return raw
 
     def focal(self, opd, scale, amp, crop):
         """Modulation focal-plane intensity [B, crop, crop]: sum over points of |fft2(E_i)|^2 / N^2 on
         the window starting at N/2 - crop/2 (Pyramid.py:638-642, 1183-1190)."""
-        if self.T == 0:
-            raise OopaoError('modulation tables not set')
-        opd = opd.reshape(-1, self.n, self.n)
-        if opd.dtype != torch.float32 or not opd.is_contiguous() or opd.device != self.device:
-            raise OopaoError('opd must be a contiguous float32 tensor on %s' % self.device)
-        if amp.dtype != torch.float32 or tuple(amp.shape) != (self.n, self.n) or not amp.is_contiguous():
-            raise OopaoError('amp must be a contiguous float32 [n, n] tensor')
+        opd = self._check(opd, amp)
         crop = int(crop)
         if not 0 < crop <= self.N:
             raise OopaoError('focal window %d outside 1..%d' % (crop, self.N))
         B = opd.shape[0]
-        nchunk, ppt = self._knobs(B)
-        nb = ctypes.c_size_t(0)
-        _lib.call('oopao_pwfs_focal_bytes', self.n, self.N, self.T, B, nchunk, crop, ctypes.byref(nb))
-        if self._ws is None or self._ws.numel() < nb.value:
-            self._ws = torch.empty(nb.value, dtype=torch.uint8, device=self.device)
         out = torch.empty((B, crop, crop), dtype=torch.float32, device=self.device)
-        _lib.call('oopao_pwfs_focal', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx), _ptr(self.ty),
-                  self.n, self.N, self.T, B, nchunk, ppt, crop, _ptr(self._ws), ctypes.c_size_t(nb.value),
-                  _ptr(out), _stream_ptr(self.device))
+        nb = ctypes.c_size_t(0)
+        if self.fast_geometry:
+            nchunk, ppt = self._knobs(B)
+            _lib.call('oopao_pwfs_focal_bytes', self.n, self.N, self.T, B, nchunk, crop, ctypes.byref(nb))
+            ws = self._scratch(nb.value)
+            _lib.call('oopao_pwfs_focal', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx),
+                      _ptr(self.ty), self.n, self.N, self.T, B, nchunk, ppt, crop, _ptr(ws),
+                      ctypes.c_size_t(nb.value), _ptr(out), _stream_ptr(self.device))
+        else:
+            nchunk = self._generic_nchunk(B)
+            _lib.call('oopao_pwfs_generic_focal_bytes', self.n, self.N, self.T, B, nchunk, crop, ctypes.byref(nb))
+            ws = self._scratch(nb.value)
+            _lib.call('oopao_pwfs_generic_focal', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx),
+                      _ptr(self.ty), self.n, self.N, self.T, B, nchunk, crop, _ptr(ws), ctypes.c_size_t(nb.value),
+                      _ptr(out), _stream_ptr(self.device))
         return out
 
     # split passes (same result as frames(); used to time the passes separately)

@@ -162,7 +162,53 @@ def phase_mask(geom, psfCentering=True, rooftop=0.0, theta_rotation=0.0, sx=(0, 
     return m, np.complex64(np.exp(1j * m))
 
 
-def pyramid_transform(amp, phase_total, mask, phasor_nn, geom, precision='fp64', psfCentering=True):
+def phase_mask_old(geom, psfCentering=True, sx=(0, 0, 0, 0), sy=(0, 0, 0, 0)):
+    """The previous pyramid mask (Pyramid.py:482-539): a zero-mean tip/tilt ramp per quadrant.
+
+    Restated per pixel from the closed forms of the centred ramps: with h = N/2,
+    linspace(-pi, pi, h, endpoint=False) minus its mean is pi (2j - h + 1) / h;
+    linspace(-pi, pi, h + 1) is already zero-mean, pi (2j - h) / h; and
+    linspace(-l, l, h - 1, endpoint=False), l = pi - 2 pi / N, minus its mean is
+    l (2j - h + 2) / (h - 1).
+    """
+    N = geom.N
+    h = N // 2
+    L = geom.nSubap + geom.sep
+    norma = (geom.n / geom.nSubap) / 4
+    m = np.zeros((N, N))
+
+    def ramp(kind, j):
+        j = np.asarray(j, np.float64)
+        if kind == 'h':
+            return np.pi * (2 * j - h + 1) / h
+        if kind == 'p':
+            return np.pi * (2 * j - h) / h
+        lm = np.pi - 2 * np.pi / N
+        return lm * (2 * j - h + 2) / (h - 1)
+
+    def fill(r0, r1, c0, c1, kx, ky, a, b):
+        rows = np.arange(r1 - r0)
+        cols = np.arange(c1 - c0)
+        tilt = ramp(ky, rows)[:, None]
+        tip = ramp(kx, cols)[None, :]
+        m[r0:r1, c0:c1] = (a * tip + b * tilt) * norma
+
+    if psfCentering:
+        fill(0, h, 0, h, 'h', 'h', L + sx[0], L - sy[0])
+        fill(0, h, N - h, N, 'h', 'h', -(L - sx[1]), L - sy[1])
+        fill(N - h, N, N - h, N, 'h', 'h', -(L - sx[2]), -(L + sy[2]))
+        fill(N - h, N, 0, h, 'h', 'h', L + sx[3], -(L + sy[3]))
+    else:
+        fill(0, h + 1, 0, h + 1, 'p', 'p', L + sx[0], L - sy[0])
+        fill(0, h + 1, h + 1, N, 'm', 'p', -(L - sx[1]), L - sy[1])
+        fill(h + 1, N, h + 1, N, 'm', 'm', -(L - sx[2]), -(L + sy[2]))
+        fill(h + 1, N, 0, h + 1, 'p', 'm', L + sx[3], -(L + sy[3]))
+    m = -m
+    return m, np.complex64(np.exp(1j * m))
+
+
+def pyramid_transform(amp, phase_total, mask, phasor_nn, geom, precision='fp64', psfCentering=True,
+                      spatial_filter=None):
     """One modulation point: field -> pad -> FFT -> mask -> IFFT -> |E|^2 (Pyramid.py:612-646).
 
     ``phase_total`` is src.phase + TT_i (already summed in the caller's precision).
@@ -175,6 +221,9 @@ def pyramid_transform(amp, phase_total, mask, phasor_nn, geom, precision='fp64',
     support[c - n // 2:c + n // 2, c - n // 2:c + n // 2] = em
     if psfCentering:
         ft = scipy.fft.fft2(support * phasor_nn).astype(cdt)
+    elif spatial_filter is not None:
+        # Pyramid.py:633-634: the filter multiplies the shifted spectrum (and is also in the mask)
+        ft = np.fft.fftshift(scipy.fft.fft2(support)) * spatial_filter
     else:
         ft = np.fft.fftshift(scipy.fft.fft2(support)).astype(cdt)
     pw = np.fft.ifft2(ft * mask).astype(cdt)
@@ -234,7 +283,8 @@ class PyramidOracle:
                  n_pix_separation=2.0, n_pix_edge=None, calibModulation=50.0,
                  nTheta_user_defined=None, psfCentering=True, precision='fp64', n_jobs=1,
                  rooftop=0.0, theta_rotation=0.0, delta_theta=0.0, user_modulation_path=None,
-                 validI4Q=None):
+                 validI4Q=None, sx=(0, 0, 0, 0), sy=(0, 0, 0, 0), spatial_filter=None, weight_vector=None,
+                 old_mask=False):
         self.geom = Geometry(n, nSubap, n_pix_separation, n_pix_edge, calibModulation)
         self.precision = precision
         self.psfCentering = psfCentering
@@ -247,7 +297,14 @@ class PyramidOracle:
         self.user_modulation_path = user_modulation_path
         self.tip, self.tilt = tip_tilt(n)
         self.phasor = phasor(self.geom.N)
-        self.m, self.mask = phase_mask(self.geom, psfCentering, rooftop, theta_rotation)
+        if old_mask:
+            self.m, self.mask = phase_mask_old(self.geom, psfCentering, sx, sy)
+        else:
+            self.m, self.mask = phase_mask(self.geom, psfCentering, rooftop, theta_rotation, sx, sy)
+        self.spatial_filter = spatial_filter
+        if spatial_filter is not None:
+            self.mask = self.mask * spatial_filter     # spatialFilter setter (Pyramid.py:1072)
+        self.weight_vector = weight_vector
         self.lightRatio = lightRatio
         self._pool = None
         if validI4Q is not None:
@@ -295,13 +352,14 @@ class PyramidOracle:
         amp = self._amp()
         rdt = np.float64 if self.precision == 'fp64' else np.float32
         ph = np.asarray(phase, dtype=rdt)
+        sf = self.spatial_filter
         if self.tt is None:
             return pyramid_transform(amp, ph, self.mask, self.phasor, self.geom, self.precision,
-                                     self.psfCentering)[0]
+                                     self.psfCentering, sf)[0]
 
         def one(i):
             return pyramid_transform(amp, ph + self.tt[i], self.mask, self.phasor, self.geom,
-                                     self.precision, self.psfCentering)[0]
+                                     self.precision, self.psfCentering, sf)[0]
         if self.n_jobs > 1:
             from concurrent.futures import ThreadPoolExecutor
             if self._pool is None:
@@ -309,6 +367,10 @@ class PyramidOracle:
             maps = list(self._pool.map(one, range(self.nTheta)))
         else:
             maps = [one(i) for i in range(self.nTheta)]
+        if self.weight_vector is not None:
+            # Pyramid.py:753-755: diag(w) @ maps, summed, divided by nTheta
+            w = np.asarray(self.weight_vector, np.float64)
+            return np.tensordot(w, np.asarray(maps), axes=1) / self.nTheta
         return np.sum(np.asarray(maps), axis=0)
 
     def focal(self, phase, resolution):
@@ -321,7 +383,7 @@ class PyramidOracle:
         inten = 0
         for tt in tts:
             ft = pyramid_transform(amp, ph + tt, self.mask, self.phasor, self.geom, self.precision,
-                                   self.psfCentering)[1]
+                                   self.psfCentering, self.spatial_filter)[1]
             inten = inten + np.abs(ft) ** 2
         N = inten.shape[0]
         if resolution > N:

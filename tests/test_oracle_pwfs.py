@@ -170,3 +170,80 @@ def test_focal_camera_parseval_and_centring():
     peak = np.unravel_index(np.argmax(f0), f0.shape)
     assert peak[0] in (N // 2 - 1, N // 2) and peak[1] in (N // 2 - 1, N // 2)
     assert abs(f0[N // 2 - 1, N // 2 - 1] / f0[N // 2, N // 2] - 1) < 1e-9
+
+
+def _frame_raw(o, ph=None):
+    n = o.geom.n
+    return o.raw(np.zeros((n, n)) if ph is None else ph)
+
+
+def test_uniform_quadrant_shift_rolls_the_frame():
+    # Pyramid.py:453-480 with sx = sy = s on all four faces: m gains -norma s x (+ norma s y), a pure
+    # focal-plane ramp of 2 pi norma s / N per pixel, so the raw frame rolls by norma s pixels.
+    kw = dict(n_pix_separation=4, n_pix_edge=2, nTheta_user_defined=8)
+    n, nS = 16, 4
+    pup = opw.telescope_pupil(n)
+    flux = opw.flux_map(pup, opw.n_photon('I3', 8), 1e-3, 8.0)
+    ph = random_phase(n, 3, 0.5, pup)
+    base = opw.PyramidOracle(n, nS, 2, 0.1, pup, flux, 0.8e-6, **kw)
+    r0 = _frame_raw(base, ph)
+    ox = opw.PyramidOracle(n, nS, 2, 0.1, pup, flux, 0.8e-6, sx=(1, 1, 1, 1), validI4Q=base.validI4Q, **kw)
+    oy = opw.PyramidOracle(n, nS, 2, 0.1, pup, flux, 0.8e-6, sy=(1, 1, 1, 1), validI4Q=base.validI4Q, **kw)
+    norma = n // nS
+    assert rel_l2(_frame_raw(ox, ph), np.roll(r0, norma, axis=1)) < 1e-6
+    assert rel_l2(_frame_raw(oy, ph), np.roll(r0, -norma, axis=0)) < 1e-6
+
+
+def test_spatial_filter_enters_once_or_twice():
+    # Pyramid.py:633-636 and 1072: the filter is in the mask; without PSF centring it also multiplies
+    # the shifted spectrum. A uniform filter 0.5 therefore scales the frame by 1/4 or by 1/16.
+    n, nS = 16, 4
+    pup = opw.telescope_pupil(n)
+    flux = opw.flux_map(pup, opw.n_photon('I3', 8), 1e-3, 8.0)
+    ph = random_phase(n, 4, 0.5, pup)
+    for centring, factor in ((True, 0.25), (False, 0.0625)):
+        kw = dict(n_pix_separation=4, n_pix_edge=2, nTheta_user_defined=8, psfCentering=centring)
+        o = opw.PyramidOracle(n, nS, 2, 0.1, pup, flux, 0.8e-6, **kw)
+        N = o.geom.N
+        of = opw.PyramidOracle(n, nS, 2, 0.1, pup, flux, 0.8e-6, validI4Q=o.validI4Q,
+                               spatial_filter=np.full((N, N), 0.5), **kw)
+        assert rel_l2(_frame_raw(of, ph), factor * _frame_raw(o, ph)) < 1e-12
+
+
+def test_weight_vector_sum():
+    # Pyramid.py:753-755: raw = sum_i w_i I_i / nTheta
+    o = small_system(T=8)
+    ph = random_phase(o.geom.n, 5, 0.5, o.pupil)
+    r0 = o.raw(ph)
+    o.weight_vector = np.ones(8)
+    assert rel_l2(o.raw(ph), r0 / 8) < 1e-13
+    w = np.zeros(8)
+    w[3] = 8.0
+    o.weight_vector = w
+    amp = o._amp()
+    single = opw.pyramid_transform(amp, ph + o.tt[3], o.mask, o.phasor, o.geom)[0]
+    assert rel_l2(o.raw(ph), single) < 1e-13
+
+
+def test_old_mask_closed_form():
+    # Pyramid.py:482-539: zero-mean ramps per quadrant with slopes (L +- s) norma 2 pi / h; the
+    # oracle's per-pixel closed form against the drop-in's meshgrid construction.
+    from types import SimpleNamespace
+    from oopao_amd.Pyramid import Pyramid
+    g = opw.Geometry(16, 4, 4, 2)
+    N, h = g.N, g.N // 2
+    L = g.nSubap + g.sep
+    norma = (g.n / g.nSubap) / 4
+    sx, sy = (0.5, -1.0, 0.25, 2.0), (1.5, 0.0, -0.5, 1.0)
+    stub = SimpleNamespace(telescope=SimpleNamespace(resolution=16), nSubap=4)
+    for centring in (True, False):
+        m, mask = opw.phase_mask_old(g, centring, sx, sy)
+        mp = Pyramid.get_phase_mask_old(stub, N, 4, 4, 2, centring, list(sx), list(sy))
+        assert np.max(np.abs(m - mp)) < 1e-12
+        assert mask.dtype == np.complex64
+    m, _ = opw.phase_mask_old(g, True, sx, sy)
+    q1 = -m[:h, :h]
+    assert abs(q1.mean()) < 1e-12
+    step = 2 * np.pi / h
+    assert np.allclose(np.diff(q1, axis=1), (L + sx[0]) * norma * step)
+    assert np.allclose(np.diff(q1, axis=0), (L - sy[0]) * norma * step)
