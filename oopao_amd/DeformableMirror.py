@@ -102,8 +102,10 @@ class DeformableMirror:
         self._gc = np.sin(theta) ** 2 / (2 * cx ** 2) + np.cos(theta) ** 2 / (2 * cy ** 2)
         self._modes_dev = None
         self._user_modes = modes is not None
+        self._user_modes_f64 = None
         if modes is not None:
             m = np.asarray(modes) * self.sign
+            self._user_modes_f64 = np.ascontiguousarray(m, dtype=np.float64)
             self._modes_dev = torch.from_numpy(np.ascontiguousarray(m, dtype=np.float32)).to(self.device)
             self.nValidAct = m.shape[1]
         self._sep = None if self._user_modes else self._separable_factors()
@@ -173,6 +175,36 @@ class DeformableMirror:
             modes = m.reshape(n * n, -1).contiguous()
         return modes
 
+    def influence_functions_f64(self, chunk=512):
+        """Influence functions [n^2, nValidAct] in float64 on the device: the Gaussian of
+        DeformableMirror.py:572-600 evaluated in double precision (user modes as given)."""
+        if self._user_modes:
+            if self._user_modes_f64 is not None:
+                return torch.from_numpy(self._user_modes_f64).to(self.device)
+            return self._modes_dev.double()
+        n = self.resolution
+        grid = torch.from_numpy(np.linspace(0, 1, n) * n).to(self.device)
+        Y, X = torch.meshgrid(grid, grid, indexing='ij')           # X varies along columns
+        X = X.reshape(-1, 1)
+        Y = Y.reshape(-1, 1)
+        x0 = torch.from_numpy(np.asarray(self.u0x, np.float64)).to(self.device)
+        y0 = torch.from_numpy(np.asarray(self.u0y, np.float64)).to(self.device)
+        out = torch.empty((n * n, self.nValidAct), dtype=torch.float64, device=self.device)
+        a, b, c = float(self._ga), float(self._gb), float(self._gc)
+        for k0 in range(0, self.nValidAct, chunk):
+            k1 = min(self.nValidAct, k0 + chunk)
+            dx = X - x0[None, k0:k1]
+            dy = Y - y0[None, k0:k1]
+            out[:, k0:k1] = self.sign * torch.exp(-(a * dx * dx + 2 * b * dx * dy + c * dy * dy))
+        if self.flip_lr or self.flip_:
+            m = out.reshape(n, n, -1)
+            if self.flip_lr:
+                m = torch.flip(m, dims=[1])
+            if self.flip_:
+                m = torch.flip(m, dims=[0, 1])
+            out = m.reshape(n * n, -1).contiguous()
+        return out
+
     @property
     def modes(self):
         """Influence functions [n^2, nValidAct] (float64 numpy copy; DeformableMirror.py:391-394)."""
@@ -183,6 +215,7 @@ class DeformableMirror:
     @modes.setter
     def modes(self, val):
         self._modes_dev = torch.from_numpy(np.ascontiguousarray(val, np.float32)).to(self.device)
+        self._user_modes_f64 = np.ascontiguousarray(val, dtype=np.float64)
         self._user_modes = True
         self._sep = None
         self.use_dense = True
