@@ -106,6 +106,41 @@ def im_leg(dev, n_modes=1000):
             'nSignal': int(wfs.nSignal), 'cond': float(calib.cond)}
 
 
+def kl_leg(tel):
+    """KL modal basis of the config-2 DM (81 x 81 grid, 5209 actuators) in float64 on the GPU:
+    IF spectra, HHt, seed basis and KL diagonalisation (compute_KL_basis defaults)."""
+    import torch
+    from types import SimpleNamespace
+    from oopao_amd import DeformableMirror
+    from oopao_amd.calibration.compute_KL_modal_basis import compute_KL_basis
+    dm = DeformableMirror(tel, N_SUBAP, 0.35, print_dm_properties=False)
+    atm = SimpleNamespace(r0=0.15, L0=25.0)
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+    t0 = time.perf_counter()
+    B = compute_KL_basis(tel, atm, dm)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {'seconds': round(dt, 3), 'nAct': int(dm.nValidAct), 'modes': int(B.shape[1]), 'dtype': 'f64',
+            'peak_mem_GB': round(torch.cuda.max_memory_allocated() / 1e9, 2)}
+
+
+def measured_traffic(kernels):
+    """HBM bytes per launch of the given kernels from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, 'profiles', 'pwfs_config2_traffic.json')
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        tot = 0.0
+        for k in kernels:
+            e = d['kernels'][k]
+            tot += (e['fetch_kib'] * d['fetch_scale'] + e['write_kib'] * d['write_scale']) * 1024
+        return int(tot)
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -114,7 +149,8 @@ def main():
     ap.add_argument('--cpu-frames', type=int, default=3)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--batch', type=int, default=8, help='screens per call for the batched form')
-    ap.add_argument('--no-extra', action='store_true', help='skip the closed-loop and interaction-matrix legs')
+    ap.add_argument('--no-extra', action='store_true',
+                    help='skip the closed-loop, interaction-matrix and KL-basis legs')
     ap.add_argument('--calib-copy', action='store_true',
                     help='also run one elementwise copy of 256 MiB (counter calibration for rocprof --pmc)')
     args = ap.parse_args()
@@ -180,11 +216,13 @@ def main():
     fps = 1000.0 / ms_step
 
     # algorithmic work of the dominant kernel (columns pass + chunk reduction), per launch
-    n, N, T = N_RES, eng.N, eng.T
+    n, N, T, P = N_RES, eng.N, eng.T, eng.P
     log2N = np.log2(N)
     cols_flops = N * T * 2 * 5 * N * log2N            # one pruned forward + one inverse N-FFT per column & point
+    # the per-point intermediates read once, the chunk partials written and read by the reduction,
+    # the mask vector and the frame written
     cols_bytes = T * n * N * 8 + buf['nchunk'] * eng.cam ** 2 * 4 * 2 + N * 8 + eng.cam ** 2 * 4
-    rows_flops = n * T * 2 * 5 * N * log2N
+    rows_flops = P * n * 2 * 5 * N * log2N
     achieved_tf = cols_flops / (cols_ms * 1e-3) / 1e12
     achieved_gbs = cols_bytes / (cols_ms * 1e-3) / 1e9
 
@@ -211,6 +249,7 @@ def main():
     if not args.no_extra:
         extra['closed_loop'] = closed_loop_leg(tel, ngs, wfs, dev)
         extra['interaction_matrix'] = im_leg(dev)
+        extra['kl_basis'] = kl_leg(tel)
 
     out = {
         'metric': 'PWFS frames/s at 80x80 subaps, 72 mod pts',
@@ -227,11 +266,12 @@ def main():
         'data': 'synthetic seeded OPD screen (150 nm RMS, seed 10), resident in HBM',
         'config': {'workload': 'config-2 modulated PWFS: n=320, N=1280, T=72, 320x320 detector, slopesMaps',
                    'n': N_RES, 'N': int(N), 'nTheta': int(T), 'nSubap': N_SUBAP, 'batch': 1,
-                   'nchunk': buf['nchunk'], 'pts_per_task': buf['ppt']},
+                   'nchunk': buf['nchunk'], 'pts_per_task': buf['ppt'], 'tip_tables': int(P)},
         'kernels_ms': {'rows_pass': round(rows_ms, 5), 'cols_pass_plus_reduce': round(cols_ms, 5)},
         'roofline': {'bound': 'valu', 'kernel': 'k_cols_sep<1280,64> + k_reduce_chunks',
                      'achieved': round(achieved_tf, 3), 'peak': VALU_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': round(achieved_tf / VALU_PEAK_TFLOPS, 4), 'traffic': None,
+                     'frac': round(achieved_tf / VALU_PEAK_TFLOPS, 4),
+                     'traffic': measured_traffic(['k_cols_sep<1280,64>', 'k_reduce_chunks']),
                      'flops_per_launch': cols_flops, 'bytes_per_launch': cols_bytes,
                      'hbm_achieved_GBs': round(achieved_gbs, 1),
                      'hbm_frac': round(achieved_gbs / HBM_PEAK_GBS, 4)},

@@ -44,49 +44,55 @@ int oopao_pwfs_supported(int n, int N, int bin);
 int oopao_pwfs_workspace_bytes(int n, int N, int T, int B, int nchunk, size_t* bytes);
 
 /* Modulated PWFS detector frames for B phase screens (Pyramid.py:612-646, 712-826).
- *   opd   [B][n][n] float32; phase = opd * scale (src.phase = OPD 2 pi / lambda,
- *         Source.py:294-296)
- *   amp   [n][n]    float32 field amplitude sqrt(src.intensity / nTheta) (Pyramid.py:674)
- *   tx,ty [T][n]    complex64 per-point tip (x) / tilt (y) phasors times the
- *                   PSF-centring phasor (Pyramid.py:334-344, 1129-1139)
- *   mx,my [N]       complex64 separable factors of the pyramid mask, with the
- *                   zero-padding offset shift and the 1/N of each inverse FFT
- *                   folded in (Pyramid.py:388, 453-480)
- *   frames[B][N/bin][N/bin] float32: sum over points of |ifft2(fft2(E) mask)|^2,
- *         sum-binned (tools/tools.py:464-483)
- * pts_per_task: modulation points per rows-pass task; nchunk: point chunks of
- * the columns pass (both are tuning knobs, any valid value gives the same
- * result up to float summation order within a chunk). */
-int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const void* tx,
-                      const void* ty, const void* mx, const void* my, int n, int N, int T, int B,
-                      int bin, int nchunk, int pts_per_task, void* workspace, size_t ws_bytes,
-                      float* frames, void* stream);
+ *   opd    [B][n][n] float32; phase = opd * scale (src.phase = OPD 2 pi / lambda,
+ *          Source.py:294-296)
+ *   amp    [n][n]    float32 field amplitude sqrt(src.intensity / nTheta) (Pyramid.py:674)
+ *   tx     [P][n]    complex64 distinct tip (x) phasors times the PSF-centring phasor
+ *   pstart [P+1]     int32: the points using tip table p are pstart[p] .. pstart[p+1]-1
+ *                    (0 = pstart[0] <= ... <= pstart[P] = T; values are clamped to that)
+ *   ty     [T][n]    complex64 per-point tilt (y) phasors times the PSF-centring
+ *                    phasor (Pyramid.py:334-344, 1129-1139), in the same point order:
+ *                    point k's field is u(y, x) tx[p][x] ty[k][y]
+ *   mx,my  [N]       complex64 separable factors of the pyramid mask, with the
+ *                    zero-padding offset shift and the 1/N of each inverse FFT
+ *                    folded in (Pyramid.py:388, 453-480)
+ *   frames [B][N/bin][N/bin] float32: sum over points of |ifft2(fft2(E) mask)|^2,
+ *          sum-binned (tools/tools.py:464-483)
+ * pts_per_task: tip tables per rows-pass task; nchunk: point chunks of the
+ * columns pass (both are tuning knobs, any valid value gives the same result
+ * up to float summation order within a chunk). */
+int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
+                      const void* ty, const void* mx, const void* my, int n, int N, int T, int B, int bin,
+                      int nchunk, int pts_per_task, void* workspace, size_t ws_bytes, float* frames,
+                      void* stream);
 
 /* The two passes of oopao_pwfs_frames as separate calls (same arguments), for
  * callers that own the intermediate buffers or time the passes:
- *   rows pass : inter[B*T][N/4][n][4] complex64 (per pupil row y, point i:
- *               IFFT_N(mx . FFT_N(pad(amp exp(i scale opd[y,:]) tx_i ty_i[y]))))
+ *   rows pass : inter[B*T][N/4][n][4] complex64 (per pupil row y and point k of
+ *               tip table p: ty_k[y] IFFT_N(mx . FFT_N(pad(amp exp(i scale opd[y,:]) tx_p))),
+ *               one transform per table, written once per point of its group)
  *   cols pass : per detector column, IFFT_N(my . FFT_N(pad(column))) for every
  *               point, |.|^2 summed over the points of each of `nchunk` chunks
  *               and binned 4x4 into part[B][nchunk][cam][cam], then reduced over
  *               chunks into frames[B][cam][cam]. */
 int oopao_pwfs_intermediate_bytes(int n, int N, int T, int B, int nchunk, size_t* inter_bytes,
                                   size_t* part_bytes);
-int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
-                    const void* mx, int n, int N, int T, int B, int pts_per_task, void* inter, void* stream);
+int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
+                    const void* ty, const void* mx, int n, int N, int T, int B, int pts_per_task, void* inter,
+                    void* stream);
 int oopao_pwfs_cols(const void* inter, const void* my, int n, int N, int T, int B, int nchunk, float* part,
                     float* frames, void* stream);
 
 /* Modulation focal-plane camera (Pyramid.py:638-642 modulation_camera_em,
  * 1183-1190 intensity crop): for B screens,
- *   out[b][crop][crop] = sum_i |fft2(pad(E_b tx_i ty_i))|^2 / N^2
+ *   out[b][crop][crop] = sum_k |fft2(pad(E_b tx_p(k) ty_k))|^2 / N^2
  * on the crop x crop window starting at N/2 - crop/2 (crop = the focal
  * camera's resolution). Same field inputs as oopao_pwfs_frames; workspace
  * size from oopao_pwfs_focal_bytes. */
 int oopao_pwfs_focal_bytes(int n, int N, int T, int B, int nchunk, int crop, size_t* bytes);
-int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void* tx, const void* ty, int n,
-                     int N, int T, int B, int nchunk, int pts_per_task, int crop, void* workspace,
-                     size_t ws_bytes, float* out, void* stream);
+int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
+                     const void* ty, int n, int N, int T, int B, int nchunk, int pts_per_task, int crop,
+                     void* workspace, size_t ws_bytes, float* out, void* stream);
 
 /* ---------------------------------------------------------------------------
  * General PWFS path: any padded length N <= 2048, any N x N mask (rooftop,
@@ -116,10 +122,13 @@ int oopao_pwfs_generic_focal(const float* opd, float scale, const float* amp, co
  *   ref2d [2w][w] float32 (referenceSignal_2D), units = slopesUnits,
  *   vidx [nsig] int32 flat indices of validSignal in the [2w][w] map,
  *   nvalid = number of valid pixels of validI4Q.
- * Outputs sig2d [B][2w][w] (signal_2D) and sig [B][nsig] (signal). */
+ * Outputs sig2d [B][2w][w] (signal_2D) and sig [B][nsig] (signal). The normalisation
+ * (mean of the valid I4Q, accumulated in double in a fixed order) uses a
+ * workspace of oopao_pwfs_slopes_workspace_bytes(B, w) bytes. */
+int oopao_pwfs_slopes_workspace_bytes(int B, int w, size_t* bytes);
 int oopao_pwfs_slopes(const float* frames, int B, int cam, const unsigned char* valid, int w,
                       const int* quad, const float* ref2d, float units, const int* vidx, int nsig,
-                      int nvalid, float* sig2d, float* sig, void* stream);
+                      int nvalid, void* workspace, size_t ws_bytes, float* sig2d, float* sig, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Deformable mirror (replaces DeformableMirror.modesComputation and the coefs

@@ -24,20 +24,21 @@ SIGNATURES = {
     'oopao_last_error': (ctypes.c_char_p, []),
     'oopao_pwfs_supported': (c_int, [c_int, c_int, c_int]),
     'oopao_pwfs_workspace_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
-    'oopao_pwfs_frames': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                  c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_size_t,
+    'oopao_pwfs_frames': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_size_t,
                                   c_void_p, c_void_p]),
+    'oopao_pwfs_slopes_workspace_bytes': (c_int, [c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_slopes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_float,
-                                  c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+                                  c_void_p, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),
     'oopao_pwfs_intermediate_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t),
                                               ctypes.POINTER(c_size_t)]),
-    'oopao_pwfs_rows': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
-                                c_int, c_int, c_void_p, c_void_p]),
+    'oopao_pwfs_rows': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,
+                                c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
     'oopao_pwfs_cols': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
                                 c_void_p]),
     'oopao_pwfs_focal_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
-    'oopao_pwfs_focal': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
-                                 c_int, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p]),
+    'oopao_pwfs_focal': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int,
+                                 c_int, c_int, c_int, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p]),
     'oopao_pwfs_generic_supported': (c_int, [c_int, c_int, c_int]),
     'oopao_pwfs_generic_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_generic_frames': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,

@@ -10,9 +10,14 @@
 // along X and Ky likewise along Y. The tip/tilt modulation TT_i = px_i Tip +
 // py_i Tilt and the PSF-centring phasor are separable too. So the frame is
 // computed in two passes with only an n x N intermediate per point:
-//   rows pass : C_i[y,:] = IFFT_N( mx' . FFT_N(pad(u[y,:] tx_i ty_i[y])) )
+//   rows pass : C_i[y,:] = ty_i[y] IFFT_N( mx' . FFT_N(pad(u[y,:] tx_p)) )
 //   cols pass : E_i[:,X] = IFFT_N( my' . FFT_N(pad(C_i[:,X])) ), |E_i|^2 summed
 //               over a chunk of points and over 4 x 4 detector pixels
+// The tilt phasor ty_i[y] is a scalar per pupil row, so it commutes with the
+// row transforms: the rows pass transforms each distinct tip table tx_p once
+// and writes it, scaled by ty_i[y], for every point i of its group (points
+// pstart[p] .. pstart[p+1]-1). On the default circular path points i and
+// T - i share their tip, which halves the rows-pass transforms.
 // followed by a tiny deterministic reduction over point chunks. The zero
 // padding offset c0 and the 1/N of each inverse transform are folded into the
 // mask vectors mx', my' on the host; tx, ty carry exp(i px lin[x]) and the
@@ -58,15 +63,16 @@ __global__ void k_field(const float* __restrict__ opd, float scale, const float*
 
 // ---------------------------------------------------------------------------
 // Rows pass. One lane group (TPT lanes) per pupil row y; the row's field is
-// loaded once and reused for a chunk of modulation points. FWD_ONLY stops
-// after the forward transform (focal-plane field rows, no mask).
+// loaded once and reused for a chunk of tip tables tx_p; each transformed row
+// is written once per point of the table's group, times that point's tilt
+// phasor. FWD_ONLY stops after the forward transform (focal-plane field rows,
+// no mask).
 // ---------------------------------------------------------------------------
 template <int N, int TPT, bool FWD_ONLY = false>
 __global__ void __launch_bounds__(64 * kWaves)
-k_rows_sep(const float2* __restrict__ field,
-           const float2* __restrict__ tx, const float2* __restrict__ ty,
-           const float2* __restrict__ mx, const float2* __restrict__ tw,
-           float2* __restrict__ ws, int B, int T, int pts_per_task) {
+k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, const int* __restrict__ pstart,
+           const float2* __restrict__ ty, const float2* __restrict__ mx, const float2* __restrict__ tw,
+           float2* __restrict__ ws, int B, int P, int T, int pts_per_task) {
   constexpr int n = N / 4;
   constexpr int E = N / TPT;
   constexpr int G = 64 / TPT;
@@ -77,7 +83,7 @@ k_rows_sep(const float2* __restrict__ field,
   const int h = lane / TPT;
   const int t = lane % TPT;
   constexpr int nrb = (n + RPB - 1) / RPB;
-  const int npc = (T + pts_per_task - 1) / pts_per_task;
+  const int npc = (P + pts_per_task - 1) / pts_per_task;
   int bid = blockIdx.x;
   const int rb = bid % nrb;
   bid /= nrb;
@@ -93,13 +99,12 @@ k_rows_sep(const float2* __restrict__ field,
 #pragma unroll
   for (int m = 0; m < E; ++m) u[m] = fp[(t + TPT * m) >> 2];
   const int i0 = pc * pts_per_task;
-  const int i1 = min(T, i0 + pts_per_task);
+  const int i1 = min(P, i0 + pts_per_task);
   for (int i = i0; i < i1; ++i) {
-    const float2 tyv = ty[(size_t)i * n + y];
     const float2* txi = launder(tx) + (size_t)i * n;
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cmul(cmul(u[m], txi[(t + TPT * m) >> 2]), tyv);
+    for (int m = 0; m < E; ++m) v[m] = cmul(u[m], txi[(t + TPT * m) >> 2]);
     fft_fwd_pruned<N, TPT, 4>(v, xl, tw, t);
     if constexpr (!FWD_ONLY) {
       const float2* mxp = launder(mx);
@@ -108,11 +113,16 @@ k_rows_sep(const float2* __restrict__ field,
       fft_full<N, TPT, +1>(v, xl, tw, t);
     }
     if (active) {
-      float2* dst = ws + ((size_t)b * T + i) * (size_t)(N / 4) * n * 4;
+      const int k0 = min(max(pstart[i], 0), T);
+      const int k1 = min(max(pstart[i + 1], k0), T);
+      for (int k = k0; k < k1; ++k) {
+        const float2 tyv = ty[(size_t)k * n + y];
+        float2* dst = ws + ((size_t)b * T + k) * (size_t)(N / 4) * n * 4;
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int X = t + TPT * m;
-        dst[((size_t)(X >> 2) * n + y) * 4 + (X & 3)] = v[m];
+        for (int m = 0; m < E; ++m) {
+          const int X = t + TPT * m;
+          dst[((size_t)(X >> 2) * n + y) * 4 + (X & 3)] = cmul(v[m], tyv);
+        }
       }
     }
   }
@@ -284,58 +294,75 @@ __global__ void k_reduce_chunks(const float* __restrict__ part, float* __restric
 }
 
 // ---------------------------------------------------------------------------
-// slopesMaps post-processing (Pyramid.py:838-854), one workgroup per frame.
+// slopesMaps post-processing (Pyramid.py:838-854) in two launches: per-block
+// sums of the valid I4Q (double), then every slope normalised by their mean.
 // quad[0..3] = row origins, quad[4..7] = column origins of quadrants 1..4.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024)
-k_slopes(const float* __restrict__ frames, int cam, const unsigned char* __restrict__ valid, int w,
-         const int* __restrict__ quad, const float* __restrict__ ref2d, float units,
-         const int* __restrict__ vidx, int nsig, int nvalid, float* __restrict__ sig2d,
-         float* __restrict__ sig) {
-  const int b = blockIdx.x;
+__global__ void __launch_bounds__(256)
+k_slopes_partial(const float* __restrict__ frames, int cam, const unsigned char* __restrict__ valid, int w,
+                 const int* __restrict__ quad, int nblk, double* __restrict__ part) {
+  // part[b][blk] = sum over this block's valid pixels of I1 + I2 + I3 + I4 (fixed-order tree)
+  const int b = blockIdx.y;
   const float* f = frames + (size_t)b * cam * cam;
-  __shared__ double red[1024];
-  __shared__ float norma_s;
+  __shared__ double red[256];
   const int r1 = quad[0], r2 = quad[1], r3 = quad[2], r4 = quad[3];
   const int c1 = quad[4], c2 = quad[5], c3 = quad[6], c4 = quad[7];
+  const int p = blockIdx.x * 256 + threadIdx.x;
   double acc = 0.0;
-  for (int p = threadIdx.x; p < w * w; p += blockDim.x) {
-    if (valid[p]) {
-      const int py = p / w, px = p % w;
-      const float I1 = f[(r1 + py) * cam + c1 + px];
-      const float I2 = f[(r2 + py) * cam + c2 + px];
-      const float I3 = f[(r3 + py) * cam + c3 + px];
-      const float I4 = f[(r4 + py) * cam + c4 + px];
-      acc += (double)I1 + (double)I2 + (double)I3 + (double)I4;
-    }
+  if (p < w * w && valid[p]) {
+    const int py = p / w, px = p % w;
+    acc = ((double)f[(r1 + py) * cam + c1 + px] + (double)f[(r2 + py) * cam + c2 + px]) +
+          ((double)f[(r3 + py) * cam + c3 + px] + (double)f[(r4 + py) * cam + c4 + px]);
   }
   red[threadIdx.x] = acc;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) norma_s = (float)(red[0] / (double)nvalid);
-  __syncthreads();
-  const float inv = 1.0f / norma_s;
-  float* s2 = sig2d + (size_t)b * 2 * w * w;
-  for (int p = threadIdx.x; p < w * w; p += blockDim.x) {
-    float sx = 0.f, sy = 0.f;
-    if (valid[p]) {
-      const int py = p / w, px = p % w;
-      const float I1 = f[(r1 + py) * cam + c1 + px];
-      const float I2 = f[(r2 + py) * cam + c2 + px];
-      const float I3 = f[(r3 + py) * cam + c3 + px];
-      const float I4 = f[(r4 + py) * cam + c4 + px];
-      sx = (I1 - I2) + (I4 - I3);
-      sy = (I1 - I4) + (I2 - I3);
-    }
-    s2[p] = (sx * inv - ref2d[p]) * units;
-    s2[w * w + p] = (sy * inv - ref2d[w * w + p]) * units;
+  if (threadIdx.x == 0) part[(size_t)b * nblk + blockIdx.x] = red[0];
+}
+
+// Normalised slopes at flat index q of the [2w][w] map (q < w*w: Sx, else Sy), Pyramid.py:846-853.
+__device__ __forceinline__ float slope_at(const float* f, int cam, const unsigned char* valid, int w,
+                                          const int* quad, const float* ref2d, float units, float inv, int q) {
+  const int ww = w * w;
+  const int p = q < ww ? q : q - ww;
+  float s = 0.f;
+  if (valid[p]) {
+    const int py = p / w, px = p % w;
+    const float I1 = f[(quad[0] + py) * cam + quad[4] + px];
+    const float I2 = f[(quad[1] + py) * cam + quad[5] + px];
+    const float I3 = f[(quad[2] + py) * cam + quad[6] + px];
+    const float I4 = f[(quad[3] + py) * cam + quad[7] + px];
+    s = q < ww ? (I1 - I2) + (I4 - I3) : (I1 - I4) + (I2 - I3);
+  }
+  return (s * inv - ref2d[q]) * units;
+}
+
+__global__ void __launch_bounds__(256)
+k_slopes_finish(const float* __restrict__ frames, int cam, const unsigned char* __restrict__ valid, int w,
+                const int* __restrict__ quad, const float* __restrict__ ref2d, float units,
+                const int* __restrict__ vidx, int nsig, int nvalid, int nblk, const double* __restrict__ part,
+                float* __restrict__ sig2d, float* __restrict__ sig) {
+  const int b = blockIdx.y;
+  __shared__ float inv_s;
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < nblk; ++k) t += part[(size_t)b * nblk + k];
+    inv_s = 1.0f / (float)(t / (double)nvalid);
   }
   __syncthreads();
-  float* so = sig + (size_t)b * nsig;
-  for (int j = threadIdx.x; j < nsig; j += blockDim.x) so[j] = s2[vidx[j]];
+  const float inv = inv_s;
+  const float* f = frames + (size_t)b * cam * cam;
+  const int n2 = 2 * w * w;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n2) {
+    sig2d[(size_t)b * n2 + i] = slope_at(f, cam, valid, w, quad, ref2d, units, inv, i);
+  } else if (i - n2 < nsig) {
+    const int j = i - n2;
+    sig[(size_t)b * nsig + j] = slope_at(f, cam, valid, w, quad, ref2d, units, inv, vidx[j]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -352,8 +379,9 @@ static size_t ws_part_bytes(int N, int B, int nchunk) {
 }
 
 template <int N, bool FWD_ONLY = false>
-static int launch_rows(const float* opd, float scale, const float* amp, const float2* tx, const float2* ty,
-                       const float2* mx, int T, int B, int pts_per_task, float2* inter, hipStream_t st) {
+static int launch_rows(const float* opd, float scale, const float* amp, const float2* tx, int P, const int* pstart,
+                       const float2* ty, const float2* mx, int T, int B, int pts_per_task, float2* inter,
+                       hipStream_t st) {
   constexpr int TPT = fast_tpt(N);
   constexpr int n = N / 4;
   constexpr int G = 64 / TPT;
@@ -362,7 +390,7 @@ static int launch_rows(const float* opd, float scale, const float* amp, const fl
   if (rc) return rc;
   constexpr int RPB = kWaves * G;
   const int nrb = (n + RPB - 1) / RPB;
-  const int npc = (T + pts_per_task - 1) / pts_per_task;
+  const int npc = (P + pts_per_task - 1) / pts_per_task;
   float2* field = inter + (size_t)B * T * (size_t)N * n;
   {
     const size_t total = (size_t)B * n * n;
@@ -371,8 +399,8 @@ static int launch_rows(const float* opd, float scale, const float* amp, const fl
     OOPAO_LAUNCH_CHECK();
   }
   dim3 grid((unsigned)(nrb * npc * B));
-  hipLaunchKernelGGL((k_rows_sep<N, TPT, FWD_ONLY>), grid, dim3(64 * kWaves), 0, st, field, tx, ty, mx, tw,
-                     inter, B, T, pts_per_task);
+  hipLaunchKernelGGL((k_rows_sep<N, TPT, FWD_ONLY>), grid, dim3(64 * kWaves), 0, st, field, tx, pstart, ty, mx, tw,
+                     inter, B, P, T, pts_per_task);
   OOPAO_LAUNCH_CHECK();
   return OK;
 }
@@ -459,10 +487,12 @@ int oopao_pwfs_intermediate_bytes(int n, int N, int T, int B, int nchunk, size_t
   return OK;
 }
 
-int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
-                    const void* mx, int n, int N, int T, int B, int pts_per_task, void* inter, void* stream) {
-  OOPAO_CHECK_ARG(opd && amp && tx && ty && mx && inter, "null pointer argument");
-  OOPAO_CHECK_ARG(T > 0 && B > 0 && pts_per_task > 0, "invalid T=%d B=%d pts_per_task=%d", T, B, pts_per_task);
+int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
+                    const void* ty, const void* mx, int n, int N, int T, int B, int pts_per_task, void* inter,
+                    void* stream) {
+  OOPAO_CHECK_ARG(opd && amp && tx && pstart && ty && mx && inter, "null pointer argument");
+  OOPAO_CHECK_ARG(P > 0 && P <= T && B > 0 && pts_per_task > 0, "invalid P=%d T=%d B=%d pts_per_task=%d", P, T, B,
+                  pts_per_task);
   int rc = check_geometry(n, N, 4);
   if (rc) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -473,7 +503,7 @@ int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void*
   switch (N) {
 #define OOPAO_CASE(NN) \
   case NN:             \
-    return launch_rows<NN>(opd, scale, amp, txp, typ, mxp, T, B, pts_per_task, ip, st);
+    return launch_rows<NN>(opd, scale, amp, txp, P, pstart, typ, mxp, T, B, pts_per_task, ip, st);
     OOPAO_FAST_SIZES(OOPAO_CASE)
 #undef OOPAO_CASE
     default:
@@ -505,12 +535,12 @@ int oopao_pwfs_cols(const void* inter, const void* my, int n, int N, int T, int 
   return ERR_UNSUPPORTED;
 }
 
-int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
-                      const void* mx, const void* my, int n, int N, int T, int B, int bin, int nchunk,
+int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
+                      const void* ty, const void* mx, const void* my, int n, int N, int T, int B, int bin, int nchunk,
                       int pts_per_task, void* workspace, size_t ws_bytes, float* frames, void* stream) {
-  OOPAO_CHECK_ARG(opd && amp && tx && ty && mx && my && workspace && frames, "null pointer argument");
-  OOPAO_CHECK_ARG(T > 0 && B > 0 && nchunk > 0 && nchunk <= T && pts_per_task > 0,
-                  "invalid T=%d B=%d nchunk=%d pts_per_task=%d", T, B, nchunk, pts_per_task);
+  OOPAO_CHECK_ARG(opd && amp && tx && pstart && ty && mx && my && workspace && frames, "null pointer argument");
+  OOPAO_CHECK_ARG(P > 0 && P <= T && B > 0 && nchunk > 0 && nchunk <= T && pts_per_task > 0,
+                  "invalid P=%d T=%d B=%d nchunk=%d pts_per_task=%d", P, T, B, nchunk, pts_per_task);
   int rc = check_geometry(n, N, bin);
   if (rc) return rc;
   size_t need = ws_inter_bytes(n, N, T, B) + ws_part_bytes(N, B, nchunk);
@@ -521,7 +551,7 @@ int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const voi
   char* w = reinterpret_cast<char*>(workspace);
   void* inter = w;
   float* part = reinterpret_cast<float*>(w + ws_inter_bytes(n, N, T, B));
-  rc = oopao_pwfs_rows(opd, scale, amp, tx, ty, mx, n, N, T, B, pts_per_task, inter, stream);
+  rc = oopao_pwfs_rows(opd, scale, amp, tx, P, pstart, ty, mx, n, N, T, B, pts_per_task, inter, stream);
   if (rc) return rc;
   return oopao_pwfs_cols(inter, my, n, N, T, B, nchunk, part, frames, stream);
 }
@@ -534,12 +564,13 @@ int oopao_pwfs_focal_bytes(int n, int N, int T, int B, int nchunk, int crop, siz
   return OK;
 }
 
-int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void* tx, const void* ty, int n, int N,
-                     int T, int B, int nchunk, int pts_per_task, int crop, void* workspace, size_t ws_bytes,
-                     float* out, void* stream) {
-  OOPAO_CHECK_ARG(opd && amp && tx && ty && workspace && out, "null pointer argument");
-  OOPAO_CHECK_ARG(T > 0 && B > 0 && nchunk > 0 && nchunk <= T && pts_per_task > 0 && crop > 0 && crop <= N,
-                  "invalid T=%d B=%d nchunk=%d pts_per_task=%d crop=%d", T, B, nchunk, pts_per_task, crop);
+int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
+                     const void* ty, int n, int N, int T, int B, int nchunk, int pts_per_task, int crop,
+                     void* workspace, size_t ws_bytes, float* out, void* stream) {
+  OOPAO_CHECK_ARG(opd && amp && tx && pstart && ty && workspace && out, "null pointer argument");
+  OOPAO_CHECK_ARG(P > 0 && P <= T && B > 0 && nchunk > 0 && nchunk <= T && pts_per_task > 0 && crop > 0 &&
+                      crop <= N,
+                  "invalid P=%d T=%d B=%d nchunk=%d pts_per_task=%d crop=%d", P, T, B, nchunk, pts_per_task, crop);
   int rc = check_geometry(n, N, 4);
   if (rc) return rc;
   size_t need = 0;
@@ -556,7 +587,7 @@ int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void
   switch (N) {
 #define OOPAO_CASE(NN)                                                                                     \
   case NN:                                                                                                 \
-    rc = launch_rows<NN, true>(opd, scale, amp, txp, typ, txp, T, B, pts_per_task, inter, st);             \
+    rc = launch_rows<NN, true>(opd, scale, amp, txp, P, pstart, typ, txp, T, B, pts_per_task, inter, st);  \
     if (rc) return rc;                                                                                     \
     return launch_focal<NN>(inter, T, B, nchunk, crop, part, out, st);
     OOPAO_FAST_SIZES(OOPAO_CASE)
@@ -568,15 +599,32 @@ int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void
   return ERR_UNSUPPORTED;
 }
 
+int oopao_pwfs_slopes_workspace_bytes(int B, int w, size_t* bytes) {
+  OOPAO_CHECK_ARG(bytes != nullptr, "bytes is null");
+  OOPAO_CHECK_ARG(B > 0 && w > 0, "invalid sizes B=%d w=%d", B, w);
+  *bytes = (size_t)B * ((w * w + 255) / 256) * sizeof(double);
+  return OK;
+}
+
 int oopao_pwfs_slopes(const float* frames, int B, int cam, const unsigned char* valid, int w,
                       const int* quad, const float* ref2d, float units, const int* vidx, int nsig,
-                      int nvalid, float* sig2d, float* sig, void* stream) {
-  OOPAO_CHECK_ARG(frames && valid && quad && ref2d && vidx && sig2d && sig, "null pointer argument");
+                      int nvalid, void* workspace, size_t ws_bytes, float* sig2d, float* sig, void* stream) {
+  OOPAO_CHECK_ARG(frames && valid && quad && ref2d && vidx && sig2d && sig && workspace, "null pointer argument");
   OOPAO_CHECK_ARG(B > 0 && cam > 0 && w > 0 && nsig >= 0 && nvalid > 0,
                   "invalid sizes B=%d cam=%d w=%d nsig=%d nvalid=%d", B, cam, w, nsig, nvalid);
+  OOPAO_CHECK_ARG(B <= 65535, "too many frames in one call: %d", B);
+  const int nblk = (w * w + 255) / 256;
+  if (ws_bytes < (size_t)B * nblk * sizeof(double)) {
+    set_error("workspace too small: %zu < %zu bytes", ws_bytes, (size_t)B * nblk * sizeof(double));
+    return ERR_WORKSPACE;
+  }
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_slopes, dim3(B), dim3(1024), 0, st, frames, cam, valid, w, quad, ref2d, units,
-                     vidx, nsig, nvalid, sig2d, sig);
+  double* part = reinterpret_cast<double*>(workspace);
+  hipLaunchKernelGGL(k_slopes_partial, dim3(nblk, B), dim3(256), 0, st, frames, cam, valid, w, quad, nblk, part);
+  OOPAO_LAUNCH_CHECK();
+  const int nout = 2 * w * w + nsig;
+  hipLaunchKernelGGL(k_slopes_finish, dim3((nout + 255) / 256, B), dim3(256), 0, st, frames, cam, valid, w, quad,
+                     ref2d, units, vidx, nsig, nvalid, nblk, part, sig2d, sig);
   OOPAO_LAUNCH_CHECK();
   return OK;
 }

@@ -61,6 +61,23 @@ def modulation_phasors(path, tip_lin, tilt_lin, N, n, psfCentering=True):
     return tx, ty
 
 
+def tip_groups(path, tol=1e-12):
+    """Points sharing their tip value px (to ``tol`` relative): returns (order, pidx, reps) with
+    ``order`` the points sorted so that each group is contiguous, ``pidx[k]`` the group of point
+    order[k] and ``reps`` one member per group. On the default circular path points i and T - i
+    share cos(theta) (Pyramid.py:1129-1132), so P = T/2 + 1."""
+    px = np.asarray(path, np.float64)[:, 0]
+    order = np.argsort(px, kind='stable')
+    pidx = np.zeros(px.size, np.int32)
+    reps = [int(order[0])]
+    for k in range(1, px.size):
+        a, b = px[order[k - 1]], px[order[k]]
+        if b - a > tol * max(1.0, abs(a), abs(b)):
+            reps.append(int(order[k]))
+        pidx[k] = len(reps) - 1
+    return order, pidx, np.asarray(reps)
+
+
 def mask_vectors(a, b, N, n):
     """my[N], mx[N] complex128 from the separable mask phase (see module docstring)."""
     c0 = N // 2 - n // 2
@@ -95,8 +112,18 @@ class PWFSEngine:
                                     psfCentering)
         self.T = tx.shape[0]
         self._tx_host = tx
+        self._ty_host = ty
+        # general path: per-point tables in the reference's point order
         self.tx = torch.from_numpy(tx.astype(np.complex64)).to(self.device)
         self.ty = torch.from_numpy(ty.astype(np.complex64)).to(self.device)
+        # fast path: distinct tip tables, points grouped by tip
+        order, pidx, reps = tip_groups(path)
+        self._order = order
+        self.P = int(reps.size)
+        self.tx_d = torch.from_numpy(tx[reps].astype(np.complex64)).to(self.device)
+        pstart = np.searchsorted(pidx, np.arange(self.P + 1)).astype(np.int32)
+        self.pstart = torch.from_numpy(pstart).to(self.device)
+        self.ty_s = torch.from_numpy(ty[order].astype(np.complex64)).to(self.device)
         self._make_weighted()
 
     def set_weights(self, w):
@@ -108,22 +135,31 @@ class PWFSEngine:
     def _make_weighted(self):
         w = getattr(self, '_weights', None)
         self.txw = None
+        self.ty_sw = None
         if w is None or self.T == 0:
             return
         if w.size != self.T:
             return      # reported when a weighted frame is asked for
         if np.any(w < 0):
             raise OopaoError('negative modulation weights are not supported')
-        txw = self._tx_host * np.sqrt(w / self.T)[:, None]
-        self.txw = torch.from_numpy(txw.astype(np.complex64)).to(self.device)
+        sc = np.sqrt(w / self.T)
+        self.txw = torch.from_numpy((self._tx_host * sc[:, None]).astype(np.complex64)).to(self.device)
+        tyw = self._ty_host[self._order] * sc[self._order][:, None]
+        self.ty_sw = torch.from_numpy(tyw.astype(np.complex64)).to(self.device)
 
-    def _tx(self, weighted):
+    def _weights_ready(self, weighted):
         w = getattr(self, '_weights', None)
         if not weighted or w is None:
-            return self.tx
+            return False
         if self.txw is None:
             raise OopaoError('weight_vector has %d entries for %d modulation points' % (w.size, self.T))
-        return self.txw
+        return True
+
+    def _tx(self, weighted):
+        return self.txw if self._weights_ready(weighted) else self.tx
+
+    def _ty_sorted(self, weighted):
+        return self.ty_sw if self._weights_ready(weighted) else self.ty_s
 
     def set_mask(self, m, mask=None, factor=None, general=False):
         """Pyramid mask from its phase m [N, N] (and the complex64 mask exp(i m) the reference
@@ -152,10 +188,10 @@ class PWFSEngine:
 
     # -------------------------------------------------------------- execution
     def _knobs(self, B):
-        """Point-chunk sizes: 6 points per columns-pass chunk and per rows-pass task (measured best at
-        n=320, N=1280, T=72; any value gives the same frames up to float summation order)."""
+        """Chunk sizes: 6 points per columns-pass chunk and 6 tip tables per rows-pass task (measured
+        best at n=320, N=1280, T=72; any value gives the same frames up to float summation order)."""
         nchunk = self.nchunk_user or max(1, -(-self.T // 6))
-        ppt = self.pts_per_task_user or max(1, min(self.T, 6))
+        ppt = self.pts_per_task_user or max(1, min(self.P, 6))
         return min(nchunk, self.T), ppt
 
     def _check(self, opd, amp):
@@ -199,15 +235,15 @@ class PWFSEngine:
         B = opd.shape[0]
         if out is None:
             out = torch.empty((B, self.cam, self.cam), dtype=torch.float32, device=self.device)
-        tx = self._tx(weighted)
         if self.mode == 'generic' or raw is not None:
-            return self._frames_generic(opd, scale, amp, out, raw, tx)
+            return self._frames_generic(opd, scale, amp, out, raw, self._tx(weighted))
+        ty = self._ty_sorted(weighted)
         nchunk, ppt = self._knobs(B)
         nb = ctypes.c_size_t(0)
         _lib.call('oopao_pwfs_workspace_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
         ws = self._scratch(nb.value)
-        _lib.call('oopao_pwfs_frames', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(tx),
-                  _ptr(self.ty), _ptr(self.mx), _ptr(self.my), self.n, self.N, self.T, B, self.bin,
+        _lib.call('oopao_pwfs_frames', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
+                  _ptr(self.pstart), _ptr(ty), _ptr(self.mx), _ptr(self.my), self.n, self.N, self.T, B, self.bin,
                   nchunk, ppt, _ptr(ws), ctypes.c_size_t(nb.value), _ptr(out), _stream_ptr(self.device))
         return out
 
@@ -265,8 +301,8 @@ class PWFSEngine:
             nchunk, ppt = self._knobs(B)
             _lib.call('oopao_pwfs_focal_bytes', self.n, self.N, self.T, B, nchunk, crop, ctypes.byref(nb))
             ws = self._scratch(nb.value)
-            _lib.call('oopao_pwfs_focal', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx),
-                      _ptr(self.ty), self.n, self.N, self.T, B, nchunk, ppt, crop, _ptr(ws),
+            _lib.call('oopao_pwfs_focal', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
+                      _ptr(self.pstart), _ptr(self.ty_s), self.n, self.N, self.T, B, nchunk, ppt, crop, _ptr(ws),
                       ctypes.c_size_t(nb.value), _ptr(out), _stream_ptr(self.device))
         else:
             nchunk = self._generic_nchunk(B)
@@ -288,9 +324,9 @@ class PWFSEngine:
         return dict(B=B, nchunk=nchunk, ppt=ppt, inter=inter, part=part)
 
     def rows(self, opd, scale, amp, buf):
-        _lib.call('oopao_pwfs_rows', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx), _ptr(self.ty),
-                  _ptr(self.mx), self.n, self.N, self.T, buf['B'], buf['ppt'], _ptr(buf['inter']),
-                  _stream_ptr(self.device))
+        _lib.call('oopao_pwfs_rows', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
+                  _ptr(self.pstart), _ptr(self.ty_s), _ptr(self.mx), self.n, self.N, self.T, buf['B'], buf['ppt'],
+                  _ptr(buf['inter']), _stream_ptr(self.device))
 
     def cols(self, buf, out):
         _lib.call('oopao_pwfs_cols', _ptr(buf['inter']), _ptr(self.my), self.n, self.N, self.T, buf['B'],
@@ -306,6 +342,7 @@ class SlopesEngine:
         self.w = w
         self.quad = torch.tensor(np.asarray(quad_origins, np.int32).reshape(8), device=self.device)
         self.units = 1.0
+        self._ws = None
 
     def set_valid(self, validI4Q):
         v = np.asarray(validI4Q, bool)
@@ -327,7 +364,11 @@ class SlopesEngine:
             sig2d = torch.empty((B, 2 * self.w, self.w), dtype=torch.float32, device=self.device)
         if sig is None:
             sig = torch.empty((B, self.nsig), dtype=torch.float32, device=self.device)
+        nb = ctypes.c_size_t(0)
+        _lib.call('oopao_pwfs_slopes_workspace_bytes', B, self.w, ctypes.byref(nb))
+        if self._ws is None or self._ws.numel() < nb.value:
+            self._ws = torch.empty(nb.value, dtype=torch.uint8, device=self.device)
         _lib.call('oopao_pwfs_slopes', _ptr(frames), B, self.cam, _ptr(self.valid), self.w, _ptr(self.quad),
                   _ptr(self.ref2d), ctypes.c_float(self.units), _ptr(self.vidx), self.nsig, self.nvalid,
-                  _ptr(sig2d), _ptr(sig), _stream_ptr(self.device))
+                  _ptr(self._ws), ctypes.c_size_t(nb.value), _ptr(sig2d), _ptr(sig), _stream_ptr(self.device))
         return sig2d, sig

@@ -43,7 +43,27 @@ def test_error_path_without_gpu_calls():
     nb = ctypes.c_size_t(0)
     assert L.oopao_pwfs_workspace_bytes(0, 1280, 72, 1, 8, ctypes.byref(nb)) != 0
     assert b'invalid' in L.oopao_last_error()
+    assert L.oopao_pwfs_workspace_bytes(320, 1280, 72, 1, 73, ctypes.byref(nb)) != 0   # nchunk > T
     assert L.oopao_pwfs_workspace_bytes(320, 1280, 72, 1, 12, ctypes.byref(nb)) == 0
     assert nb.value >= 72 * 320 * 1280 * 8
     with pytest.raises(Exception):
         _lib.check(1)
+
+
+def test_tip_groups_default_and_user_paths():
+    # Pyramid.py:1129-1132: theta_i = 2 pi i / T, px = mod cos(theta): points i and T - i share their
+    # tip, so the rows pass needs T/2 + 1 tables; a square user path has 3 distinct x values per side
+    import numpy as np
+    from oopao_amd.pwfs_engine import tip_groups
+    from oracle.pwfs import modulation_path
+    for T in (48, 72, 96):
+        path = modulation_path(3, T)
+        order, pidx, reps = tip_groups(path)
+        assert reps.size == T // 2 + 1
+        assert sorted(order.tolist()) == list(range(T))
+        assert np.all(np.diff(pidx) >= 0) and pidx[-1] == reps.size - 1
+        px = path[order, 0]
+        assert np.max(np.abs(px - path[reps[pidx], 0])) < 1e-12
+    sq = [[x, y] for x in (-1.0, 0.0, 1.0) for y in (-1.0, 0.0, 1.0) if (x, y) != (0.0, 0.0)]
+    _, pidx, reps = tip_groups(np.asarray(sq))
+    assert reps.size == 3
