@@ -106,6 +106,35 @@ def im_leg(dev, n_modes=1000):
             'nSignal': int(wfs.nSignal), 'cond': float(calib.cond)}
 
 
+def adjoint_leg(n_screens=64, steps=10):
+    """Differentiable PWFS (config 1 sensor: 80 px pupil, 20x20 subapertures, 48 points): gradients of
+    ||s(phi) - s_target||^2 w.r.t. 64 seeded phase screens per step through the hand-written adjoint
+    (forward + backward, modulation recomputed in backward)."""
+    import torch
+    from oopao_amd import Telescope, Source, Pyramid
+    from oopao_amd.differentiable import pwfs_signal
+    tel = Telescope(80, 8.0, 1e-3)
+    ngs = Source('I3', 8, display_properties=False)
+    ngs * tel
+    wfs = Pyramid(20, tel, 3, 0.5, n_pix_separation=20, n_pix_edge=10, nTheta_user_defined=48)
+    rs = np.random.RandomState(100)
+    ph = torch.from_numpy((rs.randn(n_screens, 80, 80) * 0.3 * tel.pupil).astype(np.float32)).cuda()
+    ph.requires_grad_(True)
+    tgt = torch.zeros((1, wfs.nSignal), device='cuda')
+    for _ in range(3):
+        ph.grad = None
+        ((pwfs_signal(wfs, ph).double() - tgt) ** 2).sum().backward()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ph.grad = None
+        ((pwfs_signal(wfs, ph).double() - tgt) ** 2).sum().backward()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return {'value': round(n_screens / dt, 1), 'unit': 'gradients/s', 'screens_per_step': n_screens,
+            'ms_per_step': round(dt * 1e3, 3), 'note': 'forward + adjoint, float32, wall clock'}
+
+
 def kl_leg(tel):
     """KL modal basis of the config-2 DM (81 x 81 grid, 5209 actuators) in float64 on the GPU:
     IF spectra, HHt, seed basis and KL diagonalisation (compute_KL_basis defaults)."""
@@ -250,6 +279,7 @@ def main():
         extra['closed_loop'] = closed_loop_leg(tel, ngs, wfs, dev)
         extra['interaction_matrix'] = im_leg(dev)
         extra['kl_basis'] = kl_leg(tel)
+        extra['adjoint'] = adjoint_leg()
 
     out = {
         'metric': 'PWFS frames/s at 80x80 subaps, 72 mod pts',

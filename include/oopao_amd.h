@@ -94,6 +94,20 @@ int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void
                      const void* ty, int n, int N, int T, int B, int nchunk, int pts_per_task, int crop,
                      void* workspace, size_t ws_bytes, float* out, void* stream);
 
+/* Adjoint of oopao_pwfs_frames (no counterpart in the reference, which has no
+ * gradients; it differentiates the forward of Pyramid.py:612-646, 712-757):
+ * for a detector-frame cotangent gframe [B][cam][cam] = dL/dframe, returns
+ * grad [B][n][n] = dL/dopd (times nothing else: scale = dphi/dopd is applied).
+ * Same field tables as oopao_pwfs_frames plus txs [T][n], the tip phasor of
+ * each point in ty's point order (txs[k] = tx[p] for pstart[p] <= k <
+ * pstart[p+1]). The modulation points are recomputed, nothing is stored
+ * between forward and backward. Workspace from oopao_pwfs_adjoint_bytes. */
+int oopao_pwfs_adjoint_bytes(int n, int N, int T, int B, int nchunk, size_t* bytes);
+int oopao_pwfs_adjoint(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
+                       const void* ty, const void* txs, const void* mx, const void* my, const float* gframe, int n,
+                       int N, int T, int B, int nchunk, int pts_per_task, void* workspace, size_t ws_bytes,
+                       float* grad, void* stream);
+
 /* ---------------------------------------------------------------------------
  * General PWFS path: any padded length N <= 2048, any N x N mask (rooftop,
  * rotation, per-quadrant shifts, spatial filter: Pyramid.py:368-480, 1051-1085),

@@ -28,24 +28,9 @@
 // is one contiguous n*32-byte slab for the columns pass.
 #include "common.hpp"
 #include "fft_core.hpp"
-
-#ifndef OOPAO_TPT_1280
-#define OOPAO_TPT_1280 64
-#endif
+#include "pwfs_fast.hpp"
 
 namespace oopao {
-
-// Lane count per transform for each supported padded length N (fast path: the
-// pupil occupies N/4 samples and the detector bins 4 x 4). Chosen so every
-// radix divides N/TPT and the pruned forward + inverse use few stages.
-__host__ __device__ constexpr int fast_tpt(int N) {
-  return N == 64 ? 16 : N == 128 ? 16 : N == 256 ? 32 : N == 320 ? 16 : N == 512 ? 32
-       : N == 640 ? 16 : N == 1024 ? 64 : N == 1280 ? OOPAO_TPT_1280 : N == 1536 ? 64 : N == 2048 ? 64 : 0;
-}
-
-#define OOPAO_FAST_SIZES(X) X(64) X(128) X(256) X(320) X(512) X(640) X(1024) X(1280) X(1536) X(2048)
-
-constexpr int kWaves = 4;  // wavefronts per workgroup
 
 // Pupil-plane field u[b][y][x] = amp[y][x] exp(i scale opd[b][y][x]), once per
 // screen (the reference re-forms it per modulation point, Pyramid.py:619-620).

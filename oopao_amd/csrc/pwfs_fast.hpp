@@ -1,0 +1,24 @@
+// Compile-time configuration of the separable fast PWFS path (N = 4 n, 4 x 4
+// detector binning), shared by the forward kernels and their adjoint.
+#pragma once
+#include "fft_core.hpp"
+
+#ifndef OOPAO_TPT_1280
+#define OOPAO_TPT_1280 64
+#endif
+
+namespace oopao {
+
+// Lane count per transform for each supported padded length N (fast path: the
+// pupil occupies N/4 samples and the detector bins 4 x 4). Chosen so every
+// radix divides N/TPT and the pruned forward + inverse use few stages.
+__host__ __device__ constexpr int fast_tpt(int N) {
+  return N == 64 ? 16 : N == 128 ? 16 : N == 256 ? 32 : N == 320 ? 16 : N == 512 ? 32
+       : N == 640 ? 16 : N == 1024 ? 64 : N == 1280 ? OOPAO_TPT_1280 : N == 1536 ? 64 : N == 2048 ? 64 : 0;
+}
+
+#define OOPAO_FAST_SIZES(X) X(64) X(128) X(256) X(320) X(512) X(640) X(1024) X(1280) X(1536) X(2048)
+
+constexpr int kWaves = 4;  // wavefronts per workgroup
+
+}  // namespace oopao

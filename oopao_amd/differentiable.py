@@ -1,0 +1,68 @@
+"""Differentiable modulated PWFS: torch.autograd.Function over the HIP forward and its adjoint.
+
+The forward is the fast-path frame kernel (oopao_pwfs_frames); the backward is the hand-written
+adjoint (oopao_pwfs_adjoint): w_i = M^H (g . E_i) with the conjugated separable mask, and
+dL/dphi = sum_i 2 Im(w_i conj(u_i)), recomputing every modulation point instead of storing it.
+The slopesMaps post-processing (Pyramid.py:835-854, grabQuadrant 946-961) is written as torch
+operations so that autograd carries the gradient from the slopes to the detector frame.
+
+There is no counterpart in the reference (OOPAO has no gradients); the forward it differentiates is
+Pyramid.pyramid_transform / wfs_measure / signalProcessing (Pyramid.py:612-646, 712-757, 835-854).
+"""
+import numpy as np
+import torch
+
+from .tools import OopaoError
+
+
+class PWFSFrames(torch.autograd.Function):
+    """frames [B, cam, cam] = PWFS(opd [B, n, n]); phase = opd * scale."""
+
+    @staticmethod
+    def forward(ctx, opd, engine, amp, scale):
+        opd = opd.detach().to(torch.float32).contiguous()
+        frames = engine.frames(opd, scale, amp)
+        ctx.save_for_backward(opd)
+        ctx.engine, ctx.amp, ctx.scale = engine, amp, scale
+        return frames
+
+    @staticmethod
+    def backward(ctx, gframes):
+        opd, = ctx.saved_tensors
+        grad = ctx.engine.adjoint(opd, ctx.scale, ctx.amp, gframes.detach().to(torch.float32).contiguous())
+        return grad, None, None, None
+
+
+def pwfs_frames(engine, opd, amp, scale):
+    """Differentiable detector frames for OPD/phase maps opd [B, n, n]."""
+    return PWFSFrames.apply(opd, engine, amp, scale)
+
+
+def slopes_maps(frames, validI4Q, quad, w, ref2d=0.0, units=1.0):
+    """slopesMaps of frames [B, cam, cam] in torch ops (Pyramid.py:838-854): returns
+    (signal_2D [B, 2w, w], signal [B, nSig])."""
+    v = validI4Q.to(frames.dtype)
+    I = [frames[:, quad[k]:quad[k] + w, quad[4 + k]:quad[4 + k] + w] * v for k in range(4)]
+    I1, I2, I3, I4 = I
+    I4Q = I1 + I2 + I3 + I4
+    norma = (I4Q.double().sum(dim=(1, 2)) / v.double().sum()).to(frames.dtype)[:, None, None]
+    Sx = I1 - I2 + I4 - I3
+    Sy = I1 - I4 + I2 - I3
+    s2d = (torch.cat((Sx, Sy), dim=1) / norma - ref2d) * units
+    vs = torch.cat((validI4Q, validI4Q), dim=0).bool()
+    return s2d, s2d[:, vs]
+
+
+def pwfs_signal(wfs, phase):
+    """Differentiable slopes [B, nSignal] of a drop-in Pyramid (slopesMaps post-processing) for phase
+    maps [B, n, n] in radians at the WFS wavelength; gradients flow to ``phase``."""
+    if wfs.postProcessing != 'slopesMaps':
+        raise OopaoError('pwfs_signal supports the slopesMaps post-processing')
+    if wfs.engine.mode != 'fast' or wfs.binning != 1:
+        raise OopaoError('the PWFS adjoint is implemented for the separable fast path without extra binning')
+    phase = phase.reshape(-1, wfs.engine.n, wfs.engine.n)
+    frames = pwfs_frames(wfs.engine, phase, wfs._amp(), 1.0)
+    quad, w = wfs._quadrant_origins()
+    valid = torch.from_numpy(np.asarray(wfs.validI4Q, bool)).to(frames.device)
+    ref = wfs._referenceSignal_2D if wfs._referenceSignal_2D is not None else 0.0
+    return slopes_maps(frames, valid, quad, w, ref, float(wfs.slopesUnits))[1]

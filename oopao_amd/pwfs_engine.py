@@ -124,6 +124,7 @@ class PWFSEngine:
         pstart = np.searchsorted(pidx, np.arange(self.P + 1)).astype(np.int32)
         self.pstart = torch.from_numpy(pstart).to(self.device)
         self.ty_s = torch.from_numpy(ty[order].astype(np.complex64)).to(self.device)
+        self.tx_s = torch.from_numpy(tx[order].astype(np.complex64)).to(self.device)   # per point, same order
         self._make_weighted()
 
     def set_weights(self, w):
@@ -312,6 +313,27 @@ class PWFSEngine:
                       _ptr(self.ty), self.n, self.N, self.T, B, nchunk, crop, _ptr(ws), ctypes.c_size_t(nb.value),
                       _ptr(out), _stream_ptr(self.device))
         return out
+
+    def adjoint(self, opd, scale, amp, gframe, weighted=False):
+        """dL/dopd [B, n, n] for the cotangent gframe = dL/dframes [B, cam, cam] of
+        frames(opd, scale, amp, weighted=weighted) (fast path; the modulation points are recomputed)."""
+        if self.mode != 'fast':
+            raise OopaoError('the PWFS adjoint is implemented for the separable fast path only')
+        opd = self._check(opd, amp)
+        B = opd.shape[0]
+        gframe = gframe.reshape(B, self.cam, self.cam)
+        if gframe.dtype != torch.float32 or not gframe.is_contiguous() or gframe.device != self.device:
+            gframe = gframe.to(self.device, torch.float32).contiguous()
+        nchunk, ppt = self._knobs(B)
+        nb = ctypes.c_size_t(0)
+        _lib.call('oopao_pwfs_adjoint_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
+        ws = self._scratch(nb.value)
+        grad = torch.empty((B, self.n, self.n), dtype=torch.float32, device=self.device)
+        _lib.call('oopao_pwfs_adjoint', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
+                  _ptr(self.pstart), _ptr(self._ty_sorted(weighted)), _ptr(self.tx_s), _ptr(self.mx), _ptr(self.my), _ptr(gframe),
+                  self.n, self.N, self.T, B, nchunk, ppt, _ptr(ws), ctypes.c_size_t(nb.value), _ptr(grad),
+                  _stream_ptr(self.device))
+        return grad
 
     # split passes (same result as frames(); used to time the passes separately)
     def buffers(self, B):
