@@ -69,6 +69,22 @@ class Detector:
         self._frame = None if val is None else torch.as_tensor(np.asarray(val), dtype=torch.float32,
                                                                 device=self.device)
 
+    def relay(self, src):
+        """Detector.relay (Detector.py:148-171): image of the source through its telescope."""
+        if src.tag == 'source':
+            src.optical_path.append([self.tag, self])
+            tel = next(obj for tag, obj in src.optical_path if tag == 'telescope')
+        else:
+            raise OopaoError('science images of asterisms are not computed by oopao_amd')
+        tel.computePSF(detector=self)
+        self.fov_arcsec = tel.xPSF_arcsec[1] - tel.xPSF_arcsec[0]
+        self.fov_rad = tel.xPSF_rad[1] - tel.xPSF_rad[0]
+        if self.integrationTime is not None and self.integrationTime < tel.samplingTime:
+            raise OopaoError('The Detector integration time is smaller than the AO loop sampling Time. ')
+        self._integrated_time += tel.samplingTime
+        self.integrate(tel._PSF)
+        tel._PSF = self._frame
+
     def integrate(self, frame):
         """Detector.integrate + readout (Detector.py:273-354) for a device frame (or a batch)."""
         self.perfect_frame = frame

@@ -118,6 +118,59 @@ class Telescope:
             s._fluxMap = self._refl_dev * flux
             s._refresh_intensity()
 
+    # --------------------------------------------------------------- science
+    def computePSF(self, zeroPaddingFactor=2, detector=None, img_resolution=None):
+        """PSF of the coupled on-axis source (Telescope.py:251-361), computed on the GPU
+        (oopao_amd.science.psf_dev). Sets PSF, PSF_norma and the image axes."""
+        from .science import psf_dev, psf_geometry
+        if detector is not None:
+            zeroPaddingFactor = detector.psf_sampling
+            if detector.resolution is not None:
+                img_resolution = detector.resolution
+            else:
+                img_resolution = int(zeroPaddingFactor * self.resolution)
+                detector.resolution = img_resolution
+        if img_resolution is None:
+            img_resolution = int(zeroPaddingFactor * self.resolution)
+        if self.src is None:
+            raise OopaoError('The telescope was not coupled to any source object! Make sure to couple it with an '
+                             'src object using src*tel')
+        if self.src.tag != 'source':
+            raise OopaoError('PSFs of asterisms are not computed by oopao_amd')
+        src = self.src
+        if np.any(np.asarray(src.coordinates)[:1] != 0) and self.apply_off_axis_tip_tilt:
+            raise OopaoError('off-axis sources are not supported by oopao_amd')
+        if src._OPD is None or src._OPD.dim() != 2:
+            raise OopaoError('computePSF needs a single 2-D OPD on the source (propagate it through the telescope)')
+        amp = torch.sqrt(src._intensity) * self._pupil_dev
+        if self._refl_dev is not None:
+            amp = amp * self._refl_dev
+        psf_geometry(self.resolution, zeroPaddingFactor, img_resolution)     # validates the sizes
+        self._PSF = psf_dev(amp.float(), src._OPD.float(), 2 * np.pi / src.wavelength, self.resolution,
+                            zeroPaddingFactor, img_resolution, self.device)
+        lam_d = src.wavelength / self.D
+        n_pix = int(img_resolution)
+        self.xPSF_rad = [-lam_d * (n_pix / 2 / zeroPaddingFactor), lam_d * (n_pix / 2 / zeroPaddingFactor)]
+        self.yPSF_rad = list(self.xPSF_rad)
+        self.xPSF_arcsec = [v * self.rad2arcsec for v in self.xPSF_rad]
+        self.yPSF_arcsec = list(self.xPSF_arcsec)
+        self.PSF_list = self._PSF
+
+    @property
+    def PSF(self):
+        p = getattr(self, '_PSF', None)
+        return None if p is None else to_host(p)
+
+    @PSF.setter
+    def PSF(self, val):
+        self._PSF = None if val is None else torch.as_tensor(np.asarray(val), dtype=torch.float32,
+                                                              device=self.device)
+
+    @property
+    def PSF_norma(self):
+        p = getattr(self, '_PSF', None)
+        return None if p is None else to_host(p / p.max())
+
     def src_flux_scale(self, src):
         """nPhoton * samplingTime * pixel area (Telescope.py:223)."""
         return float(src.nPhoton * self.samplingTime * (self.D / self.resolution) ** 2)

@@ -1,0 +1,79 @@
+"""Science path on the GPU: PSF of a source through the telescope and the OTF Strehl ratio.
+
+Telescope.PropagateField (Telescope.py:374-423) zero-pads the pupil field
+amp * exp(i phase) to N = zeroPaddingFactor * oversampling * n samples,
+multiplies by the centring phasor exp(-i pi (N+1)/N (x+y)) and keeps the
+centred img x img window of |fft2(.)/N|^2. That is the modulation focal-plane
+camera of the general PWFS kernels with a single "modulation point" whose tip
+and tilt tables are the phasor's row and column factors, so the PSF is
+computed by oopao_pwfs_generic_focal (row transforms of the pupil rows, then
+only the window's column transforms). strehlMeter (tools/tools.py:102-157)
+compares the OTF of the image with the OTF of the diffraction-limited image.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._device import ptr, stream_ptr
+from .tools import OopaoError
+
+
+def psf_geometry(n, zeroPaddingFactor, img_resolution):
+    """(oversampling, padded N, pad offset, image size) of Telescope.PropagateField (Telescope.py:375-393)."""
+    over = 1
+    if zeroPaddingFactor * over < 2:
+        over = int(math.ceil(2.0 / zeroPaddingFactor))
+    if img_resolution is None:
+        img_resolution = zeroPaddingFactor * n
+    if img_resolution > zeroPaddingFactor * n:
+        raise OopaoError('Error: image has too many pixels for this pupil sampling. Try using a pupil mask with '
+                         'more pixels')
+    img_size = int(math.ceil(img_resolution * over))
+    N = int(np.fix(zeroPaddingFactor * over * n))
+    pad = int(math.ceil((N - n) / 2))
+    N = n + 2 * pad
+    if N % 2 or img_size % 2 or n % 2:
+        raise OopaoError('PSF computation supports even pupil, padded and image sizes (n=%d, N=%d, image %d)'
+                         % (n, N, img_size))
+    return over, N, pad, img_size
+
+
+def psf_dev(amp, opd, scale, n, zeroPaddingFactor, img_resolution, device):
+    """|fft2(pad(amp exp(i scale opd)) phasor) / N|^2 on the centred image window, float32 [img, img]."""
+    over, N, pad, img = psf_geometry(n, zeroPaddingFactor, img_resolution)
+    X = np.arange(n) + pad
+    ph = np.exp(-1j * np.pi * (N + 1) / N * X).astype(np.complex64)[None, :]
+    tab = torch.from_numpy(np.ascontiguousarray(ph)).to(device)
+    if not _lib.lib().oopao_pwfs_generic_supported(n, N, 1):
+        raise OopaoError('padded PSF size N=%d outside the supported range' % N)
+    nb = ctypes.c_size_t(0)
+    _lib.call('oopao_pwfs_generic_focal_bytes', n, N, 1, 1, 1, img, ctypes.byref(nb))
+    ws = torch.empty(nb.value, dtype=torch.uint8, device=device)
+    out = torch.empty((1, img, img), dtype=torch.float32, device=device)
+    _lib.call('oopao_pwfs_generic_focal', ptr(opd.contiguous()), ctypes.c_float(scale), ptr(amp.contiguous()),
+              ptr(tab), ptr(tab), n, N, 1, 1, 1, img, ptr(ws), ctypes.c_size_t(nb.value), ptr(out),
+              stream_ptr(device))
+    psf = out[0]
+    if over != 1:
+        r = img // over
+        psf = psf.reshape(r, over, r, over).sum(dim=(1, 3))
+    return psf
+
+
+def strehl_dev(psf, psf_ref, resolution, zeroPaddingFactor=2):
+    """Strehl ratio [%] from the OTFs of an image and of the reference image (tools/tools.py:102-157)."""
+    s = min(psf_ref.shape[0], psf.shape[0])
+
+    def crop(a):
+        c = a.shape[0] // 2
+        return a[c - s // 2:c + s // 2, c - s // 2:c + s // 2]
+    P = crop(psf).double()
+    A = crop(psf_ref).double()
+    otf = torch.abs(torch.fft.fft2(P))
+    otfa = torch.abs(torch.fft.fft2(A))
+    otf = otf / otf.max()
+    otfa = otfa / otfa.max()
+    return float(otf.sum() / otfa.sum() * 100)

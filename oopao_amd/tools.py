@@ -30,3 +30,23 @@ def set_binning(array, binning_factor, mode='sum'):
     shape = (new[0], array.shape[0] // new[0], new[1], array.shape[1] // new[1], new[2])
     r = array.reshape(shape)
     return r.sum(-2).sum(1) if mode == 'sum' else r.mean(-2).mean(1)
+
+
+def strehlMeter(PSF, tel, PSF_ref=None, zeroPaddingFactor=2, display=False, title=''):
+    """Strehl ratio [%] from the OTF of a focal-plane image (tools/tools.py:102-157), on the GPU.
+    Without PSF_ref the diffraction-limited image is computed after resetting the OPD, as the
+    reference does (tel.resetOPD(); tel.computePSF(zeroPaddingFactor))."""
+    import torch
+    from .science import strehl_dev
+    if PSF_ref is None:
+        tel.resetOPD()
+        tel.computePSF(zeroPaddingFactor)
+        ref = tel._PSF
+    else:
+        ref = torch.as_tensor(np.asarray(PSF_ref) if not isinstance(PSF_ref, torch.Tensor) else PSF_ref,
+                              device=tel.device)
+    psf = PSF if isinstance(PSF, torch.Tensor) else torch.as_tensor(np.asarray(PSF), device=tel.device)
+    sr = strehl_dev(psf.to(tel.device), ref.to(tel.device), tel.resolution, zeroPaddingFactor)
+    if display:
+        print('Strehl ratio [%] : ', sr)
+    return sr
