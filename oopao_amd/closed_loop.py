@@ -8,7 +8,10 @@ closed_loop/run_cl.py:176-217): per iteration
     frame_delay == 2: s = wfs.signal
 with the OPD sum, PWFS frame, slopes, reconstructor GEMV and integrator update
 kept on the device (no host round trip per iteration). Records the
-turbulence and residual wavefront error in the pupil (nm RMS) per iteration.
+turbulence and residual wavefront error in the pupil (nm RMS) per iteration,
+and, with ``strehl_every`` > 0, the Strehl ratio of the source's image behind
+the DM (ngs*tel*dm*cam with strehlMeter against the diffraction-limited image,
+tutorials/AO_closed_loop_Pyramid_WFS.py:609-614), computed on the GPU.
 """
 import numpy as np
 import torch
@@ -23,7 +26,7 @@ def _pupil_std_nm(opd, pupil_mask, npix):
 
 
 def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100, frame_delay=2,
-                    record_slopes=False):
+                    record_slopes=False, strehl_every=0, psf_sampling=2):
     dev = wfs.device
     R = reconstructor if isinstance(reconstructor, torch.Tensor) else torch.from_numpy(
         np.ascontiguousarray(reconstructor, dtype=np.float32))
@@ -47,6 +50,14 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
     scale = 2 * np.pi / ngs.wavelength
     eng, se = wfs.engine, wfs.slopes_engine
     buf = eng.buffers(1)
+    if strehl_every:
+        from .science import psf_dev, strehl_dev
+        amp_psf = (torch.sqrt(ngs._intensity) * pupil).float().contiguous()
+        psf_scale = 2 * np.pi / ngs.wavelength
+        zero = torch.zeros((tel.resolution, tel.resolution), dtype=torch.float32, device=dev)
+        psf_ref = psf_dev(amp_psf, zero, psf_scale, tel.resolution, psf_sampling, None, dev)
+        n_sr = (n_iter + strehl_every - 1) // strehl_every
+        sr = torch.zeros(n_sr, dtype=torch.float64, device=dev)
     frame = torch.empty((1, eng.cam, eng.cam), dtype=torch.float32, device=dev)
     s2d = torch.empty((1, 2 * se.w, se.w), dtype=torch.float32, device=dev)
     s = torch.empty((1, se.nsig), dtype=torch.float32, device=dev)
@@ -57,6 +68,9 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
         dm_opd = dm.opd_from_coefs(coefs)
         opd = ((turb + dm_opd) * pupil).contiguous()
         wfe_res[i] = _pupil_std_nm(opd, pmask, npix)
+        if strehl_every and i % strehl_every == 0:
+            psf = psf_dev(amp_psf, opd, psf_scale, tel.resolution, psf_sampling, None, dev)
+            sr[i // strehl_every] = strehl_dev(psf, psf_ref, tel.resolution, psf_sampling, as_tensor=True)
         eng.rows(opd[None], scale, amp, buf)
         eng.cols(buf, frame)
         se.slopes(frame, s2d, s)
@@ -76,4 +90,7 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
                coefs=coefs.double().cpu().numpy())
     if record_slopes:
         out['slopes'] = slopes.double().cpu().numpy()
+    if strehl_every:
+        out['strehl'] = sr.cpu().numpy()
+        out['strehl_every'] = strehl_every
     return out

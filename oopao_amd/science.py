@@ -63,8 +63,9 @@ def psf_dev(amp, opd, scale, n, zeroPaddingFactor, img_resolution, device):
     return psf
 
 
-def strehl_dev(psf, psf_ref, resolution, zeroPaddingFactor=2):
-    """Strehl ratio [%] from the OTFs of an image and of the reference image (tools/tools.py:102-157)."""
+def strehl_dev(psf, psf_ref, resolution, zeroPaddingFactor=2, as_tensor=False):
+    """Strehl ratio [%] from the OTFs of an image and of the reference image (tools/tools.py:102-157);
+    a 0-d device tensor with ``as_tensor`` (no host synchronisation)."""
     s = min(psf_ref.shape[0], psf.shape[0])
 
     def crop(a):
@@ -76,4 +77,5 @@ def strehl_dev(psf, psf_ref, resolution, zeroPaddingFactor=2):
     otfa = torch.abs(torch.fft.fft2(A))
     otf = otf / otf.max()
     otfa = otfa / otfa.max()
-    return float(otf.sum() / otfa.sum() * 100)
+    sr = otf.sum() / otfa.sum() * 100
+    return sr if as_tensor else float(sr)

@@ -98,8 +98,12 @@ def test_closed_loop_matches_oracle_loop():
         ratio = [lay.vX * tel.samplingTime / lay.pixel_size, lay.vY * tel.samplingTime / lay.pixel_size]
         layers.append(LayerOracle(ms, lay.A, lay.B, lay.innerMask, lay.outerMask, rs, ratio))
     n_iter, gain = 25, 0.4
-    out = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=gain, n_iter=n_iter, record_slopes=True)
+    out = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=gain, n_iter=n_iter, record_slopes=True, strehl_every=1)
     o = _oracle_pwfs(wfs, tel, ngs)
+    from oracle import science as osc
+    amp = np.sqrt(o.flux) * tel.pupil
+    psf_flat = osc.propagate_field(amp, np.zeros((32, 32)), 2)
+    sr_trace = []
     od = DMOracle(32, 8, 8, 0.35)
     modes = od.modes()
     coefs = np.zeros(dm.nValidAct)
@@ -111,6 +115,7 @@ def test_closed_loop_matches_oracle_loop():
             L.update()
         opd = (atmosphere_opd(layers, [0.7, 0.3], 32) + od.opd(modes, coefs)) * tel.pupil
         wfe.append(np.std(opd[tel.pupil > 0]) * 1e9)
+        sr_trace.append(osc.strehl(osc.propagate_field(amp, opd * 2 * np.pi / o.wavelength, 2), psf_flat))
         s = o.measure_opd(opd)[1]
         s_trace.append(s)
         coefs = coefs - gain * R @ sig
@@ -122,3 +127,7 @@ def test_closed_loop_matches_oracle_loop():
     print('worst per-iteration slope rel_l2', err)
     assert err < 1e-4
     assert out['wfe_residual'][-1] < 0.5 * out['wfe_atmosphere'][-1]      # the loop corrects
+    # Strehl trace (strehlMeter on the image behind the DM, tools/tools.py:102-157)
+    sr_err = np.max(np.abs(out['strehl'] - np.array(sr_trace)))
+    print('Strehl trace device', out['strehl'][-3:], 'oracle', sr_trace[-3:], 'max abs diff [%]', sr_err)
+    assert sr_err < 1e-3
