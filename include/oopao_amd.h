@@ -162,11 +162,14 @@ int oopao_gemv(const float* M, const float* x, float* y, const float* y0, int R,
                float beta, void* stream);
 
 /* Separable OPD of a Cartesian, zero-misregistration Gaussian DM for B
- * coefficient grids C[B][nA][nA] (row = actuator row along Y):
- * opd[B][n][n] = gy C gx^T with gx, gy [n][nA]; tmp [B][nA][n] scratch. Equal
- * to modes @ coefs for that DM (DeformableMirror.py:572-600, 651-686). */
-int oopao_dm_separable_opd(const float* C, const float* gx, const float* gy, float* tmp, float* opd, int n,
-                           int nA, int B, void* stream);
+ * coefficient vectors coefs [nAct][B] (column j = one command vector):
+ * opd[B][n][n] = gy C_j gx^T, where C_j[a][b] = coefs[grid[a*nA+b]][j] on the
+ * nA x nA actuator grid (grid = actuator index, or -1 where there is no valid
+ * actuator; row a along Y), gxT [nA][n] (transposed), gy [n][nA]; tmp scratch of
+ * B (nA*nA + n*nA) floats. Equal to modes @ coefs for that DM
+ * (DeformableMirror.py:572-600, 651-686). */
+int oopao_dm_separable_opd(const float* coefs, const int* grid, const float* gxT, const float* gy, float* tmp,
+                           float* opd, int n, int nA, int B, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Atmosphere frozen-flow update (replaces Atmosphere.add_row / updateLayer and
@@ -191,6 +194,18 @@ int oopao_atm_minmax(const double* map, int count, double* mm, void* stream);
  * clip=True). accumulate = 0 overwrites out. */
 int oopao_atm_layer_opd(const double* map, int M, const double* mm, double tx, double ty, int r0, int c0, double w,
                         float* out, int n, int accumulate, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Closed loop (tutorials/AO_closed_loop_Pyramid_WFS.py:601-638): residual OPD
+ * opd = (turb + dm) * pupil (dm may be null) and the RMS wavefront errors over
+ * the pupil in nm, np.std(tel.OPD[pupil > 0]) * 1e9 for the turbulence alone
+ * and for the residual, written to device scalars wfe_turb / wfe_res (either
+ * may be null). npix = number of pupil pixels; workspace of
+ * oopao_loop_residual_workspace_bytes bytes.
+ * ------------------------------------------------------------------------- */
+int oopao_loop_residual_workspace_bytes(size_t* bytes);
+int oopao_loop_residual(const float* turb, const float* dm, const float* pupil, float* opd, int count, double npix,
+                        void* workspace, double* wfe_turb, double* wfe_res, void* stream);
 
 #ifdef __cplusplus
 }

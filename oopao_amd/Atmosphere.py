@@ -195,12 +195,34 @@ class Atmosphere:
     def _add_row(self, lay, step):
         """Atmosphere.add_row (Atmosphere.py:410-421) on the device."""
         sx, sy = int(step[0]), int(step[1])
-        # blocking copy: the draw must reach the device before the host draws the next ring
-        lay.xi.copy_(torch.from_numpy(lay.randomState.normal(size=lay.outer_idx.numel())))
+        self._stage_noise(lay, lay.randomState.normal(size=lay.outer_idx.numel()))
         _lib.call('oopao_atm_add_row', ptr(lay.map), ptr(lay.tmp), lay.M, sx, sy, ptr(lay.inner_idx),
                   lay.inner_idx.numel(), ptr(lay.outer_idx), lay.outer_idx.numel(), ptr(lay.A_dev), ptr(lay.B_dev),
                   ptr(lay.xi), ptr(lay.z), ptr(lay.X), ptr(lay.mm), stream_ptr(self.device))
         lay.map, lay.tmp = lay.tmp, lay.map
+
+    _NOISE_RING = 8
+
+    def _stage_noise(self, lay, draw):
+        """Host draw -> device noise vector without stalling the host: the draw is written into one of a
+        ring of pinned buffers and copied asynchronously on the stream; a buffer is rewritten only after
+        the event of its previous copy has completed."""
+        ring = getattr(lay, '_xi_ring', None)
+        if ring is None or ring[0].numel() != draw.size:
+            ring = [torch.empty(draw.size, dtype=torch.float64, pin_memory=True) for _ in range(self._NOISE_RING)]
+            lay._xi_ring = ring
+            lay._xi_events = [None] * self._NOISE_RING
+            lay._xi_k = 0
+        k = lay._xi_k
+        lay._xi_k = (k + 1) % self._NOISE_RING
+        ev = lay._xi_events[k]
+        if ev is not None:
+            ev.synchronize()
+        ring[k].numpy()[:] = draw
+        lay.xi.copy_(ring[k], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.device))
+        lay._xi_events[k] = ev
 
     def updateLayer(self, lay, shift=None):
         """Atmosphere.updateLayer (Atmosphere.py:447-516)."""

@@ -152,11 +152,13 @@ class DeformableMirror:
         nA_y, nA_x = len(uy), len(ux)
         if nA_x != nA_y:
             return None
-        flat = torch.from_numpy((iy * nA_x + ix).astype(np.int64)).to(self.device)
+        flat = iy * nA_x + ix
+        grid = -np.ones(nA_x * nA_y, np.int32)
+        grid[flat] = np.arange(flat.size, dtype=np.int32)
         return dict(nA=nA_x,
-                    gx=torch.from_numpy(np.ascontiguousarray(gx, np.float32)).to(self.device),
+                    gxT=torch.from_numpy(np.ascontiguousarray(gx.T, np.float32)).to(self.device),
                     gy=torch.from_numpy(np.ascontiguousarray(gy, np.float32)).to(self.device),
-                    flat=flat)
+                    grid=torch.from_numpy(grid).to(self.device))
 
     # ---------------------------------------------------------------- modes
     def _build_modes(self):
@@ -255,12 +257,14 @@ class DeformableMirror:
         if not self.use_dense:
             s = self._sep
             nA = s['nA']
-            C = torch.zeros((B, nA * nA), dtype=torch.float32, device=self.device)
-            C[:, s['flat']] = cb.t()
-            tmp = torch.empty((B, nA, n), dtype=torch.float32, device=self.device)
+            cc = cb.contiguous()
             opd = torch.empty((B, n, n), dtype=torch.float32, device=self.device)
-            _lib.call('oopao_dm_separable_opd', ptr(C), ptr(s['gx']), ptr(s['gy']), ptr(tmp), ptr(opd), n, nA, B,
-                      stream_ptr(self.device))
+            for j0 in range(0, B, 65535):
+                j1 = min(B, j0 + 65535)
+                cj = cc if (j0 == 0 and j1 == B) else cc[:, j0:j1].contiguous()
+                tmp = torch.empty(((j1 - j0) * (nA * nA + n * nA),), dtype=torch.float32, device=self.device)
+                _lib.call('oopao_dm_separable_opd', ptr(cj), ptr(s['grid']), ptr(s['gxT']), ptr(s['gy']), ptr(tmp),
+                          ptr(opd[j0:j1]), n, nA, j1 - j0, stream_ptr(self.device))
         else:
             if self._modes_dev is None:
                 self._modes_dev = self._build_modes()

@@ -52,6 +52,9 @@ SIGNATURES = {
                                                ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_generic_focal': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                          c_int, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p]),
+    'oopao_loop_residual_workspace_bytes': (c_int, [ctypes.POINTER(c_size_t)]),
+    'oopao_loop_residual': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p,
+                                    c_void_p, c_void_p]),
     'oopao_atm_add_row':(c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'oopao_atm_minmax': (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
@@ -60,8 +63,8 @@ SIGNATURES = {
     'oopao_dm_gaussian_modes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_double, c_double, c_double,
                                         c_float, c_void_p]),
     'oopao_gemv': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_float, c_void_p]),
-    'oopao_dm_separable_opd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
-                                       c_void_p]),
+    'oopao_dm_separable_opd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                       c_int, c_void_p]),
 }
 
 

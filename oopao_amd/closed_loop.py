@@ -13,16 +13,13 @@ and, with ``strehl_every`` > 0, the Strehl ratio of the source's image behind
 the DM (ngs*tel*dm*cam with strehlMeter against the diffraction-limited image,
 tutorials/AO_closed_loop_Pyramid_WFS.py:609-614), computed on the GPU.
 """
+import ctypes
+
 import numpy as np
 import torch
 
 from . import _lib
 from ._device import ptr, stream_ptr
-
-
-def _pupil_std_nm(opd, pupil_mask, npix):
-    sel = opd[pupil_mask]
-    return sel.double().std(unbiased=False) * 1e9
 
 
 def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100, frame_delay=2,
@@ -36,8 +33,12 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
         raise ValueError('reconstructor shape %s does not match (nAct=%d, nSignal=%d)'
                          % (tuple(R.shape), dm.nValidAct, wfs.nSignal))
     pupil = tel._pupil_dev
-    pmask = pupil > 0
-    npix = int(pmask.sum())
+    pupil_f = (pupil > 0).float().contiguous()
+    npix = int((pupil > 0).sum())
+    nb = ctypes.c_size_t(0)
+    _lib.call('oopao_loop_residual_workspace_bytes', ctypes.byref(nb))
+    resid_ws = torch.empty(nb.value, dtype=torch.uint8, device=dev)
+    opd = torch.empty((tel.resolution, tel.resolution), dtype=torch.float32, device=dev)
     coefs = torch.as_tensor(np.asarray(dm.coefs, np.float32), device=dev).clone()
     sig_buf = torch.zeros(nSig, dtype=torch.float32, device=dev)
     new_coefs = torch.empty_like(coefs)
@@ -64,10 +65,10 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
     for i in range(n_iter):
         atm.update()
         turb = atm._OPD_dev
-        wfe_atm[i] = _pupil_std_nm(turb * pupil, pmask, npix)
         dm_opd = dm.opd_from_coefs(coefs)
-        opd = ((turb + dm_opd) * pupil).contiguous()
-        wfe_res[i] = _pupil_std_nm(opd, pmask, npix)
+        # opd = (turb + dm) * pupil and both RMS errors, written into the traces on the device
+        _lib.call('oopao_loop_residual', ptr(turb), ptr(dm_opd), ptr(pupil_f), ptr(opd), opd.numel(),
+                  ctypes.c_double(npix), ptr(resid_ws), ptr(wfe_atm[i:]), ptr(wfe_res[i:]), stream_ptr(dev))
         if strehl_every and i % strehl_every == 0:
             psf = psf_dev(amp_psf, opd, psf_scale, tel.resolution, psf_sampling, None, dev)
             sr[i // strehl_every] = strehl_dev(psf, psf_ref, tel.resolution, psf_sampling, as_tensor=True)

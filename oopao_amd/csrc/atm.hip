@@ -40,27 +40,73 @@ __global__ void k_scatter(double* __restrict__ map, const int* __restrict__ idx,
 }
 
 // X[r] = sum_k A[r][k] z[k] + sum_k B[r][k] xi[k]  (float64, one wavefront per row)
+// Partial dot product of one row with a vector: each lane reads double2 pairs, four
+// independent partial sums keep several 16-byte loads in flight.
+__device__ __forceinline__ double row_dot(const double* __restrict__ a, const double* __restrict__ v, int K,
+                                          int lane) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const int K2 = K / 2;
+  const double2* a2 = reinterpret_cast<const double2*>(a);
+  const double2* v2 = reinterpret_cast<const double2*>(v);
+  const bool aligned = ((reinterpret_cast<size_t>(a) | reinterpret_cast<size_t>(v)) & 15) == 0;
+  int k = lane;
+  if (aligned) {
+    for (; k + 192 < K2; k += 256) {
+      const double2 x0 = a2[k], x1 = a2[k + 64], x2 = a2[k + 128], x3 = a2[k + 192];
+      const double2 y0 = v2[k], y1 = v2[k + 64], y2 = v2[k + 128], y3 = v2[k + 192];
+      s0 = fma(x0.x, y0.x, fma(x0.y, y0.y, s0));
+      s1 = fma(x1.x, y1.x, fma(x1.y, y1.y, s1));
+      s2 = fma(x2.x, y2.x, fma(x2.y, y2.y, s2));
+      s3 = fma(x3.x, y3.x, fma(x3.y, y3.y, s3));
+    }
+    for (; k < K2; k += 64) {
+      const double2 x0 = a2[k], y0 = v2[k];
+      s0 = fma(x0.x, y0.x, fma(x0.y, y0.y, s0));
+    }
+    if ((K & 1) && lane == 0) s1 = fma(a[K - 1], v[K - 1], s1);
+  } else {
+    for (k = lane; k < K; k += 64) s0 = fma(a[k], v[k], s0);
+  }
+  return (s0 + s1) + (s2 + s3);
+}
+
+// X[r] = A[r,:] z + B[r,:] xi (the outer-ring extrusion, float64), one wavefront per row.
 __global__ void __launch_bounds__(256) k_extrude(const double* __restrict__ A, const double* __restrict__ z,
                                                  const double* __restrict__ Bm, const double* __restrict__ xi,
                                                  double* __restrict__ X, int R, int KA, int KB) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = blockIdx.x * 4 + wave;
   if (r >= R) return;
-  const double* a = A + (size_t)r * KA;
-  const double* b = Bm + (size_t)r * KB;
-  double acc = 0.0;
-  for (int k = lane; k < KA; k += 64) acc = fma(a[k], z[k], acc);
-  for (int k = lane; k < KB; k += 64) acc = fma(b[k], xi[k], acc);
+  double acc = row_dot(A + (size_t)r * KA, z, KA, lane) + row_dot(Bm + (size_t)r * KB, xi, KB, lane);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
   if (lane == 0) X[r] = acc;
 }
 
-// mm[0] = min(map), mm[1] = max(map); one workgroup.
-__global__ void __launch_bounds__(1024) k_minmax(const double* __restrict__ map, int n, double* __restrict__ mm) {
-  __shared__ double lo[1024], hi[1024];
+// min / max of a map in three launches: sentinels, per-block reductions merged with 64-bit
+// atomic min / max on an order-preserving integer image of the doubles (the result does not
+// depend on the order of the merges), conversion back in place.
+__device__ __forceinline__ unsigned long long dbl_ord(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dbl_unord(unsigned long long o) {
+  const unsigned long long u = (o >> 63) ? (o & 0x7FFFFFFFFFFFFFFFull) : ~o;
+  return __longlong_as_double((long long)u);
+}
+
+__global__ void k_minmax_init(unsigned long long* __restrict__ mm) {
+  if (threadIdx.x == 0) {
+    mm[0] = ~0ull;
+    mm[1] = 0ull;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_minmax(const double* __restrict__ map, int n,
+                                                unsigned long long* __restrict__ mm) {
+  __shared__ double lo[256], hi[256];
   double a = INFINITY, b = -INFINITY;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double v = map[i];
     a = fmin(a, v);
     b = fmax(b, v);
@@ -76,9 +122,28 @@ __global__ void __launch_bounds__(1024) k_minmax(const double* __restrict__ map,
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    mm[0] = lo[0];
-    mm[1] = hi[0];
+    atomicMin(&mm[0], dbl_ord(lo[0]));
+    atomicMax(&mm[1], dbl_ord(hi[0]));
   }
+}
+
+__global__ void k_minmax_final(unsigned long long* __restrict__ mm) {
+  if (threadIdx.x < 2) {
+    const double v = dbl_unord(mm[threadIdx.x]);
+    reinterpret_cast<double*>(mm)[threadIdx.x] = v;
+  }
+}
+
+static int launch_minmax(const double* map, int n, double* mm, hipStream_t st) {
+  unsigned long long* mb = reinterpret_cast<unsigned long long*>(mm);
+  hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, st, mb);
+  OOPAO_LAUNCH_CHECK();
+  const int blocks = (int)(((size_t)n + 1023) / 1024 < 256 ? ((size_t)n + 1023) / 1024 : 256);
+  hipLaunchKernelGGL(k_minmax, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, map, n, mb);
+  OOPAO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, st, mb);
+  OOPAO_LAUNCH_CHECK();
+  return OK;
 }
 
 __device__ __forceinline__ double cubic(double x, double f0, double f1, double f2, double f3) {
@@ -140,16 +205,12 @@ int oopao_atm_add_row(const double* map, double* tmp, int M, int sx, int sy, con
   OOPAO_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scatter, dim3((n_outer + 255) / 256), dim3(256), 0, st, tmp, outer_idx, X, n_outer);
   OOPAO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_minmax, dim3(1), dim3(1024), 0, st, tmp, M * M, mm);
-  OOPAO_LAUNCH_CHECK();
-  return OK;
+  return launch_minmax(tmp, M * M, mm, st);
 }
 
 int oopao_atm_minmax(const double* map, int count, double* mm, void* stream) {
   OOPAO_CHECK_ARG(map && mm && count > 0, "invalid arguments");
-  hipLaunchKernelGGL(k_minmax, dim3(1), dim3(1024), 0, reinterpret_cast<hipStream_t>(stream), map, count, mm);
-  OOPAO_LAUNCH_CHECK();
-  return OK;
+  return launch_minmax(map, count, mm, reinterpret_cast<hipStream_t>(stream));
 }
 
 int oopao_atm_layer_opd(const double* map, int M, const double* mm, double tx, double ty, int r0, int c0, double w,

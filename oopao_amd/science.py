@@ -8,7 +8,9 @@ camera of the general PWFS kernels with a single "modulation point" whose tip
 and tilt tables are the phasor's row and column factors, so the PSF is
 computed by oopao_pwfs_generic_focal (row transforms of the pupil rows, then
 only the window's column transforms). strehlMeter (tools/tools.py:102-157)
-compares the OTF of the image with the OTF of the diffraction-limited image.
+compares the OTF of the image with the OTF of the diffraction-limited image; the
+OTF magnitude |fft2(image)| comes from the same focal kernel (a unit phasor, the
+whole image as window).
 """
 import ctypes
 import math
@@ -41,26 +43,61 @@ def psf_geometry(n, zeroPaddingFactor, img_resolution):
     return over, N, pad, img_size
 
 
+class _FocalEngine:
+    """Cached tables and workspace for |fft2(pad(amp exp(i scale opd)) tab_y tab_x) / N|^2 on a centred window
+    (the general focal kernel with one point)."""
+
+    def __init__(self, n, N, img, tab, device):
+        self.n, self.N, self.img, self.device = n, N, img, device
+        if not _lib.lib().oopao_pwfs_generic_supported(n, N, 1):
+            raise OopaoError('padded size N=%d outside the supported range' % N)
+        self.tab = torch.from_numpy(np.ascontiguousarray(tab.astype(np.complex64)[None, :])).to(device)
+        nb = ctypes.c_size_t(0)
+        _lib.call('oopao_pwfs_generic_focal_bytes', n, N, 1, 1, 1, img, ctypes.byref(nb))
+        self.nb = nb.value
+        self.ws = torch.empty(self.nb, dtype=torch.uint8, device=device)
+        self.zero = torch.zeros((n, n), dtype=torch.float32, device=device)
+
+    def __call__(self, amp, opd, scale):
+        out = torch.empty((1, self.img, self.img), dtype=torch.float32, device=self.device)
+        opd = self.zero if opd is None else opd.contiguous()
+        _lib.call('oopao_pwfs_generic_focal', ptr(opd), ctypes.c_float(scale), ptr(amp.contiguous()),
+                  ptr(self.tab), ptr(self.tab), self.n, self.N, 1, 1, 1, self.img, ptr(self.ws),
+                  ctypes.c_size_t(self.nb), ptr(out), stream_ptr(self.device))
+        return out[0]
+
+
+_ENGINES = {}
+
+
+def _engine(key, make):
+    e = _ENGINES.get(key)
+    if e is None:
+        e = _ENGINES[key] = make()
+    return e
+
+
 def psf_dev(amp, opd, scale, n, zeroPaddingFactor, img_resolution, device):
     """|fft2(pad(amp exp(i scale opd)) phasor) / N|^2 on the centred image window, float32 [img, img]."""
     over, N, pad, img = psf_geometry(n, zeroPaddingFactor, img_resolution)
-    X = np.arange(n) + pad
-    ph = np.exp(-1j * np.pi * (N + 1) / N * X).astype(np.complex64)[None, :]
-    tab = torch.from_numpy(np.ascontiguousarray(ph)).to(device)
-    if not _lib.lib().oopao_pwfs_generic_supported(n, N, 1):
-        raise OopaoError('padded PSF size N=%d outside the supported range' % N)
-    nb = ctypes.c_size_t(0)
-    _lib.call('oopao_pwfs_generic_focal_bytes', n, N, 1, 1, 1, img, ctypes.byref(nb))
-    ws = torch.empty(nb.value, dtype=torch.uint8, device=device)
-    out = torch.empty((1, img, img), dtype=torch.float32, device=device)
-    _lib.call('oopao_pwfs_generic_focal', ptr(opd.contiguous()), ctypes.c_float(scale), ptr(amp.contiguous()),
-              ptr(tab), ptr(tab), n, N, 1, 1, 1, img, ptr(ws), ctypes.c_size_t(nb.value), ptr(out),
-              stream_ptr(device))
-    psf = out[0]
+    key = ('psf', str(device), n, N, img)
+
+    def make():
+        X = np.arange(n) + pad
+        return _FocalEngine(n, N, img, np.exp(-1j * np.pi * (N + 1) / N * X), device)
+    psf = _engine(key, make)(amp.float(), opd.float(), scale)
     if over != 1:
         r = img // over
         psf = psf.reshape(r, over, r, over).sum(dim=(1, 3))
     return psf
+
+
+def otf_ratio_dev(img):
+    """sum |OTF| / max |OTF| of an image, OTF = fft2(image), through the general focal kernel (0-d tensor)."""
+    s = img.shape[0]
+    e = _engine(('otf', str(img.device), s), lambda: _FocalEngine(s, s, s, np.ones(s), img.device))
+    otf = torch.sqrt(e(img.float(), None, 0.0).double())
+    return otf.sum() / otf.max()
 
 
 def strehl_dev(psf, psf_ref, resolution, zeroPaddingFactor=2, as_tensor=False):
@@ -70,12 +107,6 @@ def strehl_dev(psf, psf_ref, resolution, zeroPaddingFactor=2, as_tensor=False):
 
     def crop(a):
         c = a.shape[0] // 2
-        return a[c - s // 2:c + s // 2, c - s // 2:c + s // 2]
-    P = crop(psf).double()
-    A = crop(psf_ref).double()
-    otf = torch.abs(torch.fft.fft2(P))
-    otfa = torch.abs(torch.fft.fft2(A))
-    otf = otf / otf.max()
-    otfa = otfa / otfa.max()
-    sr = otf.sum() / otfa.sum() * 100
+        return a[c - s // 2:c + s // 2, c - s // 2:c + s // 2].contiguous()
+    sr = otf_ratio_dev(crop(psf)) / otf_ratio_dev(crop(psf_ref)) * 100
     return sr if as_tensor else float(sr)
