@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 N_RES, N_SUBAP, SEP, EDGE, MOD, T_POINTS, DIAM, TS = 320, 80, 80, 40, 3, 72, 8.0, 1e-3
 VALU_PEAK_TFLOPS = 78.6    # FP32 vector FMA, non-packed (256 CU x 64 lanes x 2 x 2.4 GHz)
 HBM_PEAK_GBS = 8000.0
+BF16_MFMA_PEAK_TFLOPS = 2500.0   # dense BF16 matrix-core peak
 
 
 def synthetic_opd(n, seed=10, rms_nm=150.0):
@@ -170,6 +171,106 @@ def kl_leg(tel):
             'peak_mem_GB': round(torch.cuda.max_memory_allocated() / 1e9, 2)}
 
 
+def _event_time(f, n):
+    import torch
+    f()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(n):
+        f()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / n * 1e-3
+
+
+def _rel(got, exact):
+    d = (got.double() - exact)
+    return {'rel_l2': float(d.norm() / exact.norm()),
+            'max_rel': float(d.abs().max() / exact.abs().max())}
+
+
+def contraction_leg(tel, dev, n_modes=1000):
+    """Dense influence-function and reconstructor contractions in BF16 on the matrix cores
+    (FP32 accumulation), each with its error against the float64 product.
+
+    * config-2 DM (81 x 81 grid, 5209 actuators, 320 x 320 pupil): OPD of n_modes unit-RMS seeded
+      commands (the batched IM / KL-mode OPDs) as one GEMM [102400 x 5209] x [5209 x n_modes], BF16 and
+      split BF16, beside the float32 GEMV route (one matrix read per command);
+    * ELT scale: 39 m pupil, 480 x 480, 30 % obstruction, 80 x 80 DM (BF16 IF [230400 x nAct]) applied to
+      one command (GEMV), and a BF16 reconstructor MVM [nAct x 15700] against the float32 GEMV."""
+    import ctypes
+    import torch
+    from oopao_amd import Telescope, DeformableMirror, _lib
+    from oopao_amd._device import ptr, stream_ptr
+    from oopao_amd.contract import Contraction, PackedBF16
+    out = {}
+    dm = DeformableMirror(tel, N_SUBAP, 0.35, print_dm_properties=False)
+    modes = dm._build_modes()
+    R, K = modes.shape
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    X = torch.randn(K, n_modes, device=dev, generator=g) * 20e-9
+    C = Contraction(PackedBF16(modes, split=True))
+    opd = torch.empty((n_modes, R), dtype=torch.float32, device=dev)
+    cols = torch.arange(0, n_modes, n_modes // 8, device=dev)
+    exact = (modes.double() @ X[:, cols].double()).t()
+    res = {'R': R, 'K': K, 'B': n_modes}
+    for prec, terms in (('bf16', 1), ('bf16x3', 3)):
+        t = _event_time(lambda: C.gemm(X, precision=prec, out=opd), 10)
+        tf = 2.0 * R * K * n_modes / t / 1e12
+        res[prec] = {'ms': round(t * 1e3, 4), 'TFLOP_s': round(tf, 1),
+                     'mfma_TFLOP_s_issued': round(tf * terms, 1),
+                     'frac_bf16_peak': round(tf * terms / BF16_MFMA_PEAK_TFLOPS, 4),
+                     'err_vs_f64': _rel(opd[cols], exact)}
+    y = torch.empty(R, dtype=torch.float32, device=dev)
+    x0 = X[:, 0].contiguous()
+    t = _event_time(lambda: _lib.call('oopao_gemv', ptr(modes), ptr(x0), ptr(y), None, R, K, 1.0, 0.0,
+                                      stream_ptr(dev)), 20)
+    res['fp32_gemv_per_command_ms'] = round(t * 1e3, 4)
+    res['fp32_gemv_GBs'] = round(R * K * 4 / t / 1e9, 1)
+    out['config2_dm_gemm'] = res
+    del C, opd, exact, modes
+    torch.cuda.empty_cache()
+
+    elt = Telescope(480, 39.0, 1e-3, centralObstruction=0.3)
+    dm5 = DeformableMirror(elt, N_SUBAP, 0.35, print_dm_properties=False)
+    modes = dm5._build_modes()
+    R, K = modes.shape
+    C = Contraction(PackedBF16(modes, split=False))
+    x = (torch.randn(K, device=dev, generator=g) * 20e-9).contiguous()
+    y = torch.empty(R, dtype=torch.float32, device=dev)
+    t16 = _event_time(lambda: C.gemv(x, out=y), 20)
+    err = _rel(y, modes.double() @ x.double())
+    y32 = torch.empty(R, dtype=torch.float32, device=dev)
+    t32 = _event_time(lambda: _lib.call('oopao_gemv', ptr(modes), ptr(x), ptr(y32), None, R, K, 1.0, 0.0,
+                                        stream_ptr(dev)), 20)
+    out['elt_dm_gemv'] = {'R': R, 'K': K, 'bf16_ms': round(t16 * 1e3, 4),
+                          'bf16_GBs': round(C.A.hi.numel() * 2 / t16 / 1e9, 1), 'bf16_err_vs_f64': err,
+                          'fp32_ms': round(t32 * 1e3, 4), 'fp32_GBs': round(R * K * 4 / t32 / 1e9, 1),
+                          'fp32_err_vs_f64': _rel(y32, modes.double() @ x.double())}
+    del C, modes
+    torch.cuda.empty_cache()
+    nsig = 15700
+    Rm = torch.randn(K, nsig, device=dev, generator=g) * 1e-3
+    s = torch.randn(nsig, device=dev, generator=g)
+    C = Contraction(PackedBF16(Rm, split=False))
+    c = torch.empty(K, dtype=torch.float32, device=dev)
+    t16 = _event_time(lambda: C.gemv(s, out=c), 50)
+    ex = Rm.double() @ s.double()
+    c32 = torch.empty(K, dtype=torch.float32, device=dev)
+    t32 = _event_time(lambda: _lib.call('oopao_gemv', ptr(Rm), ptr(s), ptr(c32), None, K, nsig, 1.0, 0.0,
+                                        stream_ptr(dev)), 50)
+    out['elt_reconstructor_mvm'] = {'R': K, 'K': nsig, 'bf16_us': round(t16 * 1e6, 2),
+                                    'bf16_GBs': round(C.A.hi.numel() * 2 / t16 / 1e9, 1),
+                                    'bf16_err_vs_f64': _rel(c, ex), 'fp32_us': round(t32 * 1e6, 2),
+                                    'fp32_GBs': round(K * nsig * 4 / t32 / 1e9, 1),
+                                    'fp32_err_vs_f64': _rel(c32, ex)}
+    del C, Rm
+    torch.cuda.empty_cache()
+    return out
+
+
 def measured_traffic(kernels):
     """HBM bytes per launch of the given kernels from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by the copy-kernel calibration)."""
@@ -296,6 +397,7 @@ def main():
         extra['interaction_matrix'] = im_leg(dev)
         extra['kl_basis'] = kl_leg(tel)
         extra['adjoint'] = adjoint_leg()
+        extra['contractions'] = contraction_leg(tel, dev)
 
     out = {
         'metric': 'PWFS frames/s at 80x80 subaps, 72 mod pts',

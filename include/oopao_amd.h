@@ -161,6 +161,38 @@ int oopao_dm_gaussian_modes(float* modes, int n, int nact, const float* x0, cons
 int oopao_gemv(const float* M, const float* x, float* y, const float* y0, int R, int K, float alpha,
                float beta, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * BF16 contractions on the matrix cores (FP32 accumulation), for the dense
+ * influence-function product OPD = modes @ coefs (DeformableMirror.py:651-686)
+ * and the reconstructor product reconstructor @ signal
+ * (tutorials/AO_closed_loop_Pyramid_WFS.py:620-624):
+ *     out[b][r] = alpha * sum_k A[r][k] X[k][b].
+ * Operands are stored in the MFMA fragment layout: for Q[i][k] (I x K) padded
+ * to 32 IT x 16 KT, element ((kt IT + it) 64 + l) 8 + j holds
+ * Q[32 it + l%32][16 kt + 8 (l/32) + j] as BF16 (round to nearest even); lo
+ * (optional) holds bf16(Q - hi). The layout has no counterpart in the
+ * reference (which keeps modes as a float64 [n^2, nAct] array).
+ * ------------------------------------------------------------------------- */
+
+/* Bytes of one packed BF16 operand (hi or lo) for an I x K matrix. */
+int oopao_bf16_packed_bytes(int I, int K, size_t* bytes);
+
+/* Pack Q[i][k] = src[i*si + k*sk] (float32, I x K) into hi (and lo if non-NULL). */
+int oopao_bf16_pack(const float* src, long long si, long long sk, int I, int K, void* hi, void* lo, void* stream);
+
+/* out[b*ldo + r] = alpha * sum_k A[r][k] X[k][b] for A packed as R x K and X
+ * packed as B x K (Q[b][k] = X[k][b]). terms = 1: hi*hi (BF16); terms = 3:
+ * hi*hi + hi*lo + lo*hi (split BF16, needs both lo parts). ldo >= R. */
+int oopao_gemm_bf16(const void* a_hi, const void* a_lo, int R, int K, const void* x_hi, const void* x_lo, int B,
+                    float* out, long long ldo, float alpha, int terms, void* stream);
+
+/* y[r] = alpha * sum_k A_hi[r][k] x[k] + beta * y0[r] (y0 may be NULL) with A
+ * packed as R x K (hi only), x float32 [K] (16-byte aligned); workspace of
+ * oopao_gemv_bf16_workspace_bytes bytes (fixed-order partial sums). */
+int oopao_gemv_bf16_workspace_bytes(int R, int K, size_t* bytes);
+int oopao_gemv_bf16(const void* a_hi, int R, int K, const float* x, float* y, const float* y0, float alpha,
+                    float beta, void* ws, size_t ws_bytes, void* stream);
+
 /* Separable OPD of a Cartesian, zero-misregistration Gaussian DM for B
  * coefficient vectors coefs [nAct][B] (column j = one command vector):
  * opd[B][n][n] = gy C_j gx^T, where C_j[a][b] = coefs[grid[a*nA+b]][j] on the

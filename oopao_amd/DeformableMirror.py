@@ -7,7 +7,10 @@ Geometry and influence functions follow the reference (306-340, 359-386,
   Gaussian IF factorises, G_k(X, Y) = gx(X - x0_k) gy(Y - y0_k), so
   OPD = Gy C Gx^T with C the coefficient grid (k_sep_stage1/2 in dm.hip);
 * dense: OPD = modes @ coefs with the [n^2, nAct] float32 modes resident in HBM
-  (user-defined modes, rotated/anamorphic grids), one GEMV per column.
+  (user-defined modes, rotated/anamorphic grids), one GEMV per column; with
+  ``contraction = 'bf16'`` or ``'bf16x3'`` the modes are packed once to BF16
+  (split BF16 for 'bf16x3') and the product runs on the matrix cores with FP32
+  accumulation (contract.py); ``use_dense = True`` selects this route for any DM.
 
 A wrong coefficient length raises OopaoError (the reference calls sys.exit(0),
 DeformableMirror.py:683-685).
@@ -110,6 +113,8 @@ class DeformableMirror:
             self.nValidAct = m.shape[1]
         self._sep = None if self._user_modes else self._separable_factors()
         self.use_dense = self._sep is None
+        self.contraction = 'fp32'
+        self._contract = None
         self._coefs = np.zeros(self.nValidAct)
         self.coefs = np.zeros(self.nValidAct)
         self.current_coefs = self.coefs.copy()
@@ -221,6 +226,7 @@ class DeformableMirror:
         self._user_modes = True
         self._sep = None
         self.use_dense = True
+        self._contract = None
 
     # ---------------------------------------------------------------- coefs
     @property
@@ -265,6 +271,8 @@ class DeformableMirror:
                 tmp = torch.empty(((j1 - j0) * (nA * nA + n * nA),), dtype=torch.float32, device=self.device)
                 _lib.call('oopao_dm_separable_opd', ptr(cj), ptr(s['grid']), ptr(s['gxT']), ptr(s['gy']), ptr(tmp),
                           ptr(opd[j0:j1]), n, nA, j1 - j0, stream_ptr(self.device))
+        elif self.contraction != 'fp32':
+            opd = self._bf16_contraction().gemm(cb, precision=self.contraction).reshape(B, n, n)
         else:
             if self._modes_dev is None:
                 self._modes_dev = self._build_modes()
@@ -275,6 +283,17 @@ class DeformableMirror:
                           self.nValidAct, 1.0, 0.0, stream_ptr(self.device))
             opd = opd.reshape(B, n, n)
         return opd if batched else opd[0]
+
+    def _bf16_contraction(self):
+        """Modes packed once to split BF16 MFMA fragments (contract.py), kept until the modes change."""
+        from .contract import PRECISIONS, Contraction, PackedBF16
+        if self.contraction not in PRECISIONS:
+            raise OopaoError("contraction must be 'fp32', 'bf16' or 'bf16x3', got %r" % (self.contraction,))
+        if self._contract is None:
+            if self._modes_dev is None:
+                self._modes_dev = self._build_modes()
+            self._contract = Contraction(PackedBF16(self._modes_dev, split=True))
+        return self._contract
 
     @property
     def OPD(self):

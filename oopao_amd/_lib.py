@@ -63,6 +63,14 @@ SIGNATURES = {
     'oopao_dm_gaussian_modes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_double, c_double, c_double,
                                         c_float, c_void_p]),
     'oopao_gemv': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_float, c_void_p]),
+    'oopao_bf16_packed_bytes': (c_int, [c_int, c_int, ctypes.POINTER(c_size_t)]),
+    'oopao_bf16_pack': (c_int, [c_void_p, ctypes.c_longlong, ctypes.c_longlong, c_int, c_int, c_void_p, c_void_p,
+                                c_void_p]),
+    'oopao_gemm_bf16': (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                ctypes.c_longlong, c_float, c_int, c_void_p]),
+    'oopao_gemv_bf16_workspace_bytes': (c_int, [c_int, c_int, ctypes.POINTER(c_size_t)]),
+    'oopao_gemv_bf16': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_float, c_float, c_void_p,
+                                c_size_t, c_void_p]),
     'oopao_dm_separable_opd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
                                        c_int, c_void_p]),
 }

@@ -7,7 +7,9 @@ closed_loop/run_cl.py:176-217): per iteration
     dm.coefs = dm.coefs - gain * reconstructor @ s
     frame_delay == 2: s = wfs.signal
 with the OPD sum, PWFS frame, slopes, reconstructor GEMV and integrator update
-kept on the device (no host round trip per iteration). Records the
+kept on the device (no host round trip per iteration); with
+``reconstructor_precision='bf16'`` the reconstructor is held in BF16 (half the
+bytes per iteration, FP32 accumulation). Records the
 turbulence and residual wavefront error in the pupil (nm RMS) per iteration,
 and, with ``strehl_every`` > 0, the Strehl ratio of the source's image behind
 the DM (ngs*tel*dm*cam with strehlMeter against the diffraction-limited image,
@@ -23,7 +25,7 @@ from ._device import ptr, stream_ptr
 
 
 def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100, frame_delay=2,
-                    record_slopes=False, strehl_every=0, psf_sampling=2):
+                    record_slopes=False, strehl_every=0, psf_sampling=2, reconstructor_precision='fp32'):
     dev = wfs.device
     R = reconstructor if isinstance(reconstructor, torch.Tensor) else torch.from_numpy(
         np.ascontiguousarray(reconstructor, dtype=np.float32))
@@ -32,6 +34,12 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
     if nAct != dm.nValidAct or nSig != wfs.nSignal:
         raise ValueError('reconstructor shape %s does not match (nAct=%d, nSignal=%d)'
                          % (tuple(R.shape), dm.nValidAct, wfs.nSignal))
+    if reconstructor_precision not in ('fp32', 'bf16'):
+        raise ValueError("reconstructor_precision must be 'fp32' or 'bf16', got %r" % (reconstructor_precision,))
+    recon_bf16 = None
+    if reconstructor_precision == 'bf16':
+        from .contract import Contraction, PackedBF16
+        recon_bf16 = Contraction(PackedBF16(R, split=False))
     pupil = tel._pupil_dev
     pupil_f = (pupil > 0).float().contiguous()
     npix = int((pupil > 0).sum())
@@ -79,9 +87,12 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
             slopes[i] = s[0]
         if frame_delay == 1:
             sig_buf.copy_(s[0])
-        # coefs <- coefs - gain * R @ sig_buf   (R [nAct, nSig] row-major GEMV)
-        _lib.call('oopao_gemv', ptr(R), ptr(sig_buf), ptr(new_coefs), ptr(coefs), nAct, nSig, float(-gain), 1.0,
-                  stream_ptr(dev))
+        # coefs <- coefs - gain * R @ sig_buf   (R [nAct, nSig] row-major GEMV, float32 or BF16 matrix)
+        if recon_bf16 is not None:
+            recon_bf16.gemv(sig_buf, y0=coefs, alpha=-gain, beta=1.0, out=new_coefs)
+        else:
+            _lib.call('oopao_gemv', ptr(R), ptr(sig_buf), ptr(new_coefs), ptr(coefs), nAct, nSig, float(-gain), 1.0,
+                      stream_ptr(dev))
         coefs, new_coefs = new_coefs, coefs
         if frame_delay == 2:
             sig_buf.copy_(s[0])

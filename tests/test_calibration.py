@@ -77,18 +77,9 @@ def test_interaction_matrix_matches_oracle():
     assert rel_l2(calib_pp.D, Dpp) < 1e-4
 
 
-@pytest.mark.gpu
-def test_closed_loop_matches_oracle_loop():
-    from oopao_amd import InteractionMatrix
-    from oopao_amd.closed_loop import run_closed_loop
-    from oracle.dm import DMOracle
-    from oracle.atmosphere import LayerOracle, atmosphere_opd
-    seed = 10
-    tel, ngs, atm, dm, wfs = _small_loop_system(seed)
-    calib = InteractionMatrix(ngs, tel, dm, wfs, np.eye(dm.nValidAct), 20e-9, nMeasurements=100)
-    calib.nTrunc = 10
-    R = calib.Mtrunc
-    # oracle atmosphere from the same initial state and RandomState streams
+def _oracle_layers(seed, tel, atm):
+    """Oracle layers from the device atmosphere's current state and the same RandomState streams."""
+    from oracle.atmosphere import LayerOracle
     layers = []
     for i in range(atm.nLayer):
         lay = getattr(atm, 'layer_%d' % (i + 1))
@@ -97,30 +88,49 @@ def test_closed_loop_matches_oracle_loop():
         rs.normal(size=lay.B.shape[1])
         ratio = [lay.vX * tel.samplingTime / lay.pixel_size, lay.vY * tel.samplingTime / lay.pixel_size]
         layers.append(LayerOracle(ms, lay.A, lay.B, lay.innerMask, lay.outerMask, rs, ratio))
-    n_iter, gain = 25, 0.4
-    out = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=gain, n_iter=n_iter, record_slopes=True, strehl_every=1)
-    o = _oracle_pwfs(wfs, tel, ngs)
+    return layers
+
+
+def _oracle_loop(layers, tel, ngs, dm, wfs, R, n_iter, gain, strehl=False):
+    """The closed loop of tutorials/AO_closed_loop_Pyramid_WFS.py:601-638 on the CPU oracle."""
+    from oracle.dm import DMOracle
+    from oracle.atmosphere import atmosphere_opd
     from oracle import science as osc
+    o = _oracle_pwfs(wfs, tel, ngs)
     amp = np.sqrt(o.flux) * tel.pupil
-    psf_flat = osc.propagate_field(amp, np.zeros((32, 32)), 2)
-    sr_trace = []
+    psf_flat = osc.propagate_field(amp, np.zeros((32, 32)), 2) if strehl else None
     od = DMOracle(32, 8, 8, 0.35)
     modes = od.modes()
     coefs = np.zeros(dm.nValidAct)
     sig = np.zeros(wfs.nSignal)
-    wfe = []
-    s_trace = []
+    wfe, s_trace, sr_trace = [], [], []
     for it in range(n_iter):
         for L in layers:
             L.update()
         opd = (atmosphere_opd(layers, [0.7, 0.3], 32) + od.opd(modes, coefs)) * tel.pupil
         wfe.append(np.std(opd[tel.pupil > 0]) * 1e9)
-        sr_trace.append(osc.strehl(osc.propagate_field(amp, opd * 2 * np.pi / o.wavelength, 2), psf_flat))
+        if strehl:
+            sr_trace.append(osc.strehl(osc.propagate_field(amp, opd * 2 * np.pi / o.wavelength, 2), psf_flat))
         s = o.measure_opd(opd)[1]
         s_trace.append(s)
         coefs = coefs - gain * R @ sig
         sig = s
-    wfe = np.array(wfe)
+    return np.array(wfe), s_trace, sr_trace
+
+
+@pytest.mark.gpu
+def test_closed_loop_matches_oracle_loop():
+    from oopao_amd import InteractionMatrix
+    from oopao_amd.closed_loop import run_closed_loop
+    seed = 10
+    tel, ngs, atm, dm, wfs = _small_loop_system(seed)
+    calib = InteractionMatrix(ngs, tel, dm, wfs, np.eye(dm.nValidAct), 20e-9, nMeasurements=100)
+    calib.nTrunc = 10
+    R = calib.Mtrunc
+    n_iter, gain = 25, 0.4
+    layers = _oracle_layers(seed, tel, atm)
+    out = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=gain, n_iter=n_iter, record_slopes=True, strehl_every=1)
+    wfe, s_trace, sr_trace = _oracle_loop(layers, tel, ngs, dm, wfs, R, n_iter, gain, strehl=True)
     print('residual wfe (nm) device', out['wfe_residual'][-5:], 'oracle', wfe[-5:])
     assert np.max(np.abs(out['wfe_residual'] - wfe) / wfe) < 1e-4
     err = max(rel_l2(out['slopes'][k], s_trace[k]) for k in range(n_iter))
@@ -131,3 +141,29 @@ def test_closed_loop_matches_oracle_loop():
     sr_err = np.max(np.abs(out['strehl'] - np.array(sr_trace)))
     print('Strehl trace device', out['strehl'][-3:], 'oracle', sr_trace[-3:], 'max abs diff [%]', sr_err)
     assert sr_err < 1e-3
+
+
+@pytest.mark.gpu
+def test_closed_loop_bf16_reconstructor():
+    """BF16 reconstructor (FP32 accumulation) against the oracle loop driven by the BF16-rounded
+    reconstructor (same tolerances as the float32 loop), and the stated distance of that loop from the
+    float64-reconstructor loop."""
+    from oopao_amd import InteractionMatrix
+    from oopao_amd.closed_loop import run_closed_loop
+    from oracle.contract import bf16_rne
+    seed = 11
+    tel, ngs, atm, dm, wfs = _small_loop_system(seed)
+    calib = InteractionMatrix(ngs, tel, dm, wfs, np.eye(dm.nValidAct), 20e-9, nMeasurements=100)
+    calib.nTrunc = 10
+    R = calib.Mtrunc
+    R16 = bf16_rne(R.astype(np.float32)).astype(np.float64)
+    print('reconstructor BF16 rel-L2', rel_l2(R16, R))
+    n_iter, gain = 25, 0.4
+    layers = _oracle_layers(seed, tel, atm)
+    out = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=gain, n_iter=n_iter, record_slopes=True,
+                          reconstructor_precision='bf16')
+    wfe16, s16, _ = _oracle_loop(layers, tel, ngs, dm, wfs, R16, n_iter, gain)
+    print('residual wfe (nm) device bf16', out['wfe_residual'][-5:], 'oracle bf16', wfe16[-5:])
+    assert np.max(np.abs(out['wfe_residual'] - wfe16) / wfe16) < 1e-4
+    assert max(rel_l2(out['slopes'][k], s16[k]) for k in range(n_iter)) < 1e-4
+    assert out['wfe_residual'][-1] < 0.5 * out['wfe_atmosphere'][-1]
