@@ -296,7 +296,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--batch', type=int, default=8, help='screens per call for the batched form')
     ap.add_argument('--no-extra', action='store_true',
-                    help='skip the closed-loop, interaction-matrix and KL-basis legs')
+                    help='skip the closed-loop, interaction-matrix, KL-basis, adjoint and contraction legs')
+    ap.add_argument('--legs', default='closed_loop,interaction_matrix,kl_basis,adjoint,contractions',
+                    help='comma-separated extra legs to run')
     ap.add_argument('--calib-copy', action='store_true',
                     help='also run one elementwise copy of 256 MiB (counter calibration for rocprof --pmc)')
     args = ap.parse_args()
@@ -393,11 +395,17 @@ def main():
 
     extra = {}
     if not args.no_extra:
-        extra['closed_loop'] = closed_loop_leg(tel, ngs, wfs, dev)
-        extra['interaction_matrix'] = im_leg(dev)
-        extra['kl_basis'] = kl_leg(tel)
-        extra['adjoint'] = adjoint_leg()
-        extra['contractions'] = contraction_leg(tel, dev)
+        legs = set(args.legs.split(','))
+        if 'closed_loop' in legs:
+            extra['closed_loop'] = closed_loop_leg(tel, ngs, wfs, dev)
+        if 'interaction_matrix' in legs:
+            extra['interaction_matrix'] = im_leg(dev)
+        if 'kl_basis' in legs:
+            extra['kl_basis'] = kl_leg(tel)
+        if 'adjoint' in legs:
+            extra['adjoint'] = adjoint_leg()
+        if 'contractions' in legs:
+            extra['contractions'] = contraction_leg(tel, dev)
 
     out = {
         'metric': 'PWFS frames/s at 80x80 subaps, 72 mod pts',

@@ -167,11 +167,12 @@ int oopao_gemv(const float* M, const float* x, float* y, const float* y0, int R,
  * and the reconstructor product reconstructor @ signal
  * (tutorials/AO_closed_loop_Pyramid_WFS.py:620-624):
  *     out[b][r] = alpha * sum_k A[r][k] X[k][b].
- * Operands are stored in the MFMA fragment layout: for Q[i][k] (I x K) padded
- * to 32 IT x 16 KT, element ((kt IT + it) 64 + l) 8 + j holds
- * Q[32 it + l%32][16 kt + 8 (l/32) + j] as BF16 (round to nearest even); lo
- * (optional) holds bf16(Q - hi). The layout has no counterpart in the
- * reference (which keeps modes as a float64 [n^2, nAct] array).
+ * Operands are stored in the MFMA fragment layout: for Q[i][k] (I x K),
+ * IT = ceil(I/32), KT = ceil(K/16), element ((kt IT + it) 64 + l) 8 + j holds
+ * Q[32 it + l%32][16 kt + 8 (l/32) + j] as BF16 (round to nearest even) for
+ * kt < 8 ceil(KT/8) + 8, zero outside Q; lo (optional) holds bf16(Q - hi). The
+ * layout has no counterpart in the reference (which keeps modes as a float64
+ * [n^2, nAct] array).
  * ------------------------------------------------------------------------- */
 
 /* Bytes of one packed BF16 operand (hi or lo) for an I x K matrix. */

@@ -63,8 +63,9 @@ class PackedBF16:
         t = self.hi if part == 'hi' else self.lo
         if t is None:
             raise OopaoError('no lo part was packed')
-        IT, KT = (self.I + 31) // 32, (self.K + 15) // 16
-        q = t.float().reshape(KT, IT, 2, 32, 8).permute(1, 3, 0, 2, 4).reshape(IT * 32, KT * 16)
+        IT = (self.I + 31) // 32
+        KTs = t.numel() // (IT * 512)     # stored k fragments (zero padding beyond K included)
+        q = t.float().reshape(KTs, IT, 2, 32, 8).permute(1, 3, 0, 2, 4).reshape(IT * 32, KTs * 16)
         return q[:self.I, :self.K]
 
 

@@ -10,28 +10,39 @@
 //
 // Storage: A and X are converted once to BF16 in the MFMA fragment layout of
 // v_mfma_f32_32x32x16_bf16. For a matrix Q[i][k] (i = row of A, or column b of X;
-// k = contraction index) padded to 32*IT x 16*KT, the fragment (kt, it) holds
-// 64 lanes x 8 values, lane l carrying Q[32 it + l%32][16 kt + 8 (l/32) + j],
+// k = contraction index), IT = ceil(I/32), KT = ceil(K/16), the fragment (kt, it)
+// holds 64 lanes x 8 values, lane l carrying Q[32 it + l%32][16 kt + 8 (l/32) + j],
 // j = 0..7, and fragments are stored kt-major: element
-//     ((kt * IT + it) * 64 + l) * 8 + j.
-// A fragment is one 1 KB contiguous block, so every global load is one 16-byte
-// load per lane, fully coalesced, and an LDS copy of it is read back conflict-free.
+//     ((kt * IT + it) * 64 + l) * 8 + j,
+// for kt < KTs = 8 ceil(KT/8) + 8; everything outside Q is zero (the extra k
+// fragments let a GEMM run whole groups of pipeline stages and prefetch past the end
+// without bounds checks). A fragment is one 1 KB contiguous block: one 16-byte load
+// per lane, fully coalesced, and it lands in LDS exactly as the DMA path writes it
+// (lane l's 16 bytes at base + 16 l), to be read back conflict-free.
 // Split-BF16: Q = hi + lo with hi = bf16(Q), lo = bf16(Q - hi) (round to
 // nearest even); the product uses hi*hi + hi*lo + lo*hi. BF16 keeps 8
 // significant bits (unit roundoff 2^-8), so the split keeps Q to 2^-16 and the
 // dropped lo*lo term is at most about 2^-16 |A||X|.
 //
-// Kernels:
-//  * k_pack_bf16   : FP32 [I x K] (any strides) -> hi (and lo) fragments.
-//  * k_gemm_bf16   : 512 threads (8 wavefronts, 4 x 2), 256 b x 256 r output tile,
-//                    each wavefront 2 x 4 MFMA 32x32 blocks; fragments staged in
-//                    LDS, two buffers, the next stage's global loads in flight
-//                    during the current stage's MFMAs; workgroup order remapped so
-//                    that the workgroups sharing an A slab run on one XCD.
-//  * k_gemv_bf16   : single right-hand side (HBM-bound): one wavefront per
-//                    (row fragment, 32-fragment K chunk), 1 KB coalesced loads,
-//                    fixed-order partial sums, then k_gemv_finish.
+// Kernels (GEMM output tile 256 b x 256 r per workgroup; fragments streamed by
+// global_load_lds_dwordx4 into a ring of LDS stages, one k fragment per stage, held in
+// distinct LDS arrays so the compiler knows which DMA writes which slot; workgroup
+// order remapped so that the workgroups reading one A slab run on one XCD):
+//  * k_pack_bf16      : FP32 [I x K] (any strides) -> hi (and lo) fragments.
+//  * k_gemm_bf16_pipe : BF16 (hi*hi). 512 threads, 8 wavefronts (4 x 2), each 2 x 4
+//                       MFMA 32x32 blocks, ring of 8 stages, the next stage's fragment
+//                       reads interleaved with the current stage's MFMAs.
+//  * k_gemm_bf16_big  : split BF16 (3 products). 256 threads, 4 wavefronts (2 x 2),
+//                       each 4 x 4 blocks (256 accumulators in AGPRs, one wavefront per
+//                       SIMD): 16 fragments read per 48 MFMAs; ring of 4 stages.
+//  * k_gemv_bf16      : single right-hand side (HBM-bound): one wavefront per
+//                       (row fragment, 32-fragment K chunk), 1 KB coalesced loads,
+//                       fixed-order partial sums, then k_gemv_finish.
+// Accumulation order over k is the same in every kernel, so a product does not depend
+// on the tile shape that computed it.
 #include "common.hpp"
+
+#include <type_traits>
 
 namespace oopao {
 
@@ -52,6 +63,9 @@ __device__ __forceinline__ unsigned bf16_rne(float x) {
   u += 0x7FFFu + ((u >> 16) & 1u);
   return u >> 16;
 }
+
+// s_waitcnt immediate (gfx9 encoding) that waits for vmcnt <= n and leaves expcnt/lgkmcnt free
+constexpr int vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) {
   return __builtin_bit_cast(bf16x8, v);
@@ -85,61 +99,71 @@ __global__ void __launch_bounds__(256) k_pack_bf16(const float* __restrict__ src
                              lw[6] | (lw[7] << 16));
 }
 
-// out[b * ldo + r] = alpha * sum_k A[r][k] X[k][b]; A fragments (IR x KT), X fragments (IB x KT).
-// SPLIT = 1: hi*hi; SPLIT = 2: hi*hi + hi*lo + lo*hi. KS k fragments per LDS stage.
-template <int SPLIT, int KS>
-__global__ void __launch_bounds__(kGemmThreads) k_gemm_bf16(const uint4* __restrict__ ah, const uint4* __restrict__ al,
-                                                            const uint4* __restrict__ xh, const uint4* __restrict__ xl,
-                                                            int IR, int IB, int KT, float* __restrict__ out,
-                                                            long long ldo, int R, int B, float alpha, int nbt,
-                                                            int nrt) {
-  // [buffer][term][k fragment][tile fragment: 8 of X then 8 of A][lane]
-  __shared__ uint4 lds[2][SPLIT][KS][kTF][64];
-  constexpr int kPer = 2 * SPLIT * KS;
-  static_assert(kTB == kGemmThreads / 64 && kTR == kTB, "one X and one A fragment per wavefront and stage slot");
+// Same product with the LDS ring held in NBUF distinct LDS arrays (one per slot) and the
+// main loop unrolled over the ring, so every slot is a compile-time object: the compiler's
+// alias tracking then knows which DMA writes which slot and adds no wait for a read of
+// another slot. Fragment reads are ordinary LDS loads issued one stage ahead (two register
+// sets), so the reads of stage k + 1 overlap the MFMAs of stage k.
+template <int SPLIT, int NBUF>
+__global__ void __launch_bounds__(kGemmThreads) k_gemm_bf16_pipe(const uint4* __restrict__ ah,
+                                                                 const uint4* __restrict__ al,
+                                                                 const uint4* __restrict__ xh,
+                                                                 const uint4* __restrict__ xl, int IR, int IB, int KT,
+                                                                 float* __restrict__ out, long long ldo, int R, int B,
+                                                                 float alpha, int nbt, int nrt) {
+  constexpr int kSlot = SPLIT * kTF * 64;  // uint4 per slot
+  __shared__ uint4 S0[kSlot], S1[kSlot], S2[kSlot], S3[kSlot], S4[kSlot], S5[kSlot], S6[kSlot], S7[kSlot];
+  static_assert(NBUF == 4 || NBUF == 8, "ring of 4 or 8 slots");
+  static_assert(SPLIT * NBUF * kTF * 1024 <= 131072, "ring fits in 128 KB of LDS");
+  constexpr int kLoads = 2 * SPLIT;
+  constexpr int kFr = 6 * SPLIT;  // fragments read per wavefront and stage
+  static_assert(kTB == kGemmThreads / 64 && kTR == kTB, "one X and one A fragment per wavefront and stage");
 
-  // XCD-aware order: workgroup i runs on XCD i % 8; give each XCD a contiguous run of
-  // tiles (b fastest) so the workgroups that read one A slab share that XCD's L2.
   const int G = nbt * nrt;
   const int i = blockIdx.x;
   const int per_xcd = (G + 7) / 8;
   const int tall = (G % 8) ? (G % 8) : 8;
-  const int xcd = i & 7, slot = i >> 3;
-  const int t = (xcd < tall) ? xcd * per_xcd + slot : tall * per_xcd + (xcd - tall) * (per_xcd - 1) + slot;
+  const int xcd = i & 7, slotid = i >> 3;
+  const int t = (xcd < tall) ? xcd * per_xcd + slotid : tall * per_xcd + (xcd - tall) * (per_xcd - 1) + slotid;
   const int rt = t / nbt, bt = t % nbt;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
-  const int wb = w & 3, wr = w >> 2;  // wavefront owns b fragments 2wb..2wb+1, r fragments 4wr..4wr+3
+  const int wb = w & 3, wr = w >> 2;
 
-  const int nst = (KT + KS - 1) / KS;
-  // Stage chunk q of this thread (kPer = 2 SPLIT KS): wavefront w copies fragment w of X
-  // (q even) and fragment w of A (q odd) for term q / (2 KS) and k fragment (q / 2) % KS.
-  const int itx = bt * kTB + w, ita = rt * kTR + w;
-  const bool okx = itx < IB, oka = ita < IR;
-  const uint4* srcx[SPLIT];
-  const uint4* srca[SPLIT];
-  srcx[0] = xh + (size_t)itx * 64 + lane;
-  srca[0] = ah + (size_t)ita * 64 + lane;
-  if constexpr (SPLIT == 2) {
-    srcx[1] = xl + (size_t)itx * 64 + lane;
-    srca[1] = al + (size_t)ita * 64 + lane;
-  }
-  uint4 pre[kPer];
-#define OOPAO_STAGE_LOAD(st_)                                                               \
-  _Pragma("unroll") for (int q = 0; q < kPer; ++q) {                                        \
-    const int s_ = q / (2 * KS), kk_ = (q >> 1) % KS;                                       \
-    const int kt_ = (st_) * KS + kk_;                                                       \
-    const bool isx_ = (q & 1) == 0;                                                         \
-    const bool ok_ = kt_ < KT && (isx_ ? okx : oka);                                        \
-    const size_t off_ = (size_t)kt_ * 64 * (isx_ ? IB : IR);                                \
-    pre[q] = ok_ ? (isx_ ? srcx[s_] : srca[s_])[off_] : make_uint4(0u, 0u, 0u, 0u);         \
-  }
-#define OOPAO_STAGE_STORE(buf_)                                                             \
-  _Pragma("unroll") for (int q = 0; q < kPer; ++q) {                                        \
-    const int s_ = q / (2 * KS), kk_ = (q >> 1) % KS;                                       \
-    lds[buf_][s_][kk_][(q & 1) ? kTB + w : w][lane] = pre[q];                               \
-  }
+  const int itx = min(bt * kTB + w, IB - 1), ita = min(rt * kTR + w, IR - 1);
+  const uint4* gx0 = xh + (size_t)itx * 64 + lane;
+  const uint4* ga0 = ah + (size_t)ita * 64 + lane;
+  const uint4* gx1 = (SPLIT == 2) ? xl + (size_t)itx * 64 + lane : nullptr;
+  const uint4* ga1 = (SPLIT == 2) ? al + (size_t)ita * 64 + lane : nullptr;
+  const size_t sx = (size_t)IB * 64, sa = (size_t)IR * 64;
+
+  uint4* const slots[8] = {S0, S1, S2, S3, S4, S5, S6, S7};
+
+  // stage layout inside a slot: [term][fragment: 8 of X then 8 of A][lane]
+  auto issue = [&](int kt, uint4* dst) {
+    __builtin_amdgcn_global_load_lds(gx0 + kt * sx, (__attribute__((address_space(3))) void*)(dst + w * 64), 16, 0,
+                                     0);
+    __builtin_amdgcn_global_load_lds(ga0 + kt * sa,
+                                     (__attribute__((address_space(3))) void*)(dst + (kTB + w) * 64), 16, 0, 0);
+    if constexpr (SPLIT == 2) {
+      __builtin_amdgcn_global_load_lds(gx1 + kt * sx,
+                                       (__attribute__((address_space(3))) void*)(dst + (kTF + w) * 64), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(ga1 + kt * sa,
+                                       (__attribute__((address_space(3))) void*)(dst + (kTF + kTB + w) * 64), 16, 0,
+                                       0);
+    }
+  };
+  // fragments of one stage: [term][2 of X, 4 of A]
+  auto read = [&](const uint4* src, uint4 (&f)[kFr]) {
+#pragma unroll
+    for (int s2 = 0; s2 < SPLIT; ++s2) {
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) f[s2 * 6 + mb] = src[(s2 * kTF + 2 * wb + mb) * 64 + lane];
+#pragma unroll
+      for (int nr = 0; nr < 4; ++nr) f[s2 * 6 + 2 + nr] = src[(s2 * kTF + kTB + 4 * wr + nr) * 64 + lane];
+    }
+  };
 
   f32x16 acc[2][4];
 #pragma unroll
@@ -149,46 +173,77 @@ __global__ void __launch_bounds__(kGemmThreads) k_gemm_bf16(const uint4* __restr
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[mb][nr][e] = 0.f;
 
-  OOPAO_STAGE_LOAD(0)
-  OOPAO_STAGE_STORE(0)
-  __syncthreads();
-  for (int st = 0; st < nst; ++st) {
-    const int buf = st & 1;
-    if (st + 1 < nst) { OOPAO_STAGE_LOAD(st + 1) }
-#pragma unroll
-    for (int kk = 0; kk < KS; ++kk) {
-      bf16x8 xh8[2], ah8[4];
-#pragma unroll
-      for (int mb = 0; mb < 2; ++mb) xh8[mb] = as_bf16x8(lds[buf][0][kk][2 * wb + mb][lane]);
-#pragma unroll
-      for (int nr = 0; nr < 4; ++nr) ah8[nr] = as_bf16x8(lds[buf][0][kk][kTB + 4 * wr + nr][lane]);
-      if constexpr (SPLIT == 2) {
-        bf16x8 xl8[2], al8[4];
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb) xl8[mb] = as_bf16x8(lds[buf][SPLIT - 1][kk][2 * wb + mb][lane]);
-#pragma unroll
-        for (int nr = 0; nr < 4; ++nr) al8[nr] = as_bf16x8(lds[buf][SPLIT - 1][kk][kTB + 4 * wr + nr][lane]);
-#pragma unroll
-        for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-          for (int nr = 0; nr < 4; ++nr) {
-            acc[mb][nr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl8[mb], ah8[nr], acc[mb][nr], 0, 0, 0);
-            acc[mb][nr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh8[mb], al8[nr], acc[mb][nr], 0, 0, 0);
-          }
-      }
+  auto mfma = [&](const uint4 (&f)[kFr]) {
+    if constexpr (SPLIT == 2) {
 #pragma unroll
       for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-        for (int nr = 0; nr < 4; ++nr)
-          acc[mb][nr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh8[mb], ah8[nr], acc[mb][nr], 0, 0, 0);
+        for (int nr = 0; nr < 4; ++nr) {
+          acc[mb][nr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f[6 + mb]), as_bf16x8(f[2 + nr]),
+                                                                acc[mb][nr], 0, 0, 0);
+          acc[mb][nr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f[mb]), as_bf16x8(f[6 + 2 + nr]),
+                                                                acc[mb][nr], 0, 0, 0);
+        }
     }
-    if (st + 1 < nst) { OOPAO_STAGE_STORE(buf ^ 1) }
-    __syncthreads();
-  }
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+      for (int nr = 0; nr < 4; ++nr)
+        acc[mb][nr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f[mb]), as_bf16x8(f[2 + nr]), acc[mb][nr],
+                                                              0, 0, 0);
+  };
 
-#undef OOPAO_STAGE_LOAD
-#undef OOPAO_STAGE_STORE
-  // D lane l, element e: row (b) 8 (e/4) + 4 (l/32) + e%4, column (r) l%32
+  // stages run: KT rounded up to whole groups of NBUF (the stored operands hold zero
+  // fragments beyond KT, and at least NBUF - 2 more for the last prefetches)
+  const int KTr = ((KT + NBUF - 1) / NBUF) * NBUF;
+#pragma unroll
+  for (int u = 0; u < NBUF - 1; ++u) issue(u, slots[u]);
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm((NBUF - 2) * kLoads));
+  asm volatile("s_barrier" ::: "memory");
+  uint4 F0[kFr], F1[kFr];
+  read(slots[0], F0);
+
+  // one fragment read after each of the first kFr MFMAs of a stage, then the rest
+  auto interleave = [&]() {
+    constexpr int kM = 8 * (SPLIT == 2 ? 3 : 1);
+#pragma unroll
+    for (int q = 0; q < kFr; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+    if constexpr (kM > kFr) __builtin_amdgcn_sched_group_barrier(0x008, kM - kFr, 0);
+  };
+  // One stage: wait for this wavefront's DMA of stage kt + 1, barrier (every wavefront's
+  // stage kt + 1 landed, stage kt - 1 fully consumed), refill the slot of stage kt - 1 with
+  // stage kt + NBUF - 1, read stage kt + 1 into the other register set, MFMAs of stage kt.
+  auto step = [&](int kt, auto U) {
+    constexpr int u = decltype(U)::value;
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm((NBUF - 3) * kLoads));
+    asm volatile("s_barrier" ::: "memory");
+    issue(kt + NBUF - 1, slots[(u + NBUF - 1) % NBUF]);
+    if constexpr (u & 1) {
+      read(slots[(u + 1) % NBUF], F0);
+      mfma(F1);
+    } else {
+      read(slots[(u + 1) % NBUF], F1);
+      mfma(F0);
+    }
+    interleave();
+  };
+  for (int kt0 = 0; kt0 < KTr; kt0 += NBUF) {
+    step(kt0 + 0, std::integral_constant<int, 0>());
+    step(kt0 + 1, std::integral_constant<int, 1>());
+    step(kt0 + 2, std::integral_constant<int, 2>());
+    step(kt0 + 3, std::integral_constant<int, 3>());
+    if constexpr (NBUF == 8) {
+      step(kt0 + 4, std::integral_constant<int, 4>());
+      step(kt0 + 5, std::integral_constant<int, 5>());
+      step(kt0 + 6, std::integral_constant<int, 6>());
+      step(kt0 + 7, std::integral_constant<int, 7>());
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // the last prefetches land before the wavefront ends
+
   const int r_base = rt * (kTR * 32) + (4 * wr) * 32 + (lane & 31);
   const int b_base = bt * (kTB * 32) + (2 * wb) * 32 + 4 * (lane >> 5);
 #pragma unroll
@@ -199,8 +254,175 @@ __global__ void __launch_bounds__(kGemmThreads) k_gemm_bf16(const uint4* __restr
       if (r >= R) continue;
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
-        const int b = b_base + mb * 32 + 8 * (e >> 2) + (e & 3);
-        if (b < B) out[(long long)b * ldo + r] = alpha * acc[mb][nr][e];
+        const int bb = b_base + mb * 32 + 8 * (e >> 2) + (e & 3);
+        if (bb < B) out[(long long)bb * ldo + r] = alpha * acc[mb][nr][e];
+      }
+    }
+}
+
+// 256-thread variant: 4 wavefronts (2 x 2), each owning a 4 x 4 block of 32 x 32 output
+// tiles (256 accumulators, one wavefront per SIMD), so each wavefront reads 8 fragments per
+// 16 MFMAs; the same 256 x 256 workgroup tile, DMA ring in NBUF distinct LDS arrays, and the
+// next stage's fragment reads interleaved with the current stage's MFMAs (two register sets).
+constexpr int kBigThreads = 256;
+template <int SPLIT, int NBUF>
+__global__ void __launch_bounds__(kBigThreads) k_gemm_bf16_big(const uint4* __restrict__ ah,
+                                                               const uint4* __restrict__ al,
+                                                               const uint4* __restrict__ xh,
+                                                               const uint4* __restrict__ xl, int IR, int IB, int KT,
+                                                               float* __restrict__ out, long long ldo, int R, int B,
+                                                               float alpha, int nbt, int nrt) {
+  constexpr int kSlot = SPLIT * kTF * 64;
+  __shared__ uint4 S0[kSlot], S1[kSlot], S2[kSlot], S3[kSlot], S4[kSlot], S5[kSlot], S6[kSlot], S7[kSlot];
+  static_assert(NBUF == 4 || NBUF == 8, "ring of 4 or 8 slots");
+  static_assert(SPLIT * NBUF * kTF * 1024 <= 131072, "ring fits in 128 KB of LDS");
+  constexpr int kLoads = 4 * SPLIT;  // DMA instructions per wavefront and stage
+  constexpr int kFr = 8 * SPLIT;     // fragments read per wavefront and stage
+  static_assert((NBUF - 2) * kLoads <= 63, "vmcnt range");
+
+  const int G = nbt * nrt;
+  const int i = blockIdx.x;
+  const int per_xcd = (G + 7) / 8;
+  const int tall = (G % 8) ? (G % 8) : 8;
+  const int xcd = i & 7, slotid = i >> 3;
+  const int t = (xcd < tall) ? xcd * per_xcd + slotid : tall * per_xcd + (xcd - tall) * (per_xcd - 1) + slotid;
+  const int rt = t / nbt, bt = t % nbt;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int wb = w & 1, wr = w >> 1;  // b fragments 4wb..4wb+3, r fragments 4wr..4wr+3
+
+  // DMA: wavefront w copies X fragments 2w, 2w+1 and A fragments 2w, 2w+1 of each stage
+  const uint4* gx[2][2];
+  const uint4* ga[2][2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int itx = min(bt * kTB + 2 * w + j, IB - 1), ita = min(rt * kTR + 2 * w + j, IR - 1);
+    gx[0][j] = xh + (size_t)itx * 64 + lane;
+    ga[0][j] = ah + (size_t)ita * 64 + lane;
+    gx[1][j] = (SPLIT == 2) ? xl + (size_t)itx * 64 + lane : nullptr;
+    ga[1][j] = (SPLIT == 2) ? al + (size_t)ita * 64 + lane : nullptr;
+  }
+  const size_t sx = (size_t)IB * 64, sa = (size_t)IR * 64;
+  uint4* const slots[8] = {S0, S1, S2, S3, S4, S5, S6, S7};
+
+  auto issue = [&](int kt, uint4* dst) {
+#pragma unroll
+    for (int s2 = 0; s2 < SPLIT; ++s2)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        __builtin_amdgcn_global_load_lds(gx[s2][j] + kt * sx,
+                                         (__attribute__((address_space(3))) void*)(dst + (s2 * kTF + 2 * w + j) * 64),
+                                         16, 0, 0);
+        __builtin_amdgcn_global_load_lds(
+            ga[s2][j] + kt * sa, (__attribute__((address_space(3))) void*)(dst + (s2 * kTF + kTB + 2 * w + j) * 64), 16,
+            0, 0);
+      }
+  };
+  // F[s2 * 8 + m] = X fragment 4wb + m (m < 4), A fragment 4wr + m - 4 (m >= 4)
+  auto read = [&](const uint4* src, uint4 (&f)[kFr]) {
+#pragma unroll
+    for (int s2 = 0; s2 < SPLIT; ++s2)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        f[s2 * 8 + m] = src[(s2 * kTF + 4 * wb + m) * 64 + lane];
+        f[s2 * 8 + 4 + m] = src[(s2 * kTF + kTB + 4 * wr + m) * 64 + lane];
+      }
+  };
+
+  f32x16 acc[4][4];
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+    for (int nr = 0; nr < 4; ++nr)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[mb][nr][e] = 0.f;
+
+  auto mfma = [&](const uint4 (&f)[kFr]) {
+    if constexpr (SPLIT == 2) {
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nr = 0; nr < 4; ++nr) {
+          acc[mb][nr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f[8 + mb]), as_bf16x8(f[4 + nr]),
+                                                                acc[mb][nr], 0, 0, 0);
+          acc[mb][nr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f[mb]), as_bf16x8(f[8 + 4 + nr]),
+                                                                acc[mb][nr], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int nr = 0; nr < 4; ++nr)
+        acc[mb][nr] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f[mb]), as_bf16x8(f[4 + nr]), acc[mb][nr], 0,
+                                                              0, 0);
+  };
+  // schedule: one fragment read after each of the first kFr MFMAs of the stage, then the
+  // rest of the MFMAs, so the next stage's fragments have landed before its barrier
+  auto interleave = [&]() {
+    constexpr int kM = 16 * (SPLIT == 2 ? 3 : 1);
+#pragma unroll
+    for (int q = 0; q < kFr; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, kM - kFr, 0);
+  };
+
+  // stages run: KT rounded up to whole groups of NBUF (the stored operands hold zero
+  // fragments beyond KT, and at least NBUF - 2 more for the last prefetches)
+  const int KTr = ((KT + NBUF - 1) / NBUF) * NBUF;
+#pragma unroll
+  for (int u = 0; u < NBUF - 1; ++u) issue(u, slots[u]);
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm((NBUF - 2) * kLoads));
+  asm volatile("s_barrier" ::: "memory");
+  uint4 F0[kFr], F1[kFr];
+  read(slots[0], F0);
+
+  // One stage: wait for this wavefront's DMA of stage kt + 1 (NBUF - 3 later stages may
+  // still be in flight), barrier (every wavefront's stage kt + 1 landed, stage kt - 1 fully
+  // consumed), refill the slot of stage kt - 1 with stage kt + NBUF - 1, read stage kt + 1
+  // into the other register set, MFMAs of stage kt.
+  auto step = [&](int kt, auto U) {
+    constexpr int u = decltype(U)::value;
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm((NBUF - 3) * kLoads));
+    asm volatile("s_barrier" ::: "memory");
+    issue(kt + NBUF - 1, slots[(u + NBUF - 1) % NBUF]);
+    if constexpr (u & 1) {
+      read(slots[(u + 1) % NBUF], F0);
+      mfma(F1);
+    } else {
+      read(slots[(u + 1) % NBUF], F1);
+      mfma(F0);
+    }
+    interleave();
+  };
+  for (int kt0 = 0; kt0 < KTr; kt0 += NBUF) {
+    step(kt0 + 0, std::integral_constant<int, 0>());
+    step(kt0 + 1, std::integral_constant<int, 1>());
+    step(kt0 + 2, std::integral_constant<int, 2>());
+    step(kt0 + 3, std::integral_constant<int, 3>());
+    if constexpr (NBUF == 8) {
+      step(kt0 + 4, std::integral_constant<int, 4>());
+      step(kt0 + 5, std::integral_constant<int, 5>());
+      step(kt0 + 6, std::integral_constant<int, 6>());
+      step(kt0 + 7, std::integral_constant<int, 7>());
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // the last prefetches land before the wavefront ends
+
+  const int r_base = rt * (kTR * 32) + (4 * wr) * 32 + (lane & 31);
+  const int b_base = bt * (kTB * 32) + (4 * wb) * 32 + 4 * (lane >> 5);
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+    for (int nr = 0; nr < 4; ++nr) {
+      const int r = r_base + nr * 32;
+      if (r >= R) continue;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int bb = b_base + mb * 32 + 8 * (e >> 2) + (e & 3);
+        if (bb < B) out[(long long)bb * ldo + r] = alpha * acc[mb][nr][e];
       }
     }
 }
@@ -266,6 +488,11 @@ int frag_counts(int I, int K, int* IT, int* KT) {
   return OK;
 }
 
+// k fragments stored per operand: the KT that hold data, rounded up to a multiple of 8, plus
+// 8 zero fragments, so a GEMM ring of up to 8 stages can run whole groups of stages and
+// prefetch past the end without a bounds check
+int stored_kt(int KT) { return ((KT + 7) / 8) * 8 + 8; }
+
 int gemv_chunks(int KT) { return (KT + kKC - 1) / kKC; }
 
 }  // namespace
@@ -276,7 +503,7 @@ int oopao_bf16_packed_bytes(int I, int K, size_t* bytes) {
   OOPAO_CHECK_ARG(bytes, "null pointer argument");
   int IT, KT;
   OOPAO_CHECK_ARG(frag_counts(I, K, &IT, &KT) == OK, "invalid sizes I=%d K=%d", I, K);
-  *bytes = (size_t)IT * KT * 1024;
+  *bytes = (size_t)IT * stored_kt(KT) * 1024;
   return OK;
 }
 
@@ -286,9 +513,10 @@ int oopao_bf16_pack(const float* src, long long si, long long sk, int I, int K, 
   OOPAO_CHECK_ARG(frag_counts(I, K, &IT, &KT) == OK, "invalid sizes I=%d K=%d", I, K);
   OOPAO_CHECK_ARG((reinterpret_cast<size_t>(hi) & 15) == 0 && (reinterpret_cast<size_t>(lo) & 15) == 0,
                   "packed buffers must be 16-byte aligned");
-  const size_t total = (size_t)IT * KT * 64;
+  const int KTs = stored_kt(KT);
+  const size_t total = (size_t)IT * KTs * 64;
   hipLaunchKernelGGL(k_pack_bf16, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                     reinterpret_cast<hipStream_t>(stream), src, si, sk, I, K, IT, KT,
+                     reinterpret_cast<hipStream_t>(stream), src, si, sk, I, K, IT, KTs,
                      reinterpret_cast<uint4*>(hi), reinterpret_cast<uint4*>(lo));
   OOPAO_LAUNCH_CHECK();
   return OK;
@@ -312,11 +540,11 @@ int oopao_gemm_bf16(const void* a_hi, const void* a_lo, int R, int K, const void
   const uint4* xh = reinterpret_cast<const uint4*>(x_hi);
   const uint4* xl = reinterpret_cast<const uint4*>(x_lo);
   if (terms == 1)
-    hipLaunchKernelGGL((k_gemm_bf16<1, 4>), grid, dim3(kGemmThreads), 0, st, ah, al, xh, xl, IR, IB, KT, out, ldo,
-                       R, B, alpha, nbt, nrt);
+    hipLaunchKernelGGL((k_gemm_bf16_pipe<1, 8>), grid, dim3(kGemmThreads), 0, st, ah, al, xh, xl, IR, IB, KT, out,
+                       ldo, R, B, alpha, nbt, nrt);
   else
-    hipLaunchKernelGGL((k_gemm_bf16<2, 2>), grid, dim3(kGemmThreads), 0, st, ah, al, xh, xl, IR, IB, KT, out, ldo,
-                       R, B, alpha, nbt, nrt);
+    hipLaunchKernelGGL((k_gemm_bf16_big<2, 4>), grid, dim3(kBigThreads), 0, st, ah, al, xh, xl, IR, IB, KT, out,
+                       ldo, R, B, alpha, nbt, nrt);
   OOPAO_LAUNCH_CHECK();
   return OK;
 }
