@@ -123,6 +123,60 @@ def im_leg(dev, n_modes=1000):
             'nSignal': int(wfs.nSignal), 'cond': float(calib.cond)}
 
 
+def im_batched_leg(n_modes=1000):
+    """Push-pull calibration of four PWFS systems of uneven sampling in one HIP graph: the config-1
+    sensor (80 px, 20x20 subapertures, 48 points) and 40 / 64 / 80 subapertures at 4 px each with
+    48 / 72 / 96 points, each with its own Cartesian DM (nSubap + 1 actuators across) and KL basis
+    (n_modes modes, or nAct - 1 when fewer), stroke 10 nm; tile and batch sizes from the autotuner;
+    float64 SVD pseudo-inverses on the GPU timed separately."""
+    import torch
+    from types import SimpleNamespace
+    from oopao_amd import Telescope, Source, DeformableMirror, Pyramid
+    from oopao_amd.calibration.compute_KL_modal_basis import compute_KL_basis
+    from oopao_amd.calibration.batched import BatchedCalibration
+    specs = [(80, 20, 48), (160, 40, 48), (256, 64, 72), (320, 80, 96)]
+    systems = []
+    for n, ns, T in specs:
+        tel = Telescope(n, 8.0, 1e-3)
+        ngs = Source('I3', 8, display_properties=False)
+        ngs * tel
+        dm = DeformableMirror(tel, ns, 0.35, print_dm_properties=False)
+        wfs = Pyramid(ns, tel, 3, 0.5, n_pix_separation=ns, n_pix_edge=ns // 2, nTheta_user_defined=T)
+        kl = compute_KL_basis(tel, SimpleNamespace(r0=0.15, L0=25.0), dm)
+        M2C = kl[:, :min(n_modes, kl.shape[1])]
+        systems.append((ngs, tel, dm, wfs, M2C))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bc = BatchedCalibration(systems, 10e-9, tune=True)
+    torch.cuda.synchronize()
+    t_tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bc.run_eager()
+    torch.cuda.synchronize()
+    t_eager = time.perf_counter() - t0
+    bc.capture()
+    bc.run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    bc.run()
+    e1.record()
+    torch.cuda.synchronize()
+    t_graph = e0.elapsed_time(e1) / 1e3
+    t0 = time.perf_counter()
+    vaults = bc.vaults()
+    torch.cuda.synchronize()
+    t_svd = time.perf_counter() - t0
+    modes = sum(cs.nModes for cs in bc.systems)
+    per = [{'n': cs.tel.resolution, 'nSubap': cs.wfs.nSubap, 'nTheta': cs.wfs.nTheta, 'nAct': cs.dm.nValidAct,
+            'modes': cs.nModes, 'nSignal': int(cs.wfs.nSignal), 'batch_modes': cs.batch,
+            'autotune': {k: v for k, v in t.items() if k != 'trials'}, 'cond': float(cv.cond)}
+           for cs, t, cv in zip(bc.systems, bc.tuning, vaults)]
+    return {'value': round(modes / t_graph, 1), 'unit': 'modes/s (push-pull, 4 sensors, one graph launch)',
+            'modes': modes, 'graph_s': round(t_graph, 4), 'eager_s': round(t_eager, 4),
+            'autotune_s': round(t_tune, 2), 'svd_pinv_f64_s': round(t_svd, 3), 'systems': per}
+
+
 def adjoint_leg(n_screens=64, steps=10):
     """Differentiable PWFS (config 1 sensor: 80 px pupil, 20x20 subapertures, 48 points): gradients of
     ||s(phi) - s_target||^2 w.r.t. 64 seeded phase screens per step through the hand-written adjoint
@@ -297,7 +351,7 @@ def main():
     ap.add_argument('--batch', type=int, default=8, help='screens per call for the batched form')
     ap.add_argument('--no-extra', action='store_true',
                     help='skip the closed-loop, interaction-matrix, KL-basis, adjoint and contraction legs')
-    ap.add_argument('--legs', default='closed_loop,interaction_matrix,kl_basis,adjoint,contractions',
+    ap.add_argument('--legs', default='closed_loop,interaction_matrix,im_batched,kl_basis,adjoint,contractions',
                     help='comma-separated extra legs to run')
     ap.add_argument('--calib-copy', action='store_true',
                     help='also run one elementwise copy of 256 MiB (counter calibration for rocprof --pmc)')
@@ -400,6 +454,8 @@ def main():
             extra['closed_loop'] = closed_loop_leg(tel, ngs, wfs, dev)
         if 'interaction_matrix' in legs:
             extra['interaction_matrix'] = im_leg(dev)
+        if 'im_batched' in legs:
+            extra['im_batched'] = im_batched_leg()
         if 'kl_basis' in legs:
             extra['kl_basis'] = kl_leg(tel)
         if 'adjoint' in legs:

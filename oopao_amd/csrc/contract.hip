@@ -438,10 +438,9 @@ __global__ void __launch_bounds__(256) k_gemv_bf16(const uint4* __restrict__ ah,
   const int kt0 = kc * kKC, kt1 = min(KT, kt0 + kKC);
   float a0 = 0.f, a1 = 0.f;
   const int koff = (lane >> 5) * 8;
-  for (int kt = kt0; kt < kt1; ++kt) {
-    const uint4 v = ah[((size_t)kt * IR + it) * 64 + lane];
+  // x[k .. k+8) for fragment kt (zero beyond K)
+  auto xload = [&](int kt, float (&xv)[8]) {
     const int k = kt * 16 + koff;
-    float xv[8];
     if (k + 8 <= K) {
       const float4 p = *reinterpret_cast<const float4*>(x + k);
       const float4 q = *reinterpret_cast<const float4*>(x + k + 4);
@@ -451,12 +450,34 @@ __global__ void __launch_bounds__(256) k_gemv_bf16(const uint4* __restrict__ ah,
 #pragma unroll
       for (int j = 0; j < 8; ++j) xv[j] = (k + j < K) ? x[k + j] : 0.f;
     }
+  };
+  auto fma8 = [&](const uint4 v, const float (&xv)[8]) {
     const unsigned u[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       a0 = fmaf(__uint_as_float(u[j] << 16), xv[2 * j], a0);
       a1 = fmaf(__uint_as_float(u[j] & 0xFFFF0000u), xv[2 * j + 1], a1);
     }
+  };
+  const uint4* base = ah + (size_t)it * 64 + lane;
+  const size_t stride = (size_t)IR * 64;
+  int kt = kt0;
+  // four 1 KB fragment loads in flight per wavefront
+  for (; kt + 4 <= kt1; kt += 4) {
+    uint4 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = base[(size_t)(kt + q) * stride];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float xv[8];
+      xload(kt + q, xv);
+      fma8(v[q], xv);
+    }
+  }
+  for (; kt < kt1; ++kt) {
+    float xv[8];
+    xload(kt, xv);
+    fma8(base[(size_t)kt * stride], xv);
   }
   float s = a0 + a1;
   s += __shfl_xor(s, 32);
