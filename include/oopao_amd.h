@@ -205,6 +205,41 @@ int oopao_dm_separable_opd(const float* coefs, const int* grid, const float* gxT
                            float* opd, int n, int nA, int B, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * 2-D FFT and gain-sensing camera (GainSensingCamera.py:78-199)
+ * ------------------------------------------------------------------------- */
+
+/* 1 if a batched N x N transform has a plan (N <= 2048). */
+int oopao_fft2_supported(int N);
+int oopao_fft2_bytes(int N, int B, size_t* bytes);
+
+/* out = scale * S(FFT2_dir(S(in))) for B complex64 N x N arrays; dir = -1 forward
+ * (exp(-2 pi i jk/N)), +1 inverse (unnormalised); S = fftshift when centred (even N),
+ * identity otherwise: the reference's fftshift(fft2(fftshift(x))) / N convention
+ * (GainSensingCamera.py:147-152) with scale = 1/N. Workspace oopao_fft2_bytes. */
+int oopao_fft2(const void* in, void* out, int N, int B, int dir, int centred, float scale, void* ws,
+               size_t ws_bytes, void* stream);
+
+/* Workspace of the three GSC calls for an N x N mask, M modes, mb modes per basis batch. */
+int oopao_gsc_workspace_bytes(int N, int M, int mb, size_t* bytes);
+
+/* ir = 2 Im(conj(fmask) F(mask frame / sum(frame))) (GainSensingCamera.py:179-180), with
+ * fmask = F(mask) = fftshift(fft2(fftshift(mask))) / N precomputed; complex64 N x N output
+ * (zero imaginary part), frame float32 N x N. */
+int oopao_gsc_impulse_response(const void* fmask, const void* mask, const float* frame, int N, void* ir, void* ws,
+                               size_t ws_bytes, void* stream);
+
+/* bp[f][m0 + j] = |F(pad_N(basis[m0 + j]))|^2 for j < mb: the reference's basis product
+ * fft3(B) * ifft3(B) (GainSensingCamera.py:155-171), real for a real basis; basis float32
+ * [M][n][n] (mode-major maps), bp float32 [N*N][M]. */
+int oopao_gsc_basis_product(const float* basis, int n, int N, int M, int m0, int mb, float* bp, void* ws,
+                            size_t ws_bytes, void* stream);
+
+/* S[m] = sum_f (F(ir1) v2)[f] bp[f][m] (complex, interleaved doubles [M][2]) with
+ * v2 = F'(ir2) = fftshift(ifft2(fftshift(ir2))) * N precomputed (GainSensingCamera.py:183-188). */
+int oopao_gsc_sensitivity(const void* ir1, const void* v2, const float* bp, int N, int M, double* S, void* ws,
+                          size_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Atmosphere frozen-flow update (replaces Atmosphere.add_row / updateLayer and
  * the footprint sum of fill_OPD_support: Atmosphere.py:410-421, 447-516,
  * 589-622; the warp of tools/tools.py:202-209)

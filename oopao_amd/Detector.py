@@ -122,6 +122,18 @@ class Detector:
     def print_properties(self):
         print(self)
 
+    def __mul__(self, obj):
+        """Detector * GainSensingCamera (Detector.py:385-393): the first product calibrates the GSC
+        with this frame, the next ones compute the optical gains."""
+        if getattr(obj, 'tag', None) == 'GSC':
+            if obj.calibration_ready is False:
+                obj.calibration(self._frame)
+                obj.detector_properties = {'resolution': f"{'Resolution [px]':<25s}|{int(self.resolution):^9d}"}
+            else:
+                obj.compute_optical_gains(self._frame)
+        else:
+            raise OopaoError(f'Coupled object should be a "GSC" but is {getattr(obj, "tag", type(obj).__name__)}')
+
     def __repr__(self):
         return f'Detector(resolution={self.resolution}, QE={self.QE}, gain={self.gain}, ' \
                f'photonNoise={self.photonNoise}, readoutNoise={self.readoutNoise})'

@@ -71,6 +71,16 @@ SIGNATURES = {
     'oopao_gemv_bf16_workspace_bytes': (c_int, [c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_gemv_bf16': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_float, c_float, c_void_p,
                                 c_size_t, c_void_p]),
+    'oopao_fft2_supported': (c_int, [c_int]),
+    'oopao_fft2_bytes': (c_int, [c_int, c_int, ctypes.POINTER(c_size_t)]),
+    'oopao_fft2': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_float, c_void_p, c_size_t, c_void_p]),
+    'oopao_gsc_workspace_bytes': (c_int, [c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
+    'oopao_gsc_impulse_response': (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_size_t,
+                                           c_void_p]),
+    'oopao_gsc_basis_product': (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_size_t,
+                                        c_void_p]),
+    'oopao_gsc_sensitivity': (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_size_t,
+                                      c_void_p]),
     'oopao_dm_separable_opd': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
                                        c_int, c_void_p]),
 }

@@ -203,6 +203,9 @@ static bool gen_plan(int N, GenPlan* p) {
   return true;
 }
 
+// the same plan for other translation units (the 2-D FFT entry point)
+bool gen_plan_public(int N, GenPlan* p) { return gen_plan(N, p); }
+
 static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 static int gen_nchunk_check(int T, int nchunk) { return nchunk > 0 && nchunk <= T; }
