@@ -202,8 +202,29 @@ def adjoint_leg(n_screens=64, steps=10):
         ((pwfs_signal(wfs, ph).double() - tgt) ** 2).sum().backward()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    # Jacobians through the adjoint at an aberrated working point: the full slopes Jacobian for the
+    # 357 DM influence functions (one VJP per slope), and optical gains of those modes (one VJP per mode)
+    from oopao_amd import DeformableMirror
+    from oopao_amd.differentiable import slopes_jacobian, optical_gains_adjoint
+    dm = DeformableMirror(tel, 20, 0.35, print_dm_properties=False)
+    pupil = torch.from_numpy(tel.pupil.astype(np.float32)).cuda()
+    modes = dm._build_modes().t().reshape(-1, 80, 80) * pupil
+    zero = torch.zeros((80, 80), device='cuda')
+    slopes_jacobian(wfs, zero, modes[:8], batch=64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    D0 = slopes_jacobian(wfs, zero, modes, batch=64)
+    torch.cuda.synchronize()
+    t_jac = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    og = optical_gains_adjoint(wfs, ph[0].detach(), modes, D0, batch=64)
+    torch.cuda.synchronize()
+    t_og = time.perf_counter() - t0
     return {'value': round(n_screens / dt, 1), 'unit': 'gradients/s', 'screens_per_step': n_screens,
-            'ms_per_step': round(dt * 1e3, 3), 'note': 'forward + adjoint, float32, wall clock'}
+            'ms_per_step': round(dt * 1e3, 3), 'note': 'forward + adjoint, float32, wall clock',
+            'jacobian': {'nSignal': int(wfs.nSignal), 'modes': int(modes.shape[0]), 'seconds': round(t_jac, 4)},
+            'optical_gains': {'modes': int(modes.shape[0]), 'seconds': round(t_og, 4),
+                              'mean_og': round(float(og.mean()), 4)}}
 
 
 def kl_leg(tel):

@@ -66,3 +66,53 @@ def pwfs_signal(wfs, phase):
     valid = torch.from_numpy(np.asarray(wfs.validI4Q, bool)).to(frames.device)
     ref = wfs._referenceSignal_2D if wfs._referenceSignal_2D is not None else 0.0
     return slopes_maps(frames, valid, quad, w, ref, float(wfs.slopesUnits))[1]
+
+
+def _vjp_batch(wfs, phase0, G):
+    """J(phase0)^T g_k for the slope cotangents G [K, nSignal]: one forward + one adjoint pass over K
+    copies of the working point; returns [K, n, n] (radians^-1 per slope unit)."""
+    n = wfs.engine.n
+    ph = phase0.reshape(1, n, n).to(torch.float32).expand(G.shape[0], n, n).contiguous().requires_grad_(True)
+    s = pwfs_signal(wfs, ph)
+    (s * G.to(s.dtype)).sum().backward()
+    return ph.grad
+
+
+def slopes_jacobian(wfs, phase0, basis, batch=64):
+    """Jacobian J [nSignal, nModes] = d signal / d a of the slopes w.r.t. the coefficients a of
+    ``basis`` [nModes, n, n] (phase in radians per unit coefficient) at the working point
+    ``phase0`` [n, n] (radians): the interaction matrix of the linearised sensor at phase0, from
+    nSignal vector-Jacobian products (unit slope cotangents) through the adjoint, batched by
+    ``batch``. At phase0 = 0 it is the push-pull interaction matrix in the small-stroke limit."""
+    dev = wfs.device
+    nsig = int(wfs.nSignal)
+    B = basis.reshape(basis.shape[0], -1).to(dev, torch.float32)
+    J = torch.empty((nsig, B.shape[0]), dtype=torch.float32, device=dev)
+    for i0 in range(0, nsig, batch):
+        i1 = min(nsig, i0 + batch)
+        G = torch.zeros((i1 - i0, nsig), dtype=torch.float32, device=dev)
+        G[torch.arange(i1 - i0, device=dev), torch.arange(i0, i1, device=dev)] = 1.0
+        g = _vjp_batch(wfs, phase0, G).reshape(i1 - i0, -1)
+        J[i0:i1] = g @ B.t()
+    return J
+
+
+def optical_gains_adjoint(wfs, phase0, basis, D_calib, batch=64):
+    """Modal optical gains at the working point phase0 [n, n] (radians) with respect to the
+    calibration interaction matrix D_calib [nSignal, nModes] (slopes per unit coefficient of
+    ``basis`` [nModes, n, n], phase in radians):
+        og_m = <D_calib[:, m], J(phase0) b_m> / <D_calib[:, m], D_calib[:, m]>,
+    the projection of each mode's sensitivity at phase0 on its calibrated response, with
+    J(phase0)^T D_calib[:, m] from one adjoint pass per mode (batched)."""
+    dev = wfs.device
+    Bm = basis.reshape(basis.shape[0], -1).to(dev, torch.float32)
+    D = D_calib.to(dev, torch.float32)
+    M = Bm.shape[0]
+    og = torch.empty(M, dtype=torch.float64, device=dev)
+    for m0 in range(0, M, batch):
+        m1 = min(M, m0 + batch)
+        G = D[:, m0:m1].t().contiguous()
+        g = _vjp_batch(wfs, phase0, G).reshape(m1 - m0, -1)
+        num = (g.double() * Bm[m0:m1].double()).sum(dim=1)
+        og[m0:m1] = num / (G.double() ** 2).sum(dim=1)
+    return og
