@@ -263,6 +263,15 @@ int oopao_atm_minmax(const double* map, int count, double* mm, void* stream);
 int oopao_atm_layer_opd(const double* map, int M, const double* mm, double tx, double ty, int r0, int c0, double w,
                         float* out, int n, int accumulate, void* stream);
 
+/* out[r][c] (+)= weight * bilinear(in; y = a r + by, x = a c + bx) for an H x W float32 image
+ * and an h x w output; 0 outside the image (scikit-image warp order 1, mode constant, as the
+ * reference's interpolate_image / interpolate_cube,
+ * tools/interpolateGeometricalTransformation.py:152-265): the sub-pixel translation of an
+ * off-axis footprint (a = 1) and the laser-guide-star cone magnification
+ * (Atmosphere.fill_OPD_support, Atmosphere.py:589-622). */
+int oopao_bilinear_affine(const float* in, int H, int W, float* out, int h, int w, double a, double by, double bx,
+                          double weight, int accumulate, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Closed loop (tutorials/AO_closed_loop_Pyramid_WFS.py:601-638): residual OPD
  * opd = (turb + dm) * pupil (dm may be null) and the RMS wavefront errors over

@@ -9,8 +9,15 @@ sum of the layers. The extrusion noise xi is drawn from each layer's host
 RandomState in the reference's order, so a seeded run reproduces the
 reference's screens.
 
-Supported: on-axis sources at infinity, fov = 0, geometric propagation, no
-boiling. Other modes raise OopaoError.
+Sources: on-axis and off-axis natural guide stars and laser guide stars at a
+finite altitude, single or in an Asterism, with a field of view (layers sized
+for it, Atmosphere.py:297-300). Each source's OPD follows fill_OPD_support
+(Atmosphere.py:589-622): the footprint of its line of sight on each layer
+(set_pupil_footprint, 423-445), the sub-pixel remainder of the footprint
+position applied as a bilinear translation of the layer, and for a laser guide
+star the cone-effect magnification (bilinear, about the footprint centre),
+on the GPU (oopao_bilinear_affine). Geometric propagation only (angular-spectrum
+scintillation and boiling raise OopaoError).
 """
 import ctypes
 
@@ -82,11 +89,6 @@ class Atmosphere:
         self.compute_covariance = compute_covariance
         self.fov = self.telescope.fov
         self.fov_rad = self.telescope.fov_rad
-        if self.fov_rad != 0:
-            raise OopaoError('a field of view (off-axis sources) is not supported by oopao_amd')
-        for s in self.src_list:
-            if s.coordinates[0] != 0 or np.isfinite(s.altitude):
-                raise OopaoError('only on-axis sources at infinity are supported by oopao_amd')
         if self.hasNotBeenInitialized:
             self.initial_r0 = self.r0
             for i in range(self.nLayer):
@@ -103,7 +105,7 @@ class Atmosphere:
         print(self)
 
     def buildLayer(self, telescope, r0, L0, i_layer):
-        """Atmosphere.buildLayer (Atmosphere.py:281-374), on-axis, fov = 0."""
+        """Atmosphere.buildLayer (Atmosphere.py:281-374)."""
         tel = self.telescope
         lay = _Layer()
         lay.randomState = RandomState(42 + i_layer * 1000)
@@ -132,17 +134,20 @@ class Atmosphere:
         u, v = np.meshgrid(x, x)
         lay.innerZ = u[inner != 0] + 1j * v[inner != 0]
         lay.outerZ = u[outer != 0] + 1j * v[outer != 0]
-        # covariance matrices, shared by all layers of the same geometry (Atmosphere.py:707-777)
-        if not hasattr(self, 'ZZt') or self.ZZt.shape[0] != lay.innerZ.size:
+        # covariance matrices (Atmosphere.py:707-777); the ring geometry depends only on the layer
+        # resolution, so layers of equal size share them
+        cache = self.__dict__.setdefault('_cov_cache', {})
+        key = (res, self.L0)
+        if key not in cache:
             print('Computing the covariance matrices ...')
-            self.ZZt = makeCovarianceMatrix(lay.innerZ, lay.innerZ, self.L0, self.r0_def)
-            self.ZZt_inv = np.linalg.pinv(self.ZZt)
-            self.ZXt = makeCovarianceMatrix(lay.innerZ, lay.outerZ, self.L0, self.r0_def)
-            self.XXt = makeCovarianceMatrix(lay.outerZ, lay.outerZ, self.L0, self.r0_def)
+            ZZt = makeCovarianceMatrix(lay.innerZ, lay.innerZ, self.L0, self.r0_def)
+            cache[key] = (ZZt, makeCovarianceMatrix(lay.innerZ, lay.outerZ, self.L0, self.r0_def),
+                          makeCovarianceMatrix(lay.outerZ, lay.outerZ, self.L0, self.r0_def), np.linalg.pinv(ZZt))
+        self.ZZt, self.ZXt, self.XXt, self.ZZt_inv = cache[key]
         lay.ZZt, lay.ZXt, lay.XXt, lay.ZZt_inv = self.ZZt, self.ZXt, self.XXt, self.ZZt_inv
         c = (self.r0_def / self.r0) ** (5 / 3)
-        lay.A = np.matmul((self.ZXt * c).T, self.ZZt_inv / c)
-        lay.B = np.linalg.cholesky(self.XXt * c - np.matmul(lay.A, self.ZXt * c))
+        lay.A = np.matmul((lay.ZXt * c).T, lay.ZZt_inv / c)
+        lay.B = np.linalg.cholesky(lay.XXt * c - np.matmul(lay.A, lay.ZXt * c))
         lay.notDoneOnce = True
         lay.buff = np.zeros(2)
         lay.ratio = np.zeros(2)
@@ -304,6 +309,9 @@ class Atmosphere:
         else:
             opd = self._OPD_dev
         n = self.telescope.resolution
+        user = opd is not self._OPD_dev
+        self._layer_full = {}
+        self._src_opd = []
         for s in self.src_list:
             if s.coordinates[0] > getattr(self, 'fov', 0) / 2:
                 raise OopaoError(f'Source zenith ({s.coordinates[0]}") outside of fov!')
@@ -313,16 +321,90 @@ class Atmosphere:
             m = s._mask_dev()
             s._scintillation = s._scintillation_no_pupil if m is None else s._scintillation_no_pupil * m
             s._refresh_intensity()
-            s._set_OPD_no_pupil_dev(opd.clone())
+            o = opd.clone() if (user or self._on_axis(s)) else self._source_opd(s)
+            self._src_opd.append(o)
+            s._set_OPD_no_pupil_dev(o)
+
+    # ------------------------------------------------------- off-axis / LGS
+    @staticmethod
+    def _on_axis(s):
+        return s.coordinates[0] == 0 and not np.isfinite(s.altitude) and s.chromatic_shift is None
+
+    def _full_layer(self, i):
+        """layer.OPD of the reference (the shifted, clipped screen, [res, res] radians at 500 nm), once per relay."""
+        f = self._layer_full.get(i)
+        if f is None:
+            lay = getattr(self, 'layer_' + str(i + 1))
+            f = torch.empty((lay.resolution, lay.resolution), dtype=torch.float32, device=self.device)
+            tx, ty = getattr(lay, 'shift', (0.0, 0.0))
+            _lib.call('oopao_atm_layer_opd', ptr(lay.map), lay.M, ptr(lay.mm), ctypes.c_double(tx),
+                      ctypes.c_double(ty), 1, 1, ctypes.c_double(1.0), ptr(f), lay.resolution, 0,
+                      stream_ptr(self.device))
+            self._layer_full[i] = f
+        return f
+
+    def footprint(self, lay, s, i_layer):
+        """(row origin, column origin, sub-pixel shift x, y) of a source's footprint on a layer
+        (set_pupil_footprint, Atmosphere.py:423-445)."""
+        if s.chromatic_shift is not None:
+            if len(s.chromatic_shift) != self.nLayer:
+                raise OopaoError('The chromatic_shift property is expected to be the same length as the number of '
+                                 'atmospheric layer. ')
+            ch = s.chromatic_shift[i_layer]
+        else:
+            ch = 0
+        rho = lay.altitude * np.tan((s.coordinates[0] + ch) / self.rad2arcsec) * lay.resolution / lay.D
+        phi = np.deg2rad(s.coordinates[1])
+        x_z, y_z = rho * np.cos(phi), rho * np.sin(phi)
+        n = self.telescope.resolution
+        cx = int(y_z) + lay.resolution // 2
+        cy = int(x_z) + lay.resolution // 2
+        return cx - n // 2, cy - n // 2, int(x_z) - x_z, int(y_z) - y_z
+
+    def _source_opd(self, s):
+        """Sum over layers of the source's footprint (fill_OPD_support, Atmosphere.py:589-622), metres."""
+        n = self.telescope.resolution
+        dev = self.device
+        st = stream_ptr(dev)
+        out = torch.zeros((n, n), dtype=torch.float32, device=dev)
+        sub = torch.empty((n, n), dtype=torch.float32, device=dev)
+        for i in range(self.nLayer):
+            lay = getattr(self, 'layer_' + str(i + 1))
+            if s.altitude <= lay.altitude:
+                raise OopaoError('The source altitude (' + str(s.altitude) + ' m) is below or at the same altitude as '
+                                 'the atmosphere layer (' + str(lay.altitude) + ' m)')
+            full = self._full_layer(i)
+            r0, c0, esx, esy = self.footprint(lay, s, i)
+            w = self.wavelength / 2 / np.pi * np.sqrt(self.fractionalR0[i])
+            res = lay.resolution
+            if np.isfinite(s.altitude):
+                # footprint (translated by the sub-pixel remainder), then the cone magnification
+                # m = (H - h) / H about the footprint centre, with the reference's half-pixel alignment
+                _lib.call('oopao_bilinear_affine', ptr(full), res, res, ptr(sub), n, n, ctypes.c_double(1.0),
+                          ctypes.c_double(r0 - esy), ctypes.c_double(c0 - esx), ctypes.c_double(1.0), 0, st)
+                m = (s.altitude - lay.altitude) / s.altitude
+                ratio = 1.0 / m
+                e = (ratio % 1) / 2 + (np.floor(ratio) - 1) * 0.5
+                b = -(n / 2 + e) / ratio + n / 2
+                _lib.call('oopao_bilinear_affine', ptr(sub), n, n, ptr(out), n, n, ctypes.c_double(1.0 / ratio),
+                          ctypes.c_double(b), ctypes.c_double(b), ctypes.c_double(w), 1, st)
+            else:
+                _lib.call('oopao_bilinear_affine', ptr(full), res, res, ptr(out), n, n, ctypes.c_double(1.0),
+                          ctypes.c_double(r0 - esy), ctypes.c_double(c0 - esx), ctypes.c_double(w), 1, st)
+        return out
 
     # -------------------------------------------------------------- properties
     @property
     def OPD(self):
+        """On-axis OPD, or the per-source OPDs [nSrc, n, n] after relaying an asterism (Atmosphere.py:624-629)."""
+        lst = getattr(self, '_src_opd', None)
+        if lst and (len(lst) > 1 or not self._on_axis(self.src_list[0])):
+            return np.squeeze(np.array([to_host(o) for o in lst]))
         return to_host(self._OPD_dev)
 
     @property
     def OPD_no_pupil(self):
-        return to_host(self._OPD_dev)
+        return self.OPD
 
     def layer_opd(self, i_layer):
         """Full shifted screen of one layer in radians at 500 nm ([res, res], layer.OPD of the reference)."""

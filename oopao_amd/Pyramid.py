@@ -481,6 +481,7 @@ class Pyramid:
                                  're-calibrate with the correct source.')
         if phase_in is not None:
             self.src.phase = phase_in
+        self._stacked_sources = False
         opd = self.src._OPD
         batched = opd.dim() == 3
         weighted = (not batched) and self._weight_vector is not None
@@ -557,6 +558,8 @@ class Pyramid:
         return to_host(s2d), self._sig_host(s)
 
     def _sig_host(self, s):
+        if getattr(self, '_stacked_sources', False):       # asterism: [nSrc, nSig] (Pyramid.py:1171-1173)
+            return s.detach().double().cpu().numpy()
         a = to_host(s)
         if a is not None and a.ndim == 2:  # [B, nSig] -> [nSig, B]
             a = a.T.copy()
@@ -578,6 +581,8 @@ class Pyramid:
 
     @property
     def signal_2D(self):
+        if getattr(self, '_stacked_sources', False):       # asterism: [nSrc, 2w, w]
+            return self._signal_2D.detach().double().cpu().numpy()
         return to_host(self._signal_2D)
 
     @property
@@ -684,6 +689,7 @@ class Pyramid:
         if len(src_list) > 1:
             self._signal_2D = torch.stack(sig2d) if sig2d[0] is not None else None
             self._signal = torch.stack(sig) if sig[0] is not None else None
+        self._stacked_sources = len(src_list) > 1
         self.frames = frames
 
     def focal_intensity_dev(self, window):

@@ -60,6 +60,8 @@ SIGNATURES = {
     'oopao_atm_minmax': (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     'oopao_atm_layer_opd': (c_int, [c_void_p, c_int, c_void_p, c_double, c_double, c_int, c_int, c_double, c_void_p,
                                     c_int, c_int, c_void_p]),
+    'oopao_bilinear_affine': (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_double, c_double, c_double,
+                                      c_double, c_int, c_void_p]),
     'oopao_dm_gaussian_modes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_double, c_double, c_double,
                                         c_float, c_void_p]),
     'oopao_gemv': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_float, c_void_p]),

@@ -187,6 +187,33 @@ __global__ void k_layer_opd(const double* __restrict__ map, int M, const double*
 
 using namespace oopao;
 
+namespace oopao {
+
+// out[r][c] (+)= weight * bilinear(in; y = a r + by, x = a c + bx) on an H x W float32 image,
+// 0 outside (scikit-image warp order 1, mode constant, cval 0: neighbours floor/ceil of the
+// coordinate, any neighbour outside the image contributes 0). Used for the off-axis footprint's
+// sub-pixel translation and the laser-guide-star cone magnification (tools/
+// interpolateGeometricalTransformation.py:152-265).
+__global__ void k_bilinear_affine(const float* __restrict__ in, int H, int W, float* __restrict__ out, int h, int w,
+                                  double a, double by, double bx, double weight, int accumulate) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= h * w) return;
+  const int r = idx / w, c = idx % w;
+  const double y = a * r + by, x = a * c + bx;
+  const double fy = floor(y), fx = floor(x);
+  const double cy = ceil(y), cx = ceil(x);
+  const double dy = y - fy, dx = x - fx;
+  auto px = [&](double rr, double cc) -> double {
+    return (rr >= 0.0 && rr <= H - 1 && cc >= 0.0 && cc <= W - 1) ? (double)in[(size_t)rr * W + (size_t)cc] : 0.0;
+  };
+  const double top = (1.0 - dx) * px(fy, fx) + dx * px(fy, cx);
+  const double bot = (1.0 - dx) * px(cy, fx) + dx * px(cy, cx);
+  const double v = weight * ((1.0 - dy) * top + dy * bot);
+  out[idx] = accumulate ? (float)((double)out[idx] + v) : (float)v;
+}
+
+}  // namespace oopao
+
 extern "C" {
 
 int oopao_atm_add_row(const double* map, double* tmp, int M, int sx, int sy, const int* inner_idx, int n_inner,
@@ -219,6 +246,16 @@ int oopao_atm_layer_opd(const double* map, int M, const double* mm, double tx, d
   OOPAO_CHECK_ARG(n > 0 && M > 0, "invalid sizes");
   hipLaunchKernelGGL(k_layer_opd, dim3((n * n + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
                      map, M, mm, tx, ty, r0, c0, w, out, n, accumulate);
+  OOPAO_LAUNCH_CHECK();
+  return OK;
+}
+
+int oopao_bilinear_affine(const float* in, int H, int W, float* out, int h, int w, double a, double by, double bx,
+                          double weight, int accumulate, void* stream) {
+  OOPAO_CHECK_ARG(in && out, "null pointer argument");
+  OOPAO_CHECK_ARG(H > 0 && W > 0 && h > 0 && w > 0 && (long long)h * w < (1LL << 31), "invalid sizes");
+  hipLaunchKernelGGL(k_bilinear_affine, dim3((unsigned)((h * w + 255) / 256)), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), in, H, W, out, h, w, a, by, bx, weight, accumulate);
   OOPAO_LAUNCH_CHECK();
   return OK;
 }

@@ -13,6 +13,16 @@ Test infrastructure only (see ``oracle/__init__.py``).
   pupil-footprint crop of fill_OPD_support (589-622) for an on-axis source at
   infinity, in float64, consuming the layer's RandomState exactly like the
   reference (one normal(size=nOuter) draw per extruded ring).
+* warp_bilinear: scikit-image's order-1 warp (mode 'constant', cval 0) for an
+  affine map y = a r + by, x = a c + bx: the four floor/ceil neighbours of the
+  mapped coordinate, any neighbour outside the image counted as cval. Pinned by
+  tests/golden/bilinear_skimage.npz (scikit-image 0.18.3 itself, on the
+  translation and scale-about-centre transforms the reference composes in
+  tools/interpolateGeometricalTransformation.py:152-265).
+* source_layer: one layer's contribution to an off-axis / laser-guide-star
+  source (set_pupil_footprint, Atmosphere.py:423-445; fill_OPD_support,
+  589-622): footprint position, bilinear sub-pixel translation of the whole
+  layer, crop, cone magnification.
 """
 import numpy as np
 
@@ -123,3 +133,47 @@ def atmosphere_opd(layers, fractionalR0, n, wavelength=500e-9):
     for lay, f in zip(layers, fractionalR0):
         out += footprint_crop(lay.OPD, n) * wavelength / 2 / np.pi * np.sqrt(f)
     return out
+
+
+def warp_bilinear(img, a, by, bx, out_shape, cval=0.0):
+    """out[r, c] = bilinear interpolation of img at (a r + by, a c + bx), neighbours outside = cval."""
+    img = np.asarray(img, dtype=np.float64)
+    H, W = img.shape
+    rr, cc = np.meshgrid(np.arange(out_shape[0]) * a + by, np.arange(out_shape[1]) * a + bx, indexing='ij')
+    r0, c0 = np.floor(rr), np.floor(cc)
+    r1, c1 = np.ceil(rr), np.ceil(cc)
+    dr, dc = rr - r0, cc - c0
+
+    def px(r, c):
+        ok = (r >= 0) & (r <= H - 1) & (c >= 0) & (c <= W - 1)
+        v = np.full(r.shape, cval)
+        v[ok] = img[r[ok].astype(np.int64), c[ok].astype(np.int64)]
+        return v
+    top = (1 - dc) * px(r0, c0) + dc * px(r0, c1)
+    bot = (1 - dc) * px(r1, c0) + dc * px(r1, c1)
+    return (1 - dr) * top + dr * bot
+
+
+def source_layer(layer_full, n, layer_D, layer_altitude, zenith_arcsec, azimuth_deg, src_altitude,
+                 chromatic_shift=0.0):
+    """The layer's phase (radians at 500 nm) on the footprint of a source (Atmosphere.py:423-445, 589-622)."""
+    res = layer_full.shape[0]
+    rad2arcsec = 180 / np.pi * 3600
+    rho = layer_altitude * np.tan((zenith_arcsec + chromatic_shift) / rad2arcsec) * res / layer_D
+    x_z, y_z = rho * np.cos(np.deg2rad(azimuth_deg)), rho * np.sin(np.deg2rad(azimuth_deg))
+    esx, esy = int(x_z) - x_z, int(y_z) - y_z
+    im = np.asarray(layer_full, dtype=np.float64)
+    if esx != 0 or esy != 0:
+        # interpolate_image with a pure translation: out[r, c] = im(r - esy, c - esx)
+        im = warp_bilinear(im, 1.0, -esy, -esx, im.shape)
+    cx, cy = int(y_z) + res // 2, int(x_z) + res // 2
+    sub = im[cx - n // 2:cx + n // 2, cy - n // 2:cy + n // 2]
+    if np.isfinite(src_altitude):
+        m = (src_altitude - layer_altitude) / src_altitude
+        ratio = 1.0 / m
+        extra = (ratio % 1) / 2 + (np.floor(ratio) - 1) * 0.5
+        # scale about the image centre n/2 then translate by extra: out(p) = sub((p - n/2 - extra) / ratio + n/2)
+        b = -(n / 2 + extra) / ratio + n / 2
+        sub = warp_bilinear(sub, 1.0 / ratio, b, b, (n, n))
+    return sub
+
