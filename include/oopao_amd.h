@@ -240,6 +240,26 @@ int oopao_gsc_sensitivity(const void* ir1, const void* v2, const float* bp, int 
                           size_t ws_bytes, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Angular-spectrum propagation for the scintillation mode of the atmosphere
+ * (Atmosphere.fill_scintillation_support / set_scintillation_support / ASM,
+ * Atmosphere.py:644-706, 938-972)
+ * ------------------------------------------------------------------------- */
+
+/* field = (init ? 1 : field) * exp(i k opd) for count complex64 values. */
+int oopao_field_phase(const float* opd, float k, void* field, int count, int init, void* stream);
+
+int oopao_asm_bytes(int N, size_t* bytes);
+
+/* In place: field = fftshift(ifft2(ifftshift(fftshift(fft2(ifftshift(field))) H))) for an even
+ * N x N complex64 field and the centred-grid transfer function H (complex64 N x N). */
+int oopao_asm_propagate(void* field, const void* H, int N, void* ws, size_t ws_bytes, void* stream);
+
+/* intensity = |field|^2 (if non-NULL); opd = angle(field) / k, or with a guide geo (non-NULL)
+ * opd = geo + angle(field exp(-i k geo)) / k (if opd non-NULL). */
+int oopao_field_readout(const void* field, const float* geo, float k, float* intensity, float* opd, int count,
+                        void* stream);
+
+/* ---------------------------------------------------------------------------
  * Atmosphere frozen-flow update (replaces Atmosphere.add_row / updateLayer and
  * the footprint sum of fill_OPD_support: Atmosphere.py:410-421, 447-516,
  * 589-622; the warp of tools/tools.py:202-209)

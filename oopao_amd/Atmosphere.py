@@ -16,8 +16,11 @@ for it, Atmosphere.py:297-300). Each source's OPD follows fill_OPD_support
 (set_pupil_footprint, 423-445), the sub-pixel remainder of the footprint
 position applied as a bilinear translation of the layer, and for a laser guide
 star the cone-effect magnification (bilinear, about the footprint centre),
-on the GPU (oopao_bilinear_affine). Geometric propagation only (angular-spectrum
-scintillation and boiling raise OopaoError).
+on the GPU (oopao_bilinear_affine). With angular_spectrum_propagation the source
+field is propagated through the layers by the angular spectrum (oopao_asm_propagate,
+Atmosphere.py:644-706, 938-972), giving the scintillation (intensity) and the phase
+(wrapped, geometric, or geometric guide + wrapped residual). Boiling raises
+OopaoError.
 """
 import ctypes
 
@@ -41,13 +44,16 @@ class Atmosphere:
                  angular_spectrum_propagation: bool = False, geometric_phase_backup: bool = False,
                  unwrap_diffractive_phase=False, rytov_var: float = None, rytov_wvl: float = 500e-9,
                  t_boiling=None):
-        if angular_spectrum_propagation:
-            raise OopaoError('angular-spectrum (scintillation) propagation is not supported by oopao_amd')
         self.tag = 'atmosphere'
         self.device = telescope.device
         self.hasNotBeenInitialized = True
-        self.angular_spectrum_propagation = False
+        self.angular_spectrum_propagation = bool(angular_spectrum_propagation)
         self.geometric_phase_backup = geometric_phase_backup
+        self.unwrap_diffractive_phase = unwrap_diffractive_phase
+        self.rytov_wvl = rytov_wvl
+        self.sampling_checked = False
+        self._saturation_warned = False
+        self._transfer = {}
         self.altitude = altitude
         self.wavelength = 500 * 1e-9
         self._elevation = max(5.0, float(elevation))
@@ -80,6 +86,9 @@ class Atmosphere:
         self.asterism = None if self.src.tag == 'source' else self.src
         self.param = param
         self._OPD_dev = None
+        self.update_rytov_variance()
+        if rytov_var is not None:
+            self.rytov_var = float(rytov_var)
 
     # ------------------------------------------------------------ init (host)
     def initializeAtmosphere(self, telescope=None, compute_covariance=True):
@@ -321,9 +330,17 @@ class Atmosphere:
             m = s._mask_dev()
             s._scintillation = s._scintillation_no_pupil if m is None else s._scintillation_no_pupil * m
             s._refresh_intensity()
-            o = opd.clone() if (user or self._on_axis(s)) else self._source_opd(s)
+            if self.angular_spectrum_propagation and not user:
+                o, inten = self._scintillation(s)
+                s._scintillation_no_pupil = inten
+                s._scintillation = inten if m is None else inten * m
+                s._refresh_intensity()
+            else:
+                o = opd.clone() if (user or self._on_axis(s)) else self._source_opd(s)
             self._src_opd.append(o)
             s._set_OPD_no_pupil_dev(o)
+        if self.angular_spectrum_propagation and not user:
+            self.scintillation_map = np.squeeze(np.array([to_host(x._scintillation_no_pupil) for x in self.src_list]))
 
     # ------------------------------------------------------- off-axis / LGS
     @staticmethod
@@ -361,37 +378,191 @@ class Atmosphere:
         cy = int(x_z) + lay.resolution // 2
         return cx - n // 2, cy - n // 2, int(x_z) - x_z, int(y_z) - y_z
 
+    def _source_layer_opd(self, s, i, out, accumulate):
+        """Layer i's OPD (metres) on the footprint of source s into out [n, n] (fill_OPD_support,
+        Atmosphere.py:589-622), added to out when ``accumulate``."""
+        n = self.telescope.resolution
+        st = stream_ptr(self.device)
+        lay = getattr(self, 'layer_' + str(i + 1))
+        if s.altitude <= lay.altitude:
+            raise OopaoError('The source altitude (' + str(s.altitude) + ' m) is below or at the same altitude as '
+                             'the atmosphere layer (' + str(lay.altitude) + ' m)')
+        w = self.wavelength / 2 / np.pi * np.sqrt(self.fractionalR0[i])
+        if self._on_axis(s):
+            tx, ty = getattr(lay, 'shift', (0.0, 0.0))
+            r0 = lay.center - n // 2 + 1
+            _lib.call('oopao_atm_layer_opd', ptr(lay.map), lay.M, ptr(lay.mm), ctypes.c_double(tx),
+                      ctypes.c_double(ty), r0, r0, ctypes.c_double(w), ptr(out), n, int(accumulate), st)
+            return
+        full = self._full_layer(i)
+        r0, c0, esx, esy = self.footprint(lay, s, i)
+        res = lay.resolution
+        if np.isfinite(s.altitude):
+            # footprint (translated by the sub-pixel remainder), then the cone magnification
+            # m = (H - h) / H about the footprint centre, with the reference's half-pixel alignment
+            sub = torch.empty((n, n), dtype=torch.float32, device=self.device)
+            _lib.call('oopao_bilinear_affine', ptr(full), res, res, ptr(sub), n, n, ctypes.c_double(1.0),
+                      ctypes.c_double(r0 - esy), ctypes.c_double(c0 - esx), ctypes.c_double(1.0), 0, st)
+            m = (s.altitude - lay.altitude) / s.altitude
+            ratio = 1.0 / m
+            e = (ratio % 1) / 2 + (np.floor(ratio) - 1) * 0.5
+            b = -(n / 2 + e) / ratio + n / 2
+            _lib.call('oopao_bilinear_affine', ptr(sub), n, n, ptr(out), n, n, ctypes.c_double(1.0 / ratio),
+                      ctypes.c_double(b), ctypes.c_double(b), ctypes.c_double(w), int(accumulate), st)
+        else:
+            _lib.call('oopao_bilinear_affine', ptr(full), res, res, ptr(out), n, n, ctypes.c_double(1.0),
+                      ctypes.c_double(r0 - esy), ctypes.c_double(c0 - esx), ctypes.c_double(w), int(accumulate), st)
+
     def _source_opd(self, s):
         """Sum over layers of the source's footprint (fill_OPD_support, Atmosphere.py:589-622), metres."""
         n = self.telescope.resolution
+        out = torch.empty((n, n), dtype=torch.float32, device=self.device)
+        for i in range(self.nLayer):
+            self._source_layer_opd(s, i, out, i > 0)
+        return out
+
+    # ---------------------------------------------------- scintillation (ASM)
+    def _transfer_function(self, wvl, distance):
+        """H = exp(-i pi lambda z (fx^2 + fy^2)) on the centred frequency grid (Atmosphere.py:938-972)."""
+        n = self.telescope.resolution
+        pitch = getattr(self.telescope, 'pixelSize', self.telescope.D / n)
+        key = (n, float(pitch), float(wvl), float(distance))
+        H = self._transfer.get(key)
+        if H is None:
+            f = np.arange(-n / 2, n / 2) / (n * pitch)
+            fx, fy = np.meshgrid(f, f)
+            Hh = np.exp(-1j * np.pi * wvl * distance * (fx ** 2 + fy ** 2))
+            H = torch.from_numpy(Hh.astype(np.complex64)).to(self.device)
+            self._transfer[key] = H
+        return H
+
+    def _scintillation(self, s):
+        """Field of source s through the layers, highest first, each multiplied in and propagated to the
+        next one (or the ground) by the angular spectrum (fill_scintillation_support,
+        Atmosphere.py:644-662), then (OPD, intensity) as set_scintillation_support (664-706)."""
+        self.check_fresnel_sampling()
+        n = self.telescope.resolution
         dev = self.device
         st = stream_ptr(dev)
-        out = torch.zeros((n, n), dtype=torch.float32, device=dev)
-        sub = torch.empty((n, n), dtype=torch.float32, device=dev)
+        k = 2 * np.pi / s.wavelength
+        order = sorted(range(self.nLayer), key=lambda i: getattr(self, 'layer_' + str(i + 1)).altitude, reverse=True)
+        field = torch.empty((n, n), dtype=torch.complex64, device=dev)
+        lopd = torch.empty((n, n), dtype=torch.float32, device=dev)
+        nb = ctypes.c_size_t(0)
+        _lib.call('oopao_asm_bytes', n, ctypes.byref(nb))
+        ws = torch.empty(nb.value, dtype=torch.uint8, device=dev)
+        alts = [getattr(self, 'layer_' + str(i + 1)).altitude for i in order]
+        for j, i in enumerate(order):
+            self._source_layer_opd(s, i, lopd, False)
+            _lib.call('oopao_field_phase', ptr(lopd), ctypes.c_float(k), ptr(field), n * n, int(j == 0), st)
+            dist = alts[j] - alts[j + 1] if j < len(order) - 1 else alts[j]
+            if dist > 1e-6:
+                _lib.call('oopao_asm_propagate', ptr(field), ptr(self._transfer_function(s.wavelength, dist)), n,
+                          ptr(ws), ws.numel(), st)
+        inten = torch.empty((n, n), dtype=torch.float32, device=dev)
+        if self.geometric_phase_backup:
+            geo = self._source_opd(s)
+            _lib.call('oopao_field_readout', ptr(field), None, ctypes.c_float(k), ptr(inten), None, n * n, st)
+            return geo, inten
+        out = torch.empty((n, n), dtype=torch.float32, device=dev)
+        geo = self._source_opd(s) if self.unwrap_diffractive_phase else None
+        _lib.call('oopao_field_readout', ptr(field), None if geo is None else ptr(geo), ctypes.c_float(k), ptr(inten),
+                  ptr(out), n * n, st)
+        if geo is not None:
+            pupil = self.telescope._pupil_dev > 0
+            delta = (out - geo)[pupil] * k
+            if delta.numel() and float(delta.max() - delta.min()) > 1.9 * np.pi:
+                warning('Diffractive residual wrapped inside the pupil! Scintillation is too strong for perfect '
+                        'unwrapping.')
+        return out, inten
+
+    def update_rytov_variance(self):
+        """Rytov variance of the profile at rytov_wvl (Atmosphere.py:917-936)."""
+        if not hasattr(self, 'nLayer') or not hasattr(self, 'fractionalR0') or not hasattr(self, 'altitude'):
+            return
+        k_t = 2 * np.pi / self.rytov_wvl
+        r0_t = self.r0 * (self.rytov_wvl / self.wavelength) ** (6 / 5)
+        cn2_total = (r0_t ** (-5 / 3)) / (0.423 * k_t ** 2)
+        cur = 0.0
         for i in range(self.nLayer):
-            lay = getattr(self, 'layer_' + str(i + 1))
-            if s.altitude <= lay.altitude:
-                raise OopaoError('The source altitude (' + str(s.altitude) + ' m) is below or at the same altitude as '
-                                 'the atmosphere layer (' + str(lay.altitude) + ' m)')
-            full = self._full_layer(i)
-            r0, c0, esx, esy = self.footprint(lay, s, i)
-            w = self.wavelength / 2 / np.pi * np.sqrt(self.fractionalR0[i])
-            res = lay.resolution
-            if np.isfinite(s.altitude):
-                # footprint (translated by the sub-pixel remainder), then the cone magnification
-                # m = (H - h) / H about the footprint centre, with the reference's half-pixel alignment
-                _lib.call('oopao_bilinear_affine', ptr(full), res, res, ptr(sub), n, n, ctypes.c_double(1.0),
-                          ctypes.c_double(r0 - esy), ctypes.c_double(c0 - esx), ctypes.c_double(1.0), 0, st)
-                m = (s.altitude - lay.altitude) / s.altitude
-                ratio = 1.0 / m
-                e = (ratio % 1) / 2 + (np.floor(ratio) - 1) * 0.5
-                b = -(n / 2 + e) / ratio + n / 2
-                _lib.call('oopao_bilinear_affine', ptr(sub), n, n, ptr(out), n, n, ctypes.c_double(1.0 / ratio),
-                          ctypes.c_double(b), ctypes.c_double(b), ctypes.c_double(w), 1, st)
-            else:
-                _lib.call('oopao_bilinear_affine', ptr(full), res, res, ptr(out), n, n, ctypes.c_double(1.0),
-                          ctypes.c_double(r0 - esy), ctypes.c_double(c0 - esx), ctypes.c_double(w), 1, st)
-        return out
+            if self.altitude[i] > 1e-3:
+                cur += (cn2_total * self.fractionalR0[i]) * (self.altitude[i] ** (5 / 6))
+        self._rytov_var = cur * 2.2524 * (k_t ** (7 / 6))
+        if self.angular_spectrum_propagation:
+            if self._rytov_var > 0.3 and not self._saturation_warned:
+                warning(f"Atmospheric conditions degraded. Rytov variance ({self._rytov_var:.2f}) exceeds 0.3 limit! "
+                        "Entering saturation regime.")
+                self._saturation_warned = True
+            elif self._rytov_var <= 0.3:
+                self._saturation_warned = False
+
+    @property
+    def rytov_var(self):
+        if not self.angular_spectrum_propagation:
+            return 0.0
+        if not hasattr(self, '_rytov_var'):
+            self.update_rytov_variance()
+        return self._rytov_var
+
+    @rytov_var.setter
+    def rytov_var(self, val):
+        """Scale r0 (and the current layer screens) to reach a target Rytov variance (Atmosphere.py:1172-1205)."""
+        if not self.angular_spectrum_propagation:
+            warning("Cannot set Rytov variance because atm.angular_spectrum_propagation is False.")
+            return
+        val = float(val)
+        if val <= 0:
+            return
+        if val > 0.3:
+            warning(f"Target Rytov variance ({val:.2f}) exceeds the weak fluctuation limit (0.3). The simulation is "
+                    "entering the moderate/strong scintillation regime (saturation). Phase unwrapping and scaling "
+                    "behaviors may become unstable.")
+        self.update_rytov_variance()
+        cur = self._rytov_var
+        if cur > 0:
+            ratio = val / cur
+            print(f"--- Scaling Rytov variance to {val:.4f} (Ratio: {ratio:.3f}) ---")
+            self.r0 = self.r0 * (ratio ** (-3 / 5))
+            scale = float(np.sqrt(ratio))
+            if not self.hasNotBeenInitialized:
+                for i in range(self.nLayer):
+                    lay = getattr(self, 'layer_' + str(i + 1))
+                    lay.map.mul_(scale)
+                    lay.mm.mul_(scale)
+                self._compute_opd()
+            self.update_rytov_variance()
+        else:
+            warning("Cannot scale the Rytov variance: every layer is at the ground.")
+
+    def check_fresnel_sampling(self):
+        """Grid checks for the Fresnel propagation (Atmosphere.py:974-1010): warnings only."""
+        if not self.angular_spectrum_propagation or self.sampling_checked:
+            return
+        tel = self.telescope
+        D = getattr(tel, 'initial_D', tel.D)
+        N = tel.resolution
+        delta = getattr(tel, 'pixelSize', tel.D / N)
+        alts = sorted(list(self.altitude) + [0.0], reverse=True)
+        max_step = max([alts[i] - alts[i + 1] for i in range(len(alts) - 1)]) if len(alts) > 1 else 0
+        z_max = max(self.altitude) if len(self.altitude) else 0
+        for src in self.src_list:
+            wvl = src.wavelength
+            r0_w = self._r0 * (wvl / self.wavelength) ** (6 / 5)
+            D_turb = 4.0 * (wvl * z_max) / r0_w
+            D_total = D + 2.0 * D_turb
+            delta_req = min(r0_w / 4.0, (wvl * z_max) / D_total) if z_max > 0 else r0_w / 4.0
+            N_min = (D_total / delta) + (wvl * z_max) / (delta ** 2)
+            z_asm = (N * delta ** 2) / wvl
+            if delta > delta_req:
+                warning(f"Fresnel [wvl={wvl*1e9:.0f}nm] - Pixel scale ({delta*1000:.1f} mm) too large! "
+                        f"Limit is {delta_req*1000:.1f} mm.")
+            if N < N_min:
+                warning(f"Fresnel Aliasing [wvl={wvl*1e9:.0f}nm] - Grid N={N} too small (Physics requires "
+                        f"N >= {int(np.ceil(N_min))}).")
+            if max_step > z_asm:
+                warning(f"Fresnel ASM Limit [wvl={wvl*1e9:.0f}nm] - Distance between layers ({max_step/1000:.1f} km) "
+                        f"> limit ({z_asm/1000:.1f} km).")
+        self.sampling_checked = True
 
     # -------------------------------------------------------------- properties
     @property

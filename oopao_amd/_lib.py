@@ -62,6 +62,10 @@ SIGNATURES = {
                                     c_int, c_int, c_void_p]),
     'oopao_bilinear_affine': (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_double, c_double, c_double,
                                       c_double, c_int, c_void_p]),
+    'oopao_field_phase': (c_int, [c_void_p, c_float, c_void_p, c_int, c_int, c_void_p]),
+    'oopao_asm_bytes': (c_int, [c_int, ctypes.POINTER(c_size_t)]),
+    'oopao_asm_propagate': (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_size_t, c_void_p]),
+    'oopao_field_readout': (c_int, [c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p]),
     'oopao_dm_gaussian_modes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_double, c_double, c_double,
                                         c_float, c_void_p]),
     'oopao_gemv': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_float, c_void_p]),

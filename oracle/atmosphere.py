@@ -23,6 +23,10 @@ Test infrastructure only (see ``oracle/__init__.py``).
   source (set_pupil_footprint, Atmosphere.py:423-445; fill_OPD_support,
   589-622): footprint position, bilinear sub-pixel translation of the whole
   layer, crop, cone magnification.
+* asm, scintillation_field: the angular-spectrum propagation (Atmosphere.ASM,
+  938-972, equal pitches) and the top-down layer loop of
+  fill_scintillation_support (644-662). Pinned by the analytic Fresnel
+  propagation of a Gaussian beam, Parseval and the zero-distance identity.
 """
 import numpy as np
 
@@ -176,4 +180,28 @@ def source_layer(layer_full, n, layer_D, layer_altitude, zenith_arcsec, azimuth_
         b = -(n / 2 + extra) / ratio + n / 2
         sub = warp_bilinear(sub, 1.0 / ratio, b, b, (n, n))
     return sub
+
+
+def asm(field, wavelength, pitch, distance):
+    """fftshift(ifft2(ifftshift(fftshift(fft2(ifftshift(E))) exp(-i pi lambda z f^2)))) (Atmosphere.py:938-972)."""
+    if distance == 0:
+        return field
+    N = field.shape[0]
+    f = np.arange(-N / 2, N / 2) / (N * pitch)
+    fx, fy = np.meshgrid(f, f)
+    H = np.exp(-1j * np.pi * wavelength * distance * (fx ** 2 + fy ** 2))
+    F = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(field)))
+    return np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(F * H)))
+
+
+def scintillation_field(layer_opds, altitudes, wavelength, pitch):
+    """Unit field multiplied by each layer's phase, highest layer first, propagated to the next layer and
+    finally to the ground (Atmosphere.py:563-571, 644-662). layer_opds [m] and altitudes sorted top-down."""
+    E = np.ones(layer_opds[0].shape, dtype=complex)
+    for j, (opd, alt) in enumerate(zip(layer_opds, altitudes)):
+        E = E * np.exp(1j * 2 * np.pi / wavelength * opd)
+        dist = alt - altitudes[j + 1] if j < len(altitudes) - 1 else alt
+        if dist > 1e-6:
+            E = asm(E, wavelength, pitch, dist)
+    return E
 

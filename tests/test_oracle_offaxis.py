@@ -55,3 +55,19 @@ def test_source_layer_closed_forms():
     assert np.allclose(got[inner], ref[inner], rtol=0, atol=1e-12)
     # the pixel-centre alignment: the magnification keeps the centre (n - 1) / 2 fixed
     assert np.isclose((((n - 1) / 2) - n / 2 - extra) / ratio + n / 2, (n - 1) / 2)
+
+
+def test_asm_gaussian_beam_parseval_identity():
+    from oracle.atmosphere import asm
+    N, pitch, wvl, w0, z = 128, 0.02, 500e-9, 0.2, 20e3
+    x = (np.arange(N) - N / 2) * pitch
+    X, Y = np.meshgrid(x, x)
+    E0 = np.exp(-(X ** 2 + Y ** 2) / w0 ** 2)
+    assert asm(E0, wvl, pitch, 0) is E0
+    E = asm(E0, wvl, pitch, z)
+    q = w0 ** 2 + 1j * wvl * z / np.pi            # paraxial Gaussian beam for H = exp(-i pi lambda z f^2)
+    ref = w0 ** 2 / q * np.exp(-(X ** 2 + Y ** 2) / q)
+    assert np.max(np.abs(E - ref)) < 1e-10
+    rs = np.random.RandomState(0)
+    F = rs.randn(N, N) + 1j * rs.randn(N, N)
+    assert np.isclose(np.sum(np.abs(asm(F, wvl, pitch, 5e3)) ** 2), np.sum(np.abs(F) ** 2), rtol=1e-12)
