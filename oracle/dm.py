@@ -57,3 +57,16 @@ class DMOracle:
         if coefs.ndim == 1:
             return out.reshape(self.n, self.n)
         return out.reshape(self.n, self.n, coefs.shape[1])
+
+    def opd_separable(self, coefs):
+        """The same OPD through the separable identity G_y C G_x^T (zero mis-registration Gaussian, pinned
+        against modes @ coefs in tests/test_oracle_dm_atm.py); avoids the dense [n^2, nAct] modes."""
+        ux = np.unique(np.round(self.u0x, 9))
+        uy = np.unique(np.round(self.u0y, 9))
+        a = 1 / (2 * self.c ** 2)
+        gx = np.exp(-a * (self.grid[:, None] - ux[None, :]) ** 2)
+        gy = np.exp(-a * (self.grid[:, None] - uy[None, :]) ** 2)
+        C = np.zeros((len(uy), len(ux)))
+        C[np.searchsorted(uy, np.round(self.u0y, 9)), np.searchsorted(ux, np.round(self.u0x, 9))] = coefs
+        return gy @ C @ gx.T
+
