@@ -267,13 +267,14 @@ int oopao_field_readout(const void* field, const float* geo, float k, float* int
 
 /* One whole-pixel step of a layer: tmp = map shifted by (sx, sy) in {-1,0,1}
  * (tmp[r][c] = map[r-sy][c-sx]), z = tmp[inner_idx], X = A z + B xi,
- * tmp[outer_idx] = X, mm = (min, max) of tmp. map/tmp [M*M], A [n_outer][n_inner],
- * B [n_outer][n_outer], xi [n_outer] float64 (the layer's RandomState draw). */
+ * tmp[outer_idx] = X, mm[0..1] = (min, max) of tmp. map/tmp [M*M], A [n_outer][n_inner],
+ * B [n_outer][n_outer], xi [n_outer] float64 (the layer's RandomState draw); mm holds 4
+ * doubles (mm[2..3] are working space). Three kernel launches. */
 int oopao_atm_add_row(const double* map, double* tmp, int M, int sx, int sy, const int* inner_idx, int n_inner,
                       const int* outer_idx, int n_outer, const double* A, const double* Bm, const double* xi,
                       double* z, double* X, double* mm, void* stream);
 
-/* mm[0] = min(map[0..count)), mm[1] = max. */
+/* mm[0] = min(map[0..count)), mm[1] = max; mm holds 4 doubles (mm[2..3] are working space). */
 int oopao_atm_minmax(const double* map, int count, double* mm, void* stream);
 
 /* out[y][x] (+)= w * clip(cubic(map, row y + r0 - ty, col x + c0 - tx)) for an

@@ -170,7 +170,7 @@ class Atmosphere:
         lay.tmp = torch.zeros(M * M, dtype=torch.float64, device=dev)
         lay.z = torch.zeros(lay.inner_idx.numel(), dtype=torch.float64, device=dev)
         lay.X = torch.zeros(lay.outer_idx.numel(), dtype=torch.float64, device=dev)
-        lay.mm = torch.zeros(2, dtype=torch.float64, device=dev)
+        lay.mm = torch.zeros(4, dtype=torch.float64, device=dev)   # (min, max) + working space
         lay.xi = torch.zeros(lay.outer_idx.numel(), dtype=torch.float64, device=dev)
         return lay
 
@@ -528,7 +528,7 @@ class Atmosphere:
                 for i in range(self.nLayer):
                     lay = getattr(self, 'layer_' + str(i + 1))
                     lay.map.mul_(scale)
-                    lay.mm.mul_(scale)
+                    lay.mm[:2].mul_(scale)
                 self._compute_opd()
             self.update_rytov_variance()
         else:

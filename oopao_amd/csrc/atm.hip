@@ -18,27 +18,28 @@
 
 namespace oopao {
 
-// dst[r][c] = src[r - sy][c - sx] (0 outside), M x M maps, |sx|,|sy| <= 1 integers.
-__global__ void k_shift_int(const double* __restrict__ src, double* __restrict__ dst, int M, int sx, int sy) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= M * M) return;
-  const int r = idx / M, c = idx % M;
+// dst[r][c] = src[r - sy][c - sx] (0 outside), M x M maps, |sx|,|sy| <= 1 integers; in the same
+// launch z[k] = dst[inner_idx[k]] (read from src through the same shift) and the min / max
+// accumulator of the next k_minmax is reset.
+__device__ __forceinline__ double shifted(const double* __restrict__ src, int M, int sx, int sy, int r, int c) {
   const int rs = r - sy, cs = c - sx;
-  dst[idx] = (rs >= 0 && rs < M && cs >= 0 && cs < M) ? src[rs * M + cs] : 0.0;
+  return (rs >= 0 && rs < M && cs >= 0 && cs < M) ? src[rs * M + cs] : 0.0;
 }
-
-__global__ void k_gather(const double* __restrict__ map, const int* __restrict__ idx, double* __restrict__ out,
-                         int n) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) out[k] = map[idx[k]];
+__global__ void k_shift_gather(const double* __restrict__ src, double* __restrict__ dst, int M, int sx, int sy,
+                               const int* __restrict__ inner_idx, int n_inner, double* __restrict__ z,
+                               unsigned long long* __restrict__ mm) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < M * M) dst[idx] = shifted(src, M, sx, sy, idx / M, idx % M);
+  if (idx < n_inner) {
+    const int p = inner_idx[idx];
+    z[idx] = shifted(src, M, sx, sy, p / M, p % M);
+  }
+  if (idx == 0) {
+    mm[0] = ~0ull;
+    mm[1] = 0ull;
+    mm[2] = 0ull;
+  }
 }
-
-__global__ void k_scatter(double* __restrict__ map, const int* __restrict__ idx, const double* __restrict__ v,
-                          int n) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) map[idx[k]] = v[k];
-}
-
 // X[r] = sum_k A[r][k] z[k] + sum_k B[r][k] xi[k]  (float64, one wavefront per row)
 // Partial dot product of one row with a vector: each lane reads double2 pairs, four
 // independent partial sums keep several 16-byte loads in flight.
@@ -70,22 +71,27 @@ __device__ __forceinline__ double row_dot(const double* __restrict__ a, const do
   return (s0 + s1) + (s2 + s3);
 }
 
-// X[r] = A[r,:] z + B[r,:] xi (the outer-ring extrusion, float64), one wavefront per row.
+// X[r] = A[r,:] z + B[r,:] xi (the outer-ring extrusion, float64), one wavefront per row; the
+// row's value is also scattered into the shifted map at outer_idx[r].
 __global__ void __launch_bounds__(256) k_extrude(const double* __restrict__ A, const double* __restrict__ z,
                                                  const double* __restrict__ Bm, const double* __restrict__ xi,
-                                                 double* __restrict__ X, int R, int KA, int KB) {
+                                                 double* __restrict__ X, const int* __restrict__ outer_idx,
+                                                 double* __restrict__ map, int R, int KA, int KB) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r = blockIdx.x * 4 + wave;
   if (r >= R) return;
   double acc = row_dot(A + (size_t)r * KA, z, KA, lane) + row_dot(Bm + (size_t)r * KB, xi, KB, lane);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-  if (lane == 0) X[r] = acc;
+  if (lane == 0) {
+    X[r] = acc;
+    map[outer_idx[r]] = acc;
+  }
 }
-
-// min / max of a map in three launches: sentinels, per-block reductions merged with 64-bit
-// atomic min / max on an order-preserving integer image of the doubles (the result does not
-// depend on the order of the merges), conversion back in place.
+// min / max of a map: per-block reductions merged with 64-bit atomic min / max on an
+// order-preserving integer image of the doubles (the result does not depend on the order of the
+// merges); the last block to finish (counted in mm[2]) converts the result back to doubles.
+// mm[0..2] must hold (~0, 0, 0) at launch (k_minmax_init or k_shift_gather).
 __device__ __forceinline__ unsigned long long dbl_ord(double v) {
   const unsigned long long u = (unsigned long long)__double_as_longlong(v);
   return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
@@ -94,17 +100,17 @@ __device__ __forceinline__ double dbl_unord(unsigned long long o) {
   const unsigned long long u = (o >> 63) ? (o & 0x7FFFFFFFFFFFFFFFull) : ~o;
   return __longlong_as_double((long long)u);
 }
-
 __global__ void k_minmax_init(unsigned long long* __restrict__ mm) {
   if (threadIdx.x == 0) {
     mm[0] = ~0ull;
     mm[1] = 0ull;
+    mm[2] = 0ull;
   }
 }
-
 __global__ void __launch_bounds__(256) k_minmax(const double* __restrict__ map, int n,
                                                 unsigned long long* __restrict__ mm) {
   __shared__ double lo[256], hi[256];
+  __shared__ int last;
   double a = INFINITY, b = -INFINITY;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double v = map[i];
@@ -124,28 +130,30 @@ __global__ void __launch_bounds__(256) k_minmax(const double* __restrict__ map, 
   if (threadIdx.x == 0) {
     atomicMin(&mm[0], dbl_ord(lo[0]));
     atomicMax(&mm[1], dbl_ord(hi[0]));
+    __threadfence();
+    last = atomicAdd(&mm[2], 1ull) == (unsigned long long)(gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    const unsigned long long ma = atomicAdd(&mm[0], 0ull);  // device-coherent reads of the merged values
+    const unsigned long long mb = atomicAdd(&mm[1], 0ull);
+    double* md = reinterpret_cast<double*>(mm);
+    md[0] = dbl_unord(ma);
+    md[1] = dbl_unord(mb);
   }
 }
-
-__global__ void k_minmax_final(unsigned long long* __restrict__ mm) {
-  if (threadIdx.x < 2) {
-    const double v = dbl_unord(mm[threadIdx.x]);
-    reinterpret_cast<double*>(mm)[threadIdx.x] = v;
-  }
-}
-
-static int launch_minmax(const double* map, int n, double* mm, hipStream_t st) {
+static int launch_minmax(const double* map, int n, double* mm, hipStream_t st, bool init) {
   unsigned long long* mb = reinterpret_cast<unsigned long long*>(mm);
-  hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, st, mb);
-  OOPAO_LAUNCH_CHECK();
+  if (init) {
+    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, st, mb);
+    OOPAO_LAUNCH_CHECK();
+  }
   const int blocks = (int)(((size_t)n + 1023) / 1024 < 256 ? ((size_t)n + 1023) / 1024 : 256);
   hipLaunchKernelGGL(k_minmax, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, st, map, n, mb);
   OOPAO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, st, mb);
-  OOPAO_LAUNCH_CHECK();
   return OK;
 }
-
 __device__ __forceinline__ double cubic(double x, double f0, double f1, double f2, double f3) {
   return f1 + 0.5 * x * (f2 - f0 + x * (2.0 * f0 - 5.0 * f1 + 4.0 * f2 - f3 + x * (3.0 * (f1 - f2) + f3 - f0)));
 }
@@ -223,21 +231,20 @@ int oopao_atm_add_row(const double* map, double* tmp, int M, int sx, int sy, con
   OOPAO_CHECK_ARG(M > 2 && n_inner > 0 && n_outer > 0, "invalid sizes");
   OOPAO_CHECK_ARG(sx >= -1 && sx <= 1 && sy >= -1 && sy <= 1, "integer shift must be -1, 0 or 1");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(k_shift_int, dim3((M * M + 255) / 256), dim3(256), 0, st, map, tmp, M, sx, sy);
+  unsigned long long* mb = reinterpret_cast<unsigned long long*>(mm);
+  const int cover = M * M > n_inner ? M * M : n_inner;
+  hipLaunchKernelGGL(k_shift_gather, dim3((cover + 255) / 256), dim3(256), 0, st, map, tmp, M, sx, sy, inner_idx,
+                     n_inner, z, mb);
   OOPAO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_gather, dim3((n_inner + 255) / 256), dim3(256), 0, st, tmp, inner_idx, z, n_inner);
+  hipLaunchKernelGGL(k_extrude, dim3((n_outer + 3) / 4), dim3(256), 0, st, A, z, Bm, xi, X, outer_idx, tmp, n_outer,
+                     n_inner, n_outer);
   OOPAO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_extrude, dim3((n_outer + 3) / 4), dim3(256), 0, st, A, z, Bm, xi, X, n_outer, n_inner,
-                     n_outer);
-  OOPAO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scatter, dim3((n_outer + 255) / 256), dim3(256), 0, st, tmp, outer_idx, X, n_outer);
-  OOPAO_LAUNCH_CHECK();
-  return launch_minmax(tmp, M * M, mm, st);
+  return launch_minmax(tmp, M * M, mm, st, false);
 }
 
 int oopao_atm_minmax(const double* map, int count, double* mm, void* stream) {
   OOPAO_CHECK_ARG(map && mm && count > 0, "invalid arguments");
-  return launch_minmax(map, count, mm, reinterpret_cast<hipStream_t>(stream));
+  return launch_minmax(map, count, mm, reinterpret_cast<hipStream_t>(stream), true);
 }
 
 int oopao_atm_layer_opd(const double* map, int M, const double* mm, double tx, double ty, int r0, int c0, double w,
