@@ -78,6 +78,19 @@ class Asterism:
         return np.array(self._each('OPD_no_pupil'))
 
     @property
+    def scintillation(self):
+        return np.array(self._each('scintillation'), dtype=object)
+
+    @scintillation.setter
+    def scintillation(self, val):
+        for s in self.src:
+            s.scintillation = val[s.ast_idx]
+
+    @property
+    def scintillation_no_pupil(self):
+        return np.array(self._each('scintillation_no_pupil'), dtype=object)
+
+    @property
     def intensity(self):
         return self._each('intensity')
 

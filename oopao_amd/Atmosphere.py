@@ -593,7 +593,9 @@ class Atmosphere:
     @r0.setter
     def r0(self, val):
         """r0 setter (Atmosphere.py:1017-1036): rescales the covariances and recomputes B."""
+        self.sampling_checked = False
         self._r0 = val
+        self.r0_zenith = val * np.sin(np.radians(self.elevation)) ** (-3 / 5)
         if self.hasNotBeenInitialized is False:
             print('Updating the Atmosphere covariance matrices...')
             self.seeingArcsec = self.rad2arcsec * (self.wavelength / val)
@@ -615,7 +617,8 @@ class Atmosphere:
         if self.hasNotBeenInitialized is False:
             if len(val) != self.nLayer:
                 raise OopaoError('Wrong value for the wind-speed! Make sure that you inpute a wind-speed for each layer')
-            self.V0 = (np.sum(np.asarray(self.fractionalR0) * np.asarray(self.windSpeed) ** (5 / 3))) ** (3 / 5)
+            # the setter's own V0 expression (Atmosphere.py:1067), not the constructor's (197)
+            self.V0 = (np.sum(np.asarray(self.fractionalR0) * np.asarray(self.windSpeed)) ** (5 / 3)) ** (3 / 5)
             self.tau0 = 0.31 * self.r0 / self.V0
             for i in range(self.nLayer):
                 lay = getattr(self, 'layer_' + str(i + 1))
@@ -636,6 +639,68 @@ class Atmosphere:
                 lay = getattr(self, 'layer_' + str(i + 1))
                 lay.direction = val[i]
                 self._update_velocity(lay)
+
+    @property
+    def L0(self):
+        return self._L0
+
+    @L0.setter
+    def L0(self, val):
+        """L0 setter (Atmosphere.py:1042-1052): rebuilds the layers (covariances, screens) for the new
+        outer scale."""
+        self._L0 = val
+        if self.hasNotBeenInitialized is False:
+            print('Updating the Atmosphere covariance matrices...')
+            self.hasNotBeenInitialized = True
+            self.initializeAtmosphere(self.telescope)
+
+    @property
+    def fractionalR0(self):
+        return self._fractionalR0
+
+    @fractionalR0.setter
+    def fractionalR0(self, val):
+        """fractionalR0 setter (Atmosphere.py:1122-1133): V0 and tau0 (the layer weights of the OPD sum
+        read this list), then the Rytov variance."""
+        self._fractionalR0 = val
+        if self.hasNotBeenInitialized is False:
+            if len(val) != self.nLayer:
+                raise OopaoError('Wrong value for the fractional r0 ! Make sure that you inpute a fractional r0 for '
+                                 'each layer! If you want to change the number of layer, re-generate a new atmosphere '
+                                 'object.')
+            print('Updating the fractional R0...BEWARE COMPLETE THE RECOMPUTATION...NOT ONLY V0 and Tau0 !')
+            self.V0 = (np.sum(np.asarray(self.fractionalR0) * np.asarray(self.windSpeed)) ** (5 / 3)) ** (3 / 5)
+            self.tau0 = 0.31 * self.r0 / self.V0
+        self.update_rytov_variance()
+
+    @property
+    def elevation(self):
+        return self._elevation
+
+    @elevation.setter
+    def elevation(self, val):
+        """elevation setter (Atmosphere.py:1139-1161): clamps to 5 deg, moves the layers to the slant
+        distances altitude_zenith / sin(el), scales r0 from its zenith value and rebuilds the layers."""
+        self.sampling_checked = True
+        val = float(val)
+        if val < 5.0:
+            warning("Very low elevation (< 5 deg). Clamping to 5 deg.")
+            val = 5.0
+        if hasattr(self, '_elevation') and val == self._elevation:
+            return
+        self._elevation = val
+        el = np.radians(val)
+        print(f"--- Atmosphere Configuration : Elev={val:.1f}\u00b0 ---")
+        self.altitude = [h_z / np.sin(el) for h_z in self.altitude_zenith]
+        self._r0 = self.r0_zenith * np.sin(el) ** (3 / 5)
+        print(f" -> New r0 : {self._r0:.4f} m (Zenith reference: {self.r0_zenith:.4f} m)")
+        if self.hasNotBeenInitialized is False:
+            self.hasNotBeenInitialized = True
+            if getattr(self, 'telescope', None) is None:
+                raise OopaoError("Warning: Atmosphere not yet linked to a telescope. Call initializeAtmosphere() "
+                                 "manually.")
+            self.initializeAtmosphere(self.telescope, compute_covariance=getattr(self, 'compute_covariance', True))
+        self.update_rytov_variance()
 
     def _update_velocity(self, lay):
         lay.vY = lay.windSpeed * np.cos(np.deg2rad(lay.direction))

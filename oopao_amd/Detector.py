@@ -51,6 +51,8 @@ class Detector:
         self.pixel_size_rad = None
         self.pixel_size_arcsec = None
         self.quantification_noise = 0
+        self.clip_unsigned = -np.inf
+        self._integer_output = False
         self.photon_noise = 0
         self.dark_shot_noise = 0
         # time-seeded generators (Detector.py:137-145); a seed can be set for reproducible noise
@@ -62,7 +64,10 @@ class Detector:
 
     @property
     def frame(self):
-        return to_host(self._frame)
+        f = to_host(self._frame)
+        if f is not None and getattr(self, '_integer_output', False):
+            f = f.astype(self.output_precision)
+        return f
 
     @frame.setter
     def frame(self, val):
@@ -103,21 +108,66 @@ class Detector:
             self.readout()
 
     def readout(self):
+        """Detector.readout (Detector.py:273-329) on the device: dark-current shot noise, full-well
+        saturation, EMCCD amplification, hardware binning (applied twice, as the reference does),
+        readout noise, CCD/CMOS gain, ADC quantisation, log scale, output type, background removal."""
         f = self.buffer_frame[0]
         for g in self.buffer_frame[1:]:
             f = f + g
+        if self.darkCurrent != 0:
+            if self.integrationTime is None:
+                raise OopaoError('dark current needs the detector integrationTime')
+            lam = float(self.darkCurrent * self.integrationTime)
+            self.dark_shot_noise = np.sqrt(lam)
+            f = f + torch.poisson(torch.full_like(f, lam), generator=self._gen)
         if self.FWC is not None:
+            self.saturation = float(100 * f.max() / self.FWC)
+            if self.saturation > 100:
+                warning('The detector is saturating, %.1f %%' % self.saturation)
             f = torch.clamp(f, 0, self.FWC)
+        if self.sensor == 'EMCCD':
+            f = torch.poisson(torch.clamp(f, min=0), generator=self._gen) * self.gain
+        if self.binning != 1:
+            for _ in range(2):                   # Detector.py:288-294 bins twice
+                b = self.binning
+                f = f.reshape(*f.shape[:-2], f.shape[-2] // b, b, f.shape[-1] // b, b).sum(dim=(-3, -1))
+        self._set_output_precision()
         if self.readoutNoise != 0:
             noise = torch.randn(f.shape, generator=self._gen, device=self.device) * self.readoutNoise
             f = f + torch.round(noise)
         if self.sensor in ('CCD', 'CMOS') and self.gain != 1:
             f = f * self.gain
+        if self.bits is not None:
+            f = self._digitalization(f)
+        if self.log_scale:
+            f = torch.log10(f)
+        if self._integer_output:
+            f = torch.trunc(f)                   # float -> unsigned integer cast of the reference
         if self.backgroundMap is not None:
             f = f - torch.as_tensor(np.asarray(self.backgroundMap), dtype=torch.float32, device=self.device)
         self._frame = f
         self.buffer_frame = []
         self._integrated_time = 0
+
+    def _set_output_precision(self):
+        """Detector.set_output_precision (Detector.py:192-213): unsigned output for 8/16/32/64 bits."""
+        if self.output_precision is None:
+            types = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
+            self.output_precision = types.get(self.bits, float)
+            self.clip_unsigned = 0 if self.bits in types else -np.inf
+        self._integer_output = np.issubdtype(np.dtype(self.output_precision), np.integer)
+
+    def _digitalization(self, f):
+        """Detector.digitalization (Detector.py:226-238)."""
+        if self.FWC is None:
+            return f / f.max() * 2 ** self.bits
+        self.quantification_noise = self.FWC * 2 ** (-self.bits) / np.sqrt(12)
+        self.saturation = float(100 * f.max() / self.FWC)
+        if self.saturation > 100:
+            warning('The ADC is saturating (gain applyed %i), %.1f %%' % (self.gain, self.saturation))
+        f = f / self.FWC * (2 ** self.bits - 1)
+        lo = max(float(f.min()), self.clip_unsigned)
+        return torch.clamp(f, lo, 2 ** self.bits - 1)
 
     def print_properties(self):
         print(self)

@@ -729,6 +729,24 @@ class Pyramid:
         else:
             raise OopaoError('Error light propagated to the wrong type of object')
 
+    def get_modulation_frame(self, radius=6, norma=True):
+        """Pyramid.get_modulation_frame (Pyramid.py:928-944): the focal-plane camera's frame cropped to
+        +-radius lambda/D about its centre, optionally normalised to its maximum."""
+        if radius <= 0:
+            warning('radius for the field of view must be a strictly positive number. Ignoring the input value.')
+            radius = self.telescope.resolution // 2
+        self.modulation_camera_frame = np.asarray(self.focal_plane_camera.frame, dtype=float)
+        n_trunc = int(self.resolution / 2 - radius * self.zeroPaddingFactor)
+        if n_trunc <= 0:
+            warning('radius Value is too high as the field of view is limited to ' +
+                    str(int(self.fov_l_d / 2)) + ' lambda/D -- ignoring')
+            zoom = self.modulation_camera_frame.copy()
+        else:
+            zoom = self.modulation_camera_frame[n_trunc:-n_trunc, n_trunc:-n_trunc]
+        if norma:
+            zoom /= zoom.max()
+        return zoom
+
     def _raw_dev(self, args, weighted):
         opd, wavelength, amp = args
         return self.engine.raw(opd[-1:].contiguous(), 2 * np.pi / wavelength, amp, weighted=weighted)[0]

@@ -288,6 +288,31 @@ class Source:
         self.__updating_flux = False
 
     # ------------------------------------------------------------------ print
+    # ------------------------------------------------- scintillation diagnostics (host, Source.py:460-521)
+    @property
+    def rytov_variance(self):
+        """Measured Rytov variance of the scintillation over the telescope pupil (Source.py:460-466)."""
+        from .tools import compute_rytov_variance
+        return compute_rytov_variance(self.scintillation, self.tel.pupil)
+
+    def get_dsp(self, telescope):
+        """Radial phase and log-amplitude power spectra of the unmasked maps (Source.py:468-482)."""
+        from .tools import compute_dsp
+        return compute_dsp(self.phase_no_pupil, self.scintillation_no_pupil, telescope.resolution,
+                           telescope.pixelSize)
+
+    def get_theoretical_pupil_variance(self, D_tel, L_source, L0, Cn2_input):
+        from .tools import compute_theoretical_pupil_log_amp_variance
+        return compute_theoretical_pupil_log_amp_variance(self.wavelength, D_tel, L_source, L0, Cn2_input)
+
+    def get_theoretical_rytov(self, atm):
+        from .tools import compute_theoretical_rytov
+        return compute_theoretical_rytov(self.wavelength, atm.r0, atm.altitude, atm.fractionalR0)
+
+    def get_theoretical_dsp(self, kappa, r0, L0, altitude):
+        from .tools import compute_theoretical_psd
+        return compute_theoretical_psd(kappa, self.wavelength, r0, L0, altitude)
+
     def print_properties(self):
         print(self)
 

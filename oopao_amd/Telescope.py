@@ -156,6 +156,25 @@ class Telescope:
         self.yPSF_arcsec = list(self.xPSF_arcsec)
         self.PSF_list = self._PSF
 
+    def PropagateField(self, amplitude, phase, zeroPaddingFactor, img_resolution=None):
+        """Telescope.PropagateField (Telescope.py:362-423) without coronagraph: PSF = the centred
+        img_resolution window of |fft2(pad(amplitude exp(i phase)) phasor) / N|^2, binned by the
+        integer oversampling used when zeroPaddingFactor < 2 (returned). Computed on the GPU."""
+        from .science import psf_dev, psf_geometry
+        if self.coronagraph_diameter is not None:
+            raise OopaoError('coronagraphic PSFs are not computed by oopao_amd')
+        n = self.pupil.shape[0]
+        over = psf_geometry(n, zeroPaddingFactor, img_resolution)[0]
+        amp = torch.as_tensor(np.asarray(amplitude, dtype=np.float32), device=self.device)
+        ph = torch.as_tensor(np.asarray(phase, dtype=np.float32), device=self.device)
+        if amp.dim() == 0:
+            amp = amp.expand(n, n)
+        if tuple(amp.shape) != (n, n) or tuple(ph.shape) != (n, n):
+            raise OopaoError('amplitude and phase must be %d x %d arrays' % (n, n))
+        self._PSF = psf_dev(amp.contiguous(), ph.contiguous(), 1.0, n, zeroPaddingFactor, img_resolution,
+                            self.device)
+        return over
+
     @property
     def PSF(self):
         p = getattr(self, '_PSF', None)

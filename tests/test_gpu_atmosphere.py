@@ -71,3 +71,36 @@ def test_paired_telescope_propagates_atmosphere():
     assert rel_l2(tel.OPD, ref) < 1e-7
     ngs ** atm * tel
     assert rel_l2(ngs.OPD, ref) < 1e-7
+
+
+def test_property_setters():
+    # elevation / L0 / fractionalR0 / windSpeed setters (Atmosphere.py:1042-1161)
+    tel, ngs, atm = _atm(32, 0.01, [10, 20], [0, 90], 3)
+    r0_zenith = atm.r0_zenith
+    assert abs(r0_zenith - 0.15) < 1e-15                      # elevation 90: slant = zenith
+    # elevation 30 deg: slant distances h_z / sin(el), r0 = r0_zenith sin(el)^(3/5), layers rebuilt
+    atm.elevation = 30
+    el = np.radians(30.0)
+    assert np.allclose(atm.altitude, [0.0, 10000 / np.sin(el)], rtol=1e-14)
+    assert abs(atm.r0 - r0_zenith * np.sin(el) ** 0.6) < 1e-15
+    assert atm.layer_2.altitude == atm.altitude[1]
+    assert np.isfinite(atm.OPD).all() and atm.OPD.std() > 0
+    atm.elevation = 2                                        # clamped to 5 deg
+    assert atm.elevation == 5.0
+    # L0: the layers are rebuilt with covariances of the new outer scale
+    xxt_25 = atm.layer_1.XXt.copy()
+    atm.L0 = 10
+    assert atm.L0 == 10
+    assert not np.allclose(atm.layer_1.XXt, xxt_25)
+    from oopao_amd.phaseStats import makeCovarianceMatrix
+    lay = atm.layer_1
+    assert np.allclose(atm.layer_1.XXt, makeCovarianceMatrix(lay.outerZ, lay.outerZ, 10, atm.r0_def), rtol=1e-12)
+    # fractionalR0 / windSpeed: the setters' V0 = sum(fr * v) (Atmosphere.py:1067, 1131)
+    atm.fractionalR0 = [0.5, 0.5]
+    assert abs(atm.V0 - (0.5 * 10 + 0.5 * 20)) < 1e-12
+    atm.windSpeed = [5, 15]
+    assert abs(atm.V0 - (0.5 * 5 + 0.5 * 15)) < 1e-12
+    assert abs(atm.tau0 - 0.31 * atm.r0 / atm.V0) < 1e-15
+    from oopao_amd.tools import OopaoError
+    with pytest.raises(OopaoError):
+        atm.fractionalR0 = [1.0]

@@ -99,3 +99,26 @@ def test_focal_plane_camera_matches_oracle():
     err = rel_l2(fr, ref)
     print('focal camera rel_l2 vs fp64 oracle', err)
     assert err <= 1e-5
+
+
+def test_get_modulation_frame_crop_and_norm():
+    # Pyramid.get_modulation_frame (Pyramid.py:928-944) on a full N x N focal-camera frame: crop of
+    # +-radius lambda/D (radius * zeroPaddingFactor pixels) about the centre, normalised to its maximum
+    tel, ngs, dm, wfs = _config1_objects()
+    o = config1_system()
+    N = wfs.resolution
+    wfs.focal_plane_camera.resolution = N
+    ngs ** tel * dm * wfs
+    wfs * wfs.focal_plane_camera
+    ref = o.focal(np.zeros((80, 80)), N)
+    for radius in (6, 10):
+        zoom = wfs.get_modulation_frame(radius=radius)
+        t = int(N / 2 - radius * wfs.zeroPaddingFactor)
+        r = ref[t:-t, t:-t] / ref[t:-t, t:-t].max()
+        assert zoom.shape == (N - 2 * t, N - 2 * t)
+        assert zoom.max() == 1.0
+        assert rel_l2(zoom, r) <= 1e-5
+    # radius beyond the field of view: the whole frame, unnormalised on request
+    full = wfs.get_modulation_frame(radius=N, norma=False)
+    assert full.shape == (N, N)
+    assert rel_l2(full, wfs.focal_plane_camera.frame) == 0.0

@@ -58,3 +58,17 @@ def test_detector_relay_and_strehl():
     sr_ref = osc.strehl(ref_img, ref_flat)
     print('Strehl', sr, 'oracle', sr_ref)
     assert abs(sr - sr_ref) < 1e-4
+
+
+@pytest.mark.parametrize('zpf,img', [(2, None), (3, 120), (1, None)])
+def test_propagate_field_matches_oracle(zpf, img):
+    # Telescope.PropagateField (Telescope.py:362-423) called directly with an amplitude and a phase
+    tel, ngs = _tel_ngs()
+    pup = tel.pupil.astype(float)
+    amp = pup * (1.0 + 0.2 * np.random.RandomState(4).rand(80, 80))
+    ph = random_phase(80, 33, 0.8, pup)
+    over = tel.PropagateField(amp, ph, zpf, img_resolution=img)
+    assert over == (2 if zpf == 1 else 1)
+    ref = osc.propagate_field(amp, ph, zpf, img)
+    assert tel.PSF.shape == ref.shape
+    assert rel_l2(tel.PSF, ref) <= 1e-5
