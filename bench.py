@@ -91,14 +91,32 @@ def closed_loop_leg(tel, ngs, wfs, dev, n_iter=1000, n_modes=1000):
     t0 = time.perf_counter()
     out = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=n_iter, strehl_every=10)
     dt = time.perf_counter() - t0
+    # the same loop (same screens, same start) with the reconstructor held in BF16 on the matrix-core
+    # GEMV path; its residual trace is compared with the float32 loop's
+    atm.generateNewPhaseScreen(10)
+    dm.coefs = np.zeros(dm.nValidAct)
+    run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=5, reconstructor_precision='bf16')
+    atm.generateNewPhaseScreen(10)
+    dm.coefs = np.zeros(dm.nValidAct)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out16 = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=n_iter, strehl_every=10,
+                            reconstructor_precision='bf16')
+    dt16 = time.perf_counter() - t0
     half = n_iter // 2
+    r32, r16 = out['wfe_residual'], out16['wfe_residual']
+    bf16 = {'value': round(n_iter / dt16, 2), 'unit': 'iterations/s',
+            'wfe_residual_nm': round(float(np.mean(r16[half:])), 1),
+            'strehl_pct_mean_second_half': round(float(np.mean(out16['strehl'][half // 10:])), 2),
+            'residual_trace_max_rel_diff_vs_fp32': float(np.max(np.abs(r16 - r32) / r32))}
     return {'value': round(n_iter / dt, 2), 'unit': 'iterations/s', 'iterations': n_iter, 'nAct': int(dm.nValidAct),
             'nSignal': int(wfs.nSignal), 'modes': n_modes, 'dm_route': 'dense' if dm.use_dense else 'separable',
             'kl_seconds': round(t_kl, 2), 'im_seconds': round(t_im, 2), 'im_cond': round(float(calib.cond), 1),
             'wfe_atmosphere_nm': round(float(np.mean(out['wfe_atmosphere'][half:])), 1),
             'wfe_residual_nm': round(float(np.mean(out['wfe_residual'][half:])), 1),
             'strehl_pct_mean_second_half': round(float(np.mean(out['strehl'][half // 10:])), 2),
-            'note': 'wall clock of the loop incl. atmosphere update (host RNG draws) and Strehl every 10 it.'}
+            'note': 'wall clock of the loop incl. atmosphere update (host RNG draws) and Strehl every 10 it.',
+            'bf16_reconstructor': bf16}
 
 
 def im_leg(dev, n_modes=1000):

@@ -34,6 +34,8 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
     if nAct != dm.nValidAct or nSig != wfs.nSignal:
         raise ValueError('reconstructor shape %s does not match (nAct=%d, nSignal=%d)'
                          % (tuple(R.shape), dm.nValidAct, wfs.nSignal))
+    if frame_delay not in (1, 2):
+        raise ValueError('frame_delay must be 1 or 2, got %r' % (frame_delay,))
     if reconstructor_precision not in ('fp32', 'bf16'):
         raise ValueError("reconstructor_precision must be 'fp32' or 'bf16', got %r" % (reconstructor_precision,))
     recon_bf16 = None
@@ -48,7 +50,6 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
     resid_ws = torch.empty(nb.value, dtype=torch.uint8, device=dev)
     opd = torch.empty((tel.resolution, tel.resolution), dtype=torch.float32, device=dev)
     coefs = torch.as_tensor(np.asarray(dm.coefs, np.float32), device=dev).clone()
-    sig_buf = torch.zeros(nSig, dtype=torch.float32, device=dev)
     new_coefs = torch.empty_like(coefs)
     wfe_atm = torch.zeros(n_iter, dtype=torch.float64, device=dev)
     wfe_res = torch.zeros(n_iter, dtype=torch.float64, device=dev)
@@ -69,7 +70,10 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
         sr = torch.zeros(n_sr, dtype=torch.float64, device=dev)
     frame = torch.empty((1, eng.cam, eng.cam), dtype=torch.float32, device=dev)
     s2d = torch.empty((1, 2 * se.w, se.w), dtype=torch.float32, device=dev)
-    s = torch.empty((1, se.nsig), dtype=torch.float32, device=dev)
+    # slope vectors of this and of the previous iteration (swapped, no copy): the 2-frame delay applies
+    # the previous one
+    s_cur = torch.zeros((1, se.nsig), dtype=torch.float32, device=dev)
+    s_prev = torch.zeros((1, se.nsig), dtype=torch.float32, device=dev)
     for i in range(n_iter):
         atm.update()
         turb = atm._OPD_dev
@@ -82,20 +86,19 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
             sr[i // strehl_every] = strehl_dev(psf, psf_ref, tel.resolution, psf_sampling, as_tensor=True)
         eng.rows(opd[None], scale, amp, buf)
         eng.cols(buf, frame)
-        se.slopes(frame, s2d, s)
+        se.slopes(frame, s2d, s_cur)
         if record_slopes:
-            slopes[i] = s[0]
-        if frame_delay == 1:
-            sig_buf.copy_(s[0])
-        # coefs <- coefs - gain * R @ sig_buf   (R [nAct, nSig] row-major GEMV, float32 or BF16 matrix)
+            slopes[i] = s_cur[0]
+        sig = s_cur[0] if frame_delay == 1 else s_prev[0]
+        # coefs <- coefs - gain * R @ sig   (R [nAct, nSig] row-major GEMV, float32 or BF16 matrix)
         if recon_bf16 is not None:
-            recon_bf16.gemv(sig_buf, y0=coefs, alpha=-gain, beta=1.0, out=new_coefs)
+            recon_bf16.gemv(sig, y0=coefs, alpha=-gain, beta=1.0, out=new_coefs)
         else:
-            _lib.call('oopao_gemv', ptr(R), ptr(sig_buf), ptr(new_coefs), ptr(coefs), nAct, nSig, float(-gain), 1.0,
+            _lib.call('oopao_gemv', ptr(R), ptr(sig), ptr(new_coefs), ptr(coefs), nAct, nSig, float(-gain), 1.0,
                       stream_ptr(dev))
         coefs, new_coefs = new_coefs, coefs
         if frame_delay == 2:
-            sig_buf.copy_(s[0])
+            s_cur, s_prev = s_prev, s_cur
     torch.cuda.synchronize(dev)
     dm.coefs = coefs.double().cpu().numpy()
     out = dict(wfe_atmosphere=wfe_atm.cpu().numpy(), wfe_residual=wfe_res.cpu().numpy(),
