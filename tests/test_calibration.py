@@ -31,6 +31,35 @@ def test_calibration_vault_pinv_identities():
     assert np.isclose(cv.cond, cv.eigenValues[0] / cv.eigenValues[-6])
 
 
+@pytest.mark.parametrize('shape', [(300, 500), (700, 300)])
+def test_calibration_vault_gram_route(shape):
+    # the Gram route ('auto' for min side >= 256 and cond <= 1e4): singular values, pseudo-inverse and
+    # factor orthonormality against numpy's float64 SVD at cond 1e3 (errors ~eps cond^2 on M)
+    from oopao_amd.calibration.CalibrationVault import CalibrationVault
+    m, k = shape
+    r = min(m, k)
+    rs = np.random.RandomState(1)
+    Ua = np.linalg.qr(rs.randn(m, r))[0]
+    Va = np.linalg.qr(rs.randn(k, r))[0]
+    D = Ua @ np.diag(np.logspace(0, -3, r)) @ Va.T
+    cv = CalibrationVault(D, device='cpu')
+    s = np.linalg.svd(D, compute_uv=False)
+    assert np.allclose(cv.eigenValues, s, rtol=1e-9)
+    pinv = np.linalg.pinv(D)
+    assert np.abs(cv.M - pinv).max() <= 1e-8 * np.abs(pinv).max()
+    assert np.allclose(cv.U @ cv.U.T, np.eye(r), atol=1e-9)
+    assert np.allclose(cv.V @ cv.V.T, np.eye(r), atol=1e-6)
+    assert np.allclose(cv.D, D, atol=1e-12)
+    assert np.isclose(cv.cond, 1e3, rtol=1e-8)
+    # an ill-conditioned matrix takes the SVD route (same results as method='svd')
+    D2 = Ua @ np.diag(np.logspace(0, -6, r)) @ Va.T
+    a = CalibrationVault(D2, device='cpu')
+    b = CalibrationVault(D2, device='cpu', method='svd')
+    assert np.allclose(a.eigenValues, b.eigenValues, rtol=1e-12)
+    with pytest.raises(ValueError):
+        CalibrationVault(D2, device='cpu', method='gram')
+
+
 def _small_loop_system(seed=10):
     from oopao_amd import Telescope, Source, DeformableMirror, Pyramid, Atmosphere
     tel = Telescope(32, 8, 0.005)
