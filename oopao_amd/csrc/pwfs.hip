@@ -90,12 +90,12 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(u[m], txi[(t + TPT * m) >> 2]);
-    fft_fwd_pruned<N, TPT, 4>(v, xl, tw, t);
+    fft_fwd_pruned<N, TPT, 4, true>(v, xl, tw, t);
     if constexpr (!FWD_ONLY) {
       const float2* mxp = launder(mx);
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = cmul(v[m], mxp[t + TPT * m]);
-      fft_full<N, TPT, +1>(v, xl, tw, t);
+      fft_full<N, TPT, +1, true>(v, xl, tw, t);
     }
     if (active) {
       const int k0 = min(max(pstart[i], 0), T);
@@ -161,8 +161,8 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
       float2 v[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + c];
-      // XOR-swizzled LDS exchange (conflict-free small-span stages): measured
-      // faster in this pass, slower in the rows pass (extra ds instructions).
+      // XOR-swizzled LDS exchange (conflict-free small-span stages), as in the
+      // rows pass.
       fft_fwd_pruned<N, TPT, 4, true>(v, xl, tw, t);
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = cmul(v[m], launder(my)[t + TPT * m]);
