@@ -119,6 +119,94 @@ def closed_loop_leg(tel, ngs, wfs, dev, n_iter=1000, n_modes=1000):
             'bf16_reconstructor': bf16}
 
 
+def _wall_ms(fn, reps=10):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps * 1e3
+
+
+def extended_leg(dev):
+    """Wall-clock timings of the sensors and propagation modes beyond the Pyramid frame, at the config-1
+    pupil (80 px, 8 m): the Bi-O Edge WFS (OOPAO's tutorial sensor: 20 x 20 subapertures, unmodulated,
+    grey width 3, and a modulated one: 3 lambda/D, 48 points), the gain-sensing camera (calibration and one
+    optical-gain estimate on 100 DM modes through the config-1 PWFS's full focal camera), an asterism of 5
+    sources (two off-axis NGS, two sodium LGS, one on axis) through a 2-layer atmosphere with a 60" field,
+    and the same atmosphere with angular-spectrum propagation (scintillation) for 3 sources."""
+    import torch
+    from oopao_amd import Telescope, Source, Atmosphere, Asterism, DeformableMirror, Pyramid
+    from oopao_amd.BioEdge import BioEdge
+    from oopao_amd.GainSensingCamera import GainSensingCamera
+    out = {}
+    tel = Telescope(80, 8.0, 1e-3)
+    ngs = Source('I3', 8, display_properties=False)
+    ngs * tel
+    opd = np.random.RandomState(3).randn(80, 80) * 30e-9
+    tel.OPD_no_pupil = opd
+    for name, kw in (('bioedge_tutorial', dict(modulation=0, grey_width=3, lightRatio=0.01, n_pix_separation=0,
+                                               postProcessing='fullFrame_incidence_flux')),
+                     ('bioedge_modulated', dict(modulation=3, grey_width=1, lightRatio=0.1, n_pix_separation=20,
+                                                nTheta_user_defined=48, postProcessing='slopesMaps'))):
+        wfs = BioEdge(nSubap=20, telescope=tel, **kw)
+
+        def relay(wfs=wfs):
+            tel.OPD_no_pupil = opd
+            ngs * tel * wfs
+        out[name] = {'ms_per_frame': round(_wall_ms(relay), 3), 'N': int(wfs.resolution),
+                     'nTheta': int(wfs.nTheta), 'nSignal': int(wfs.nSignal)}
+    dm = DeformableMirror(tel, 20, 0.35, print_dm_properties=False)
+    pwfs = Pyramid(20, tel, 3, 0.5, n_pix_separation=20, n_pix_edge=10, nTheta_user_defined=48)
+    N = pwfs.nRes
+    pwfs.focal_plane_camera.resolution = N
+    basis = dm.modes.reshape(80, 80, dm.nValidAct)[:, :, :100] * tel.pupil[:, :, None]
+    t0 = time.perf_counter()
+    gsc = GainSensingCamera(pwfs.mask, basis)
+    tel.resetOPD()
+    ngs * tel * pwfs
+    pwfs * pwfs.focal_plane_camera
+    pwfs.focal_plane_camera * gsc
+    torch.cuda.synchronize()
+    t_cal = time.perf_counter() - t0
+
+    def og():
+        tel.OPD_no_pupil = opd
+        ngs * tel * pwfs
+        pwfs * pwfs.focal_plane_camera
+        pwfs.focal_plane_camera * gsc
+    out['gain_sensing_camera'] = {'calibration_ms': round(t_cal * 1e3, 2),
+                                  'ms_per_estimate_incl_frame_and_focal_camera': round(_wall_ms(og), 3),
+                                  'modes': 100, 'N': int(N)}
+    telf = Telescope(80, 8.0, 1e-3, fov=60)
+    ngs2 = Source('I3', 8, display_properties=False)
+    ngs2 * telf
+    for name, kw, srcs in (
+            ('asterism_offaxis_lgs', {}, [Source('I3', 8, coordinates=[0, 0], display_properties=False),
+                                          Source('I3', 8, coordinates=[20, 35], display_properties=False),
+                                          Source('I3', 8, coordinates=[25, 215], display_properties=False),
+                                          Source('Na', 8, coordinates=[0, 0], altitude=90e3, display_properties=False),
+                                          Source('Na', 8, coordinates=[15, 200], altitude=90e3,
+                                                 display_properties=False)]),
+            ('scintillation_asm', {'angular_spectrum_propagation': True},
+             [Source('I3', 8, coordinates=[0, 0], display_properties=False),
+              Source('I3', 8, coordinates=[8, 60], display_properties=False),
+              Source('Na', 8, coordinates=[0, 0], altitude=90e3, display_properties=False)])):
+        atm = Atmosphere(telf, 0.15, 25, windSpeed=[10, 20], fractionalR0=[0.7, 0.3], windDirection=[0, 90],
+                         altitude=[0, 10000], **kw)
+        atm.initializeAtmosphere(telf)
+        atm.generateNewPhaseScreen(12)
+        ast = Asterism(srcs)
+
+        def step(atm=atm, ast=ast):
+            atm.update()
+            ast ** atm * telf
+        out[name] = {'ms_per_update_and_relay': round(_wall_ms(step), 3), 'sources': len(srcs), 'n': 80}
+    return out
+
+
 def im_leg(dev, n_modes=1000):
     """Interaction-matrix modes/s: push-pull of n_modes seeded random DM modes (+-10 nm) through the
     config-1 PWFS (80 px pupil, 20x20 subapertures, 48 points), batched, FP64 SVD included."""
@@ -390,7 +478,7 @@ def main():
     ap.add_argument('--batch', type=int, default=8, help='screens per call for the batched form')
     ap.add_argument('--no-extra', action='store_true',
                     help='skip the closed-loop, interaction-matrix, KL-basis, adjoint and contraction legs')
-    ap.add_argument('--legs', default='closed_loop,interaction_matrix,im_batched,kl_basis,adjoint,contractions',
+    ap.add_argument('--legs', default='closed_loop,interaction_matrix,im_batched,kl_basis,adjoint,contractions,extended',
                     help='comma-separated extra legs to run')
     ap.add_argument('--calib-copy', action='store_true',
                     help='also run one elementwise copy of 256 MiB (counter calibration for rocprof --pmc)')
@@ -501,6 +589,8 @@ def main():
             extra['adjoint'] = adjoint_leg()
         if 'contractions' in legs:
             extra['contractions'] = contraction_leg(tel, dev)
+        if 'extended' in legs:
+            extra['extended'] = extended_leg(dev)
 
     out = {
         'metric': 'PWFS frames/s at 80x80 subaps, 72 mod pts',
