@@ -136,7 +136,8 @@ def extended_leg(dev):
     grey width 3, and a modulated one: 3 lambda/D, 48 points), the gain-sensing camera (calibration and one
     optical-gain estimate on 100 DM modes through the config-1 PWFS's full focal camera), an asterism of 5
     sources (two off-axis NGS, two sodium LGS, one on axis) through a 2-layer atmosphere with a 60" field,
-    and the same atmosphere with angular-spectrum propagation (scintillation) for 3 sources."""
+    and the same atmosphere with angular-spectrum propagation (scintillation) for 3 sources; and an ELT-scale
+    PWFS frame."""
     import torch
     from oopao_amd import Telescope, Source, Atmosphere, Asterism, DeformableMirror, Pyramid
     from oopao_amd.BioEdge import BioEdge
@@ -204,6 +205,21 @@ def extended_leg(dev):
             atm.update()
             ast ** atm * telf
         out[name] = {'ms_per_update_and_relay': round(_wall_ms(step), 3), 'sources': len(srcs), 'n': 80}
+    # ELT-scale PWFS: 39 m pupil, 30 % obstruction, 480 px, 120 x 120 subapertures at 4 px (the 100 x 100 at
+    # 4.8 px variant is not constructible, Pyramid.py:216-217), N = 1920, 96 points; N = 1920 runs on the
+    # general kernels (the register-resident transforms do not fit its radix plan)
+    elt = Telescope(480, 39.0, 1e-3, centralObstruction=0.3)
+    ngs3 = Source('I3', 8, display_properties=False)
+    ngs3 * elt
+    wfs_elt = Pyramid(120, elt, 3, 0.5, n_pix_separation=120, n_pix_edge=60, nTheta_user_defined=96)
+    opd_elt = np.random.RandomState(5).randn(480, 480) * 50e-9
+
+    def elt_frame():
+        elt.OPD_no_pupil = opd_elt
+        ngs3 * elt * wfs_elt
+    out['elt_pwfs_120x120'] = {'ms_per_frame': round(_wall_ms(elt_frame, reps=5), 2), 'N': int(wfs_elt.nRes),
+                               'nTheta': int(wfs_elt.nTheta), 'nSignal': int(wfs_elt.nSignal),
+                               'path': wfs_elt.engine.mode}
     return out
 
 
