@@ -220,6 +220,19 @@ def extended_leg(dev):
     out['elt_pwfs_120x120'] = {'ms_per_frame': round(_wall_ms(elt_frame, reps=5), 2), 'N': int(wfs_elt.nRes),
                                'nTheta': int(wfs_elt.nTheta), 'nSignal': int(wfs_elt.nSignal),
                                'path': wfs_elt.engine.mode}
+    # the same telescope sampled at 512 px with 128 x 128 subapertures (N = 2048, register-resident path)
+    elt2 = Telescope(512, 39.0, 1e-3, centralObstruction=0.3)
+    ngs4 = Source('I3', 8, display_properties=False)
+    ngs4 * elt2
+    wfs_elt2 = Pyramid(128, elt2, 3, 0.5, n_pix_separation=128, n_pix_edge=64, nTheta_user_defined=96)
+    opd_elt2 = np.random.RandomState(6).randn(512, 512) * 50e-9
+
+    def elt2_frame():
+        elt2.OPD_no_pupil = opd_elt2
+        ngs4 * elt2 * wfs_elt2
+    out['elt_pwfs_128x128'] = {'ms_per_frame': round(_wall_ms(elt2_frame, reps=10), 3), 'N': int(wfs_elt2.nRes),
+                               'nTheta': int(wfs_elt2.nTheta), 'nSignal': int(wfs_elt2.nSignal),
+                               'path': wfs_elt2.engine.mode}
     return out
 
 
