@@ -552,6 +552,36 @@ class Pyramid:
             return s2d[0], s[0]
         return s2d, s
 
+    @property
+    def norma(self):
+        """Normalisation of the last signal (the value signalProcessing stores, Pyramid.py:838-926),
+        computed from the current camera frame; one value per frame for a batched measurement."""
+        frame = self.cam._frame
+        if frame is None:
+            return None
+        f = (frame if frame.dim() == 3 else frame[None]).double()
+        pp = self.postProcessing
+        sub_flux = float(self.src.nPhoton * self.telescope.samplingTime * (self.telescope.D / self.nSubap) ** 2)
+        if pp == 'slopesMaps':
+            quad, w = self._quadrant_origins()
+            i4q = sum(f[:, quad[k]:quad[k] + w, quad[4 + k]:quad[4 + k] + w] for k in range(4))
+            v = torch.from_numpy(np.asarray(self.validI4Q, dtype=bool)).to(self.device)
+            out = i4q[:, v].mean(dim=1)
+        elif pp in ('slopesMaps_camera_flux', 'fullFrame_camera_flux'):
+            out = f.mean(dim=(1, 2))
+        elif pp == 'slopesMaps_incidence_flux':
+            out = torch.full((f.shape[0],), sub_flux, dtype=torch.float64)
+        elif pp == 'fullFrame_incidence_flux':
+            out = torch.full((f.shape[0],), sub_flux / 4, dtype=torch.float64)
+        elif pp == 'fullFrame_sum_flux':
+            out = f.sum(dim=(1, 2))
+        elif pp == 'fullFrame':
+            out = f[:, self._valid_full].sum(dim=1)
+        else:
+            raise OopaoError('unknown postProcessing ' + pp)
+        out = out.cpu().numpy()
+        return float(out[0]) if frame.dim() == 2 else out
+
     def signalProcessing(self, cameraFrame=None):
         """(signal_2D, signal) of the current camera frame (Pyramid.py:835-926)."""
         s2d, s = self._signal_processing_dev()

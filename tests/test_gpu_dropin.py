@@ -122,3 +122,44 @@ def test_get_modulation_frame_crop_and_norm():
     full = wfs.get_modulation_frame(radius=N, norma=False)
     assert full.shape == (N, N)
     assert rel_l2(full, wfs.focal_plane_camera.frame) == 0.0
+
+
+@pytest.mark.parametrize('mode', ['slopesMaps_incidence_flux', 'slopesMaps_camera_flux', 'fullFrame_camera_flux',
+                                  'fullFrame_incidence_flux', 'fullFrame_sum_flux', 'fullFrame'])
+def test_post_processing_modes(mode):
+    # signalProcessing's other modes (Pyramid.py:856-926) on the config-1 sensor: the signal of the
+    # calibration (flat) wavefront is zero, the modes agree with the reference's expressions evaluated on
+    # the camera frame, and the slopes modes share the raw quadrant differences of 'slopesMaps'
+    from oopao_amd import Pyramid
+    tel, ngs, dm, wfs0 = _config1_objects()
+    wfs = Pyramid(20, tel, 3, 0.5, n_pix_separation=20, n_pix_edge=10, nTheta_user_defined=48, postProcessing=mode)
+    tel.resetOPD()
+    ngs * tel * wfs
+    s_flat = np.asarray(wfs.signal)
+    assert np.abs(s_flat).max() <= 1e-6 * max(1.0, np.abs(wfs.referenceSignal_2D).max())
+    opd = np.random.RandomState(9).randn(80, 80) * 40e-9
+    tel.OPD_no_pupil = opd
+    ngs * tel * wfs
+    frame = wfs.cam.frame
+    ref2d = wfs.referenceSignal_2D
+    valid = np.asarray(wfs.validSignal).astype(bool)
+    sub_flux = ngs.nPhoton * tel.samplingTime * (tel.D / wfs.nSubap) ** 2
+    if mode.startswith('slopesMaps'):
+        I = [wfs.grabQuadrant(k, frame) * wfs.validI4Q for k in (1, 2, 3, 4)]
+        raw = np.concatenate((I[0] - I[1] + I[3] - I[2], I[0] - I[3] + I[1] - I[2]))
+        norma = sub_flux if mode == 'slopesMaps_incidence_flux' else frame.mean()
+        exp2d = (raw / norma - ref2d) * wfs.slopesUnits
+        # the same raw differences as the default mode, up to the normalisation
+        tel.OPD_no_pupil = opd
+        ngs * tel * wfs0
+        raw0 = (wfs0.signal_2D / wfs0.slopesUnits + wfs0.referenceSignal_2D) * wfs0.norma
+        I4Q = sum(I)
+        assert abs(wfs0.norma / I4Q[wfs.validI4Q].mean() - 1) < 1e-6
+        assert rel_l2(raw0, raw) <= 1e-5
+    else:
+        norma = {'fullFrame_camera_flux': frame.mean(), 'fullFrame_incidence_flux': sub_flux / 4,
+                 'fullFrame_sum_flux': frame.sum(), 'fullFrame': frame[valid].sum()}[mode]
+        exp2d = frame / norma - ref2d
+    assert rel_l2(wfs.signal_2D, exp2d) <= 1e-5
+    assert rel_l2(wfs.signal, exp2d[valid]) <= 1e-5
+    assert abs(wfs.norma / norma - 1) < 1e-6
