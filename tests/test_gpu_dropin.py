@@ -163,3 +163,36 @@ def test_post_processing_modes(mode):
     assert rel_l2(wfs.signal_2D, exp2d) <= 1e-5
     assert rel_l2(wfs.signal, exp2d[valid]) <= 1e-5
     assert abs(wfs.norma / norma - 1) < 1e-6
+
+
+@pytest.mark.parametrize('nTheta_user', [48, None])
+def test_modulation_and_light_ratio_setters(nTheta_user):
+    # modulation setter (Pyramid.py:1107-1154: new path, recalibrated reference) and lightRatio setter
+    # (re-threshold of the stored I4Q) against the oracle built with the new values
+    from oopao_amd import Pyramid
+    from oracle.pwfs import PyramidOracle, telescope_pupil, flux_map, n_photon
+    tel, ngs, dm, _ = _config1_objects()
+    wfs = Pyramid(20, tel, 3, 0.5, n_pix_separation=20, n_pix_edge=10, nTheta_user_defined=nTheta_user)
+    pup = telescope_pupil(80)
+    flux = flux_map(pup, n_photon('I3', 8), 1e-3, 8.0)
+    opd = np.random.RandomState(11).randn(80, 80) * 30e-9
+    o = None
+    for mod, lr in ((5, 0.5), (5, 0.2), (2, 0.2)):
+        if mod != wfs.modulation:
+            wfs.modulation = mod
+            o = PyramidOracle(80, 20, mod, lr, pup, flux, 0.8e-6, n_pix_separation=20, n_pix_edge=10,
+                              nTheta_user_defined=nTheta_user)
+        if lr != wfs.lightRatio:
+            # the setter re-thresholds the stored I4Q and keeps the reference slopes of the last calibration
+            wfs.lightRatio = lr
+            o.validI4Q = o.I4Q >= lr * o.I4Q.max()
+            o.validSignal = np.concatenate((o.validI4Q, o.validI4Q))
+            o.nSignal = int(o.validSignal.sum())
+        assert wfs.nTheta == len(o.path)
+        assert np.array_equal(np.asarray(wfs.validI4Q, bool), o.validI4Q)
+        assert wfs.nSignal == o.nSignal
+        tel.OPD_no_pupil = opd
+        ngs * tel * wfs
+        err = rel_l2(wfs.signal, o.measure_opd(opd)[1])
+        print('modulation', mod, 'lightRatio', lr, 'nTheta', wfs.nTheta, 'rel_l2', err)
+        assert err <= 1e-5
