@@ -64,6 +64,42 @@ class Telescope:
         self.pupil = pupil
         self.pupilReflectivity = (self.pupil * refl).astype(np.float64)
 
+    def apply_spiders(self, angle, thickness_spider, offset_X=None, offset_Y=None):
+        """Telescope.apply_spiders (Telescope.py:425-459): removes, from the default pupil, the strips of
+        width thickness_spider [m] through the centre (shifted by offset_X/Y [m]) at each angle [deg],
+        each strip kept on one side of the centre only."""
+        self.isInitialized = False
+        if thickness_spider > 0:
+            self.set_pupil()
+            pup = self.pupil.copy()
+            max_offset = self.centralObstruction * self.D / 2 - thickness_spider / 2
+            offset_X = np.zeros(len(angle)) if offset_X is None else np.asarray(offset_X, dtype=float)
+            offset_Y = np.zeros(len(angle)) if offset_Y is None else np.asarray(offset_Y, dtype=float)
+            if np.max(np.abs(offset_X)) >= max_offset or np.max(np.abs(offset_Y)) > max_offset:
+                warning('The spider offsets are too large! Weird things could happen!')
+            h = self.resolution // 2
+            x = np.linspace(-self.D / 2, self.D / 2, self.resolution)
+            for a, ox, oy in zip(angle, offset_X, offset_Y):
+                a = (a + 90) % 360
+                X, Y = np.meshgrid(x + ox, x)
+                Y = Y + oy
+                dist = np.abs(X * np.cos(np.deg2rad(a)) + Y * np.sin(np.deg2rad(-a)))
+                # the strip only extends from the centre towards its own direction
+                if 0 <= a < 90:
+                    dist[:h, :] = thickness_spider
+                elif 90 <= a < 180:
+                    dist[:, :h] = thickness_spider
+                elif 180 <= a < 270:
+                    dist[h:, :] = thickness_spider
+                else:
+                    dist[:, h:] = thickness_spider
+                pup = pup * (dist > thickness_spider / 2)
+            self.isInitialized = True
+            self.pupil = pup
+        else:
+            warning('Thickness is <=0, returning default pupil')
+            self.set_pupil()
+
     @property
     def pupil(self):
         return self._pupil
