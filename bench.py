@@ -56,9 +56,29 @@ def cpu_baseline(validI4Q, opd, frames=3, warmup=1):
     for _ in range(frames):
         o.measure_opd(opd)
     dt = time.perf_counter() - t0
+    # the same oracle over every core of the process's affinity set (the survey's "all cores" variant)
+    n_all = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else ncpu
+    if o._pool is not None:
+        o._pool.shutdown()
+    o.n_jobs = n_all
+    o._pool = None
+    o.measure_opd(opd)
+    t1 = time.perf_counter()
+    for _ in range(frames):
+        o.measure_opd(opd)
+    dt_all = time.perf_counter() - t1
+    model = ''
+    try:
+        with open('/proc/cpuinfo') as f:
+            model = next((ln.split(':', 1)[1].strip() for ln in f if ln.startswith('model name')), '')
+    except OSError:
+        pass
+    env = {k: os.environ.get(k) for k in ('OMP_NUM_THREADS', 'OPENBLAS_NUM_THREADS', 'MKL_NUM_THREADS')}
     return dict(value=frames / dt, unit='frames/s', cores=n_jobs, kind='port',
                 sample=f'{frames} full config-2 frames ({T_POINTS} points, 2 x 1280^2 complex128 FFTs each) '
-                       f'after {warmup} warm-up, {n_jobs} threads; host reports {ncpu} CPUs')
+                       f'after {warmup} warm-up, {n_jobs} threads; host reports {ncpu} CPUs',
+                all_cores={'value': frames / dt_all, 'threads': n_all},
+                host={'cpu_count': ncpu, 'affinity': n_all, 'model': model, 'thread_env': env})
 
 
 def closed_loop_leg(tel, ngs, wfs, dev, n_iter=1000, n_modes=1000):
