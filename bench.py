@@ -67,6 +67,19 @@ def cpu_baseline(validI4Q, opd, frames=3, warmup=1):
     for _ in range(frames):
         o.measure_opd(opd)
     dt_all = time.perf_counter() - t1
+    if o._pool is not None:
+        o._pool.shutdown()
+    # complex64 variant of the same restatement (the reference with precision 32), reference thread count
+    o32 = opw.PyramidOracle(N_RES, N_SUBAP, MOD, 0.5, pup, flux, 0.8e-6, n_pix_separation=SEP,
+                            n_pix_edge=EDGE, nTheta_user_defined=T_POINTS, precision='fp32', n_jobs=n_jobs,
+                            validI4Q=validI4Q)
+    o32.measure_opd(opd)
+    t2 = time.perf_counter()
+    for _ in range(frames):
+        o32.measure_opd(opd)
+    dt32 = time.perf_counter() - t2
+    if o32._pool is not None:
+        o32._pool.shutdown()
     model = ''
     try:
         with open('/proc/cpuinfo') as f:
@@ -78,6 +91,7 @@ def cpu_baseline(validI4Q, opd, frames=3, warmup=1):
                 sample=f'{frames} full config-2 frames ({T_POINTS} points, 2 x 1280^2 complex128 FFTs each) '
                        f'after {warmup} warm-up, {n_jobs} threads; host reports {ncpu} CPUs',
                 all_cores={'value': frames / dt_all, 'threads': n_all},
+                fp32={'value': frames / dt32, 'threads': n_jobs},
                 host={'cpu_count': ncpu, 'affinity': n_all, 'model': model, 'thread_env': env})
 
 
