@@ -126,7 +126,7 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
 // Output: part[b][chunk][g][r] (detector column-major partial sums).
 // ---------------------------------------------------------------------------
 template <int N, int TPT>
-__global__ void __launch_bounds__(64 * kWaves)
+__global__ void __launch_bounds__(64 * kWaves, fast_cols_min_waves(N))
 k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
            const float2* __restrict__ tw, float* __restrict__ part, int B, int T, int nchunk) {
   constexpr int n = N / 4;
@@ -367,7 +367,7 @@ template <int N, bool FWD_ONLY = false>
 static int launch_rows(const float* opd, float scale, const float* amp, const float2* tx, int P, const int* pstart,
                        const float2* ty, const float2* mx, int T, int B, int pts_per_task, float2* inter,
                        hipStream_t st) {
-  constexpr int TPT = fast_tpt(N);
+  constexpr int TPT = fast_tpt_rows(N);
   constexpr int n = N / 4;
   constexpr int G = 64 / TPT;
   const float2* tw = nullptr;

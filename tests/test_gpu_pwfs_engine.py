@@ -34,7 +34,25 @@ def _frames(eng, amp, phases):
     return out.cpu().numpy()
 
 
-@pytest.mark.parametrize('builder', [small_system, config1_system, config2_system])
+def n160_system():
+    # 40 x 40 subapertures at 4 px (N = 640: 32-lane rows pass, 16-lane columns pass)
+    from oracle import pwfs as opw
+    pup = opw.telescope_pupil(160)
+    flux = opw.flux_map(pup, opw.n_photon('I3', 8), 1e-3, 8.0)
+    return opw.PyramidOracle(160, 40, 3, 0.5, pup, flux, 0.8e-6, n_pix_separation=40, n_pix_edge=20,
+                             nTheta_user_defined=16)
+
+
+def n512_system():
+    # 128 x 128 subapertures at 4 px (N = 2048), 4 modulation points to keep the float64 oracle short
+    from oracle import pwfs as opw
+    pup = opw.telescope_pupil(512)
+    flux = opw.flux_map(pup, opw.n_photon('I3', 8), 1e-3, 39.0)
+    return opw.PyramidOracle(512, 128, 3, 0.5, pup, flux, 0.8e-6, n_pix_separation=128, n_pix_edge=64,
+                             nTheta_user_defined=4, n_jobs=4)
+
+
+@pytest.mark.parametrize('builder', [small_system, config1_system, config2_system, n160_system, n512_system])
 def test_frames_match_oracle(builder):
     o = builder()
     eng, amp = _engine_for(o)
