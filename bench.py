@@ -56,7 +56,7 @@ def cpu_baseline(validI4Q, opd, frames=3, warmup=1):
     for _ in range(frames):
         o.measure_opd(opd)
     dt = time.perf_counter() - t0
-    # the same oracle over every core of the process's affinity set (the survey's "all cores" variant)
+    # the same oracle over every core of the process's affinity set
     n_all = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else ncpu
     if o._pool is not None:
         o._pool.shutdown()
