@@ -240,8 +240,7 @@ def extended_leg(dev):
             ast ** atm * telf
         out[name] = {'ms_per_update_and_relay': round(_wall_ms(step), 3), 'sources': len(srcs), 'n': 80}
     # ELT-scale PWFS: 39 m pupil, 30 % obstruction, 480 px, 120 x 120 subapertures at 4 px (the 100 x 100 at
-    # 4.8 px variant is not constructible, Pyramid.py:216-217), N = 1920, 96 points; N = 1920 runs on the
-    # general kernels (the register-resident transforms do not fit its radix plan)
+    # 4.8 px variant is not constructible, Pyramid.py:216-217), N = 1920, 96 points (32-lane transforms)
     elt = Telescope(480, 39.0, 1e-3, centralObstruction=0.3)
     ngs3 = Source('I3', 8, display_properties=False)
     ngs3 * elt

@@ -52,7 +52,17 @@ def n512_system():
                              nTheta_user_defined=4, n_jobs=4)
 
 
-@pytest.mark.parametrize('builder', [small_system, config1_system, config2_system, n160_system, n512_system])
+def n480_system():
+    # 120 x 120 subapertures at 4 px (N = 1920 = 2^7 x 15, 32-lane transforms), 4 points
+    from oracle import pwfs as opw
+    pup = opw.telescope_pupil(480, 0.3)
+    flux = opw.flux_map(pup, opw.n_photon('I3', 8), 1e-3, 39.0)
+    return opw.PyramidOracle(480, 120, 3, 0.5, pup, flux, 0.8e-6, n_pix_separation=120, n_pix_edge=60,
+                             nTheta_user_defined=4, n_jobs=4)
+
+
+@pytest.mark.parametrize('builder', [small_system, config1_system, config2_system, n160_system, n512_system,
+                                     n480_system])
 def test_frames_match_oracle(builder):
     o = builder()
     eng, amp = _engine_for(o)
