@@ -16,8 +16,8 @@ namespace oopao {
 // kernels run at one wave per SIMD (the rows pass spills), still 4.5x faster
 // than the general path at that size.
 __host__ __device__ constexpr int fast_tpt(int N) {
-  return N == 64 ? 16 : N == 128 ? 16 : N == 256 ? 32 : N == 320 ? 16 : N == 512 ? 32
-       : N == 640 ? 16 : N == 1024 ? 64 : N == 1280 ? OOPAO_TPT_1280 : N == 1536 ? 64 : N == 1920 ? 32 : N == 2048 ? 64 : 0;
+  return N == 64 ? 16 : N == 128 ? 16 : N == 256 ? 32 : N == 320 ? 16 : N == 384 ? 16 : N == 512 ? 32
+       : N == 640 ? 16 : N == 768 ? 32 : N == 1024 ? 64 : N == 1280 ? OOPAO_TPT_1280 : N == 1536 ? 64 : N == 1920 ? 32 : N == 2048 ? 64 : 0;
 }
 
 // Lane count of the rows pass. It may differ from the columns pass's (the intermediate layout does not
@@ -29,7 +29,7 @@ __host__ __device__ constexpr int fast_tpt_rows(int N) { return N == 640 ? 32 : 
 // spills when asked for two waves, which runs it 25 % faster.
 __host__ __device__ constexpr int fast_cols_min_waves(int N) { return N == 2048 ? 2 : 1; }
 
-#define OOPAO_FAST_SIZES(X) X(64) X(128) X(256) X(320) X(512) X(640) X(1024) X(1280) X(1536) X(1920) X(2048)
+#define OOPAO_FAST_SIZES(X) X(64) X(128) X(256) X(320) X(384) X(512) X(640) X(768) X(1024) X(1280) X(1536) X(1920) X(2048)
 
 constexpr int kWaves = 4;  // wavefronts per workgroup
 

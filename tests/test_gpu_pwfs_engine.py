@@ -52,6 +52,22 @@ def n512_system():
                              nTheta_user_defined=4, n_jobs=4)
 
 
+def _square_system(n, nsub, T):
+    from oracle import pwfs as opw
+    pup = opw.telescope_pupil(n)
+    flux = opw.flux_map(pup, opw.n_photon('I3', 8), 1e-3, 8.0)
+    return opw.PyramidOracle(n, nsub, 3, 0.5, pup, flux, 0.8e-6, n_pix_separation=nsub, n_pix_edge=nsub // 2,
+                             nTheta_user_defined=T, n_jobs=4)
+
+
+def n96_system():
+    return _square_system(96, 24, 12)        # N = 384 (2^7 x 3), 16-lane transforms
+
+
+def n192_system():
+    return _square_system(192, 48, 8)        # N = 768 (2^8 x 3), 32-lane transforms
+
+
 def n480_system():
     # 120 x 120 subapertures at 4 px (N = 1920 = 2^7 x 15, 32-lane transforms), 4 points
     from oracle import pwfs as opw
@@ -61,8 +77,8 @@ def n480_system():
                              nTheta_user_defined=4, n_jobs=4)
 
 
-@pytest.mark.parametrize('builder', [small_system, config1_system, config2_system, n160_system, n512_system,
-                                     n480_system])
+@pytest.mark.parametrize('builder', [small_system, config1_system, config2_system, n96_system, n160_system,
+                                     n192_system, n512_system, n480_system])
 def test_frames_match_oracle(builder):
     o = builder()
     eng, amp = _engine_for(o)
