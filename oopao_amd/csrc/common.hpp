@@ -12,6 +12,16 @@
 
 namespace oopao {
 
+// Logical task of workgroup i out of G such that consecutive tasks run on one XCD: workgroups are
+// dispatched round-robin over the 8 XCDs (i % 8), each XCD with its own L2, so tasks that share
+// cache lines should land on the same one. A bijection of [0, G).
+__device__ __forceinline__ int xcd_task(int i, int G) {
+  const int per = (G + 7) / 8;
+  const int tall = (G % 8) ? (G % 8) : 8;
+  const int x = i & 7, slot = i >> 3;
+  return x < tall ? x * per + slot : tall * per + (x - tall) * (per - 1) + slot;
+}
+
 // Error codes returned by every C-ABI entry point (0 = success).
 enum : int {
   OK = 0,

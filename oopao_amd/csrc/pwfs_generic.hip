@@ -57,17 +57,23 @@ k_gen_rows(const float* __restrict__ opd, float scale, const float* __restrict__
   for (int X = lane; X < N; X += 64) dst[X] = r[X];
 }
 
-// Pass 2: one wavefront per (screen, point, focal column kx).
-__global__ void __launch_bounds__(64)
+// Pass 2: one wavefront per (screen, point, focal column kx), blockDim / 64 adjacent columns per
+// workgroup and consecutive column groups on one XCD: the 8-byte column reads of R and writes of G then
+// fill whole cache lines in one L1 / L2 instead of reaching memory as partial lines.
+__global__ void __launch_bounds__(256)
 k_gen_cols(const float2* __restrict__ inter1, const float2* __restrict__ maskT, const float2* __restrict__ tw,
            GenPlan p, int n, int c0, float2* __restrict__ inter2) {
   const int N = p.N;
-  const int lane = threadIdx.x;
-  const size_t task = blockIdx.x;
-  const int kx = (int)(task % N);
-  const size_t bi = task / N;
-  float2* a = gen_lds;
-  float2* s = gen_lds + N;
+  const int wpg = (int)(blockDim.x >> 6);
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int NG = (N + wpg - 1) / wpg;
+  const size_t task = (size_t)xcd_task((int)blockIdx.x, (int)gridDim.x);
+  const int kx = (int)(task % NG) * wpg + wave;
+  const size_t bi = task / NG;
+  if (kx >= N) return;  // whole wavefront; the waves of a workgroup share no data
+  float2* a = gen_lds + (size_t)wave * 2 * N;
+  float2* s = a + N;
   const float2* src = inter1 + bi * n * N + kx;
   for (int Y = lane; Y < N; Y += 64) {
     const int y = Y - c0;
@@ -273,7 +279,14 @@ int oopao_pwfs_generic_frames(const float* opd, float scale, const float* amp, c
                      reinterpret_cast<const float2*>(tx), reinterpret_cast<const float2*>(ty), tw, p, n, T, c0,
                      inter1);
   OOPAO_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_gen_cols, dim3((unsigned)((size_t)B * T * N)), dim3(64), l2, st, inter1,
+  // columns per workgroup: 4 while their LDS (2 N complex each) stays within 64 KB, else 2
+  const int wpg = (4 * l2 <= 65536) ? 4 : 2;
+  const size_t ncol_groups = (size_t)B * T * ((N + wpg - 1) / wpg);
+  if (ncol_groups > 0x7FFFFFFF) {
+    set_error("too many column tasks (B*T*N/%d = %zu)", wpg, ncol_groups);
+    return ERR_UNSUPPORTED;
+  }
+  hipLaunchKernelGGL(k_gen_cols, dim3((unsigned)ncol_groups), dim3(64 * wpg), wpg * l2, st, inter1,
                      reinterpret_cast<const float2*>(maskT), tw, p, n, c0, inter2);
   OOPAO_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_gen_rows_inv, dim3((unsigned)((size_t)B * nchunk * N)), dim3(64), l3, st, inter2, tw, p, T,

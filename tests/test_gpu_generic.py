@@ -61,8 +61,9 @@ def _check_frames(o, eng, amp, ph, label):
 
 
 @pytest.mark.parametrize('geom', [(120, 20, 2, 1), (24, 12, 2, 1), (48, 24, 2, 1), (40, 20, 3, 1.5),
-                                  (240, 60, 2, 1)])
+                                  (240, 60, 2, 1), (160, 80, 2, 1), (208, 104, 2, 1)])
 def test_generic_geometries_frames(geom):
+    # N = 264, 56, 104, 92, 496, 328 (= 8 x 41) and 424 (= 8 x 53, the LDS direct-DFT stage)
     o = _oracle(*geom)
     eng, amp = _engine(o)
     assert eng.mode == 'generic'
