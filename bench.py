@@ -636,6 +636,36 @@ def main():
     torch.cuda.synchronize()
     batched_fps = B * KB * 1000.0 / s0.elapsed_time(s1)
 
+    # the single-screen frame captured once in a HIP graph (the library's calls are stream-ordered and
+    # allocate nothing) and replayed K times; its slopes must equal the eager ones bit for bit
+    step()
+    torch.cuda.synchronize()
+    sig_eager = sig.clone()
+    gs = torch.cuda.Stream(dev)
+    gs.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(gs):
+        for _ in range(3):
+            step()
+    torch.cuda.current_stream(dev).wait_stream(gs)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    for _ in range(5):
+        graph.replay()
+    torch.cuda.synchronize()
+    graph_match = bool(torch.equal(sig, sig_eager))
+    g0 = torch.cuda.Event(enable_timing=True)
+    g1 = torch.cuda.Event(enable_timing=True)
+    g0.record()
+    for _ in range(K):
+        graph.replay()
+    g1.record()
+    torch.cuda.synchronize()
+    graph_ms = g0.elapsed_time(g1) / K
+    graph_match = graph_match and bool(torch.equal(sig, sig_eager))
+    del graph
+
     extra = {}
     if not args.no_extra:
         legs = set(args.legs.split(','))
@@ -679,6 +709,8 @@ def main():
                      'hbm_achieved_GBs': round(achieved_gbs, 1),
                      'hbm_frac': round(achieved_gbs / HBM_PEAK_GBS, 4)},
         'batched': {'batch': B, 'value': round(batched_fps, 2), 'unit': 'frames/s'},
+        'graph': {'value': round(1000.0 / graph_ms, 2), 'unit': 'frames/s', 'ms_per_step': round(graph_ms, 5),
+                  'replays': K, 'slopes_equal_eager': graph_match},
         'wall_s_timed': round(wall, 4),
     }
     out.update(extra)
