@@ -8,6 +8,8 @@ atmosphere state and RandomState streams (oracle/atmosphere.LayerOracle), the PW
 oracle/science; the reconstructor (a KL push-pull calibration of 100 modes) is shared. Tolerances: residual
 wfe trace 1e-4 relative, per-iteration slopes rel-L2 1e-4, Strehl 1e-3 %.
 """
+import os
+
 import numpy as np
 import pytest
 from numpy.random import RandomState
@@ -48,7 +50,8 @@ def test_config2_loop_traces_vs_oracle():
         rs.normal(size=lay.B.shape[1])
         ratio = [lay.vX * tel.samplingTime / lay.pixel_size, lay.vY * tel.samplingTime / lay.pixel_size]
         layers.append(LayerOracle(ms, lay.A, lay.B, lay.innerMask, lay.outerMask, rs, ratio))
-    n_iter, gain = 6, 0.4
+    # 6 iterations by default (the float64 oracle frame takes ~1 s); longer runs via the environment
+    n_iter, gain = int(os.environ.get('OOPAO_CONFIG2_LOOP_ITERS', '6')), 0.4
     out = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=gain, n_iter=n_iter, record_slopes=True, strehl_every=1)
     torch.cuda.synchronize()
     flux = opw.flux_map(tel.pupil, ngs.nPhoton, tel.samplingTime, tel.D)
