@@ -17,5 +17,5 @@ from .GainSensingCamera import GainSensingCamera  # noqa: F401
 from .BioEdge import BioEdge  # noqa: F401
 from .Asterism import Asterism  # noqa: F401
 from .Atmosphere import Atmosphere  # noqa: F401
-from .calibration.InteractionMatrix import InteractionMatrix  # noqa: F401
+from .calibration.InteractionMatrix import InteractionMatrix, InteractionMatrixFromPhaseScreen  # noqa: F401
 from .calibration.CalibrationVault import CalibrationVault  # noqa: F401

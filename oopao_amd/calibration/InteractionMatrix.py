@@ -19,6 +19,11 @@ def measure_modes(ngs, tel, dm, wfs, coefs):
     """Slopes [nSignal, B] (device float32) of the DM commands coefs [nAct, B] (device) through ngs**tel*dm*wfs."""
     opd = dm.opd_from_coefs(coefs)                     # [B, n, n]
     ngs ** tel
+    return measure_opd(tel, wfs, opd)
+
+
+def measure_opd(tel, wfs, opd):
+    """Slopes [nSignal, B] (device float32) of the OPD maps opd [B, n, n] (device, metres) through tel*wfs."""
     pupil = tel._pupil_dev
     frames = wfs.frames_dev((opd * pupil).contiguous())
     if wfs.binning != 1:
@@ -71,5 +76,61 @@ def InteractionMatrix(ngs, tel, dm, wfs, M2C, stroke, atm=None, phaseOffset=0, n
     dm.coefs = 0
     ngs ** tel
     out = CalibrationVault(intMat, invert=invert, nTrunc=nTrunc, device=dev)
+    out.D_dev = intMat
+    return out
+
+
+def InteractionMatrixFromPhaseScreen(ngs, atm, tel, wfs, phasScreens, stroke, phaseOffset=0, nMeasurements=50,
+                                     noise='off', invert=True, print_time=True):
+    """Push-pull interaction matrix of OPD maps (calibration/InteractionMatrix.py:172-259).
+
+    phasScreens is [n, n, nModes] (or one [n, n] map), numpy or a device tensor, in the reference's
+    layout; column k is 0.5 (s(+stroke screen_k) - s(-stroke screen_k)) / stroke. phaseOffset (radians,
+    [n, n]) is added to the phase of both the push and the pull maps as the reference's
+    tel.src.phase += phaseBuffer (InteractionMatrix.py:225-232). Each batch of nMeasurements maps goes
+    through the WFS as one batched device call; the columns accumulate in a device float64 matrix.
+    """
+    if noise == 'off':
+        wfs.cam.photonNoise = 0
+        wfs.cam.readoutNoise = 0
+    else:
+        print('Warning: Keeping the noise configuration for the WFS')
+    tel.isPaired = False
+    ngs * tel
+    dev = wfs.device
+    scr = phasScreens if isinstance(phasScreens, torch.Tensor) else torch.from_numpy(np.asarray(phasScreens))
+    if scr.dim() == 2:
+        scr = scr[:, :, None]
+    if scr.dim() != 3 or scr.shape[0] != tel.resolution or scr.shape[1] != tel.resolution:
+        raise OopaoError('phasScreens must be [%d, %d, nModes], got %s'
+                         % (tel.resolution, tel.resolution, tuple(scr.shape)))
+    scr = scr.to(dev, torch.float32).permute(2, 0, 1)   # [nModes, n, n] view
+    nModes = scr.shape[0]
+    nMeasurements = max(1, min(int(nMeasurements), nModes))
+    offset = None
+    if np.ndim(phaseOffset) == 2:
+        off = phaseOffset if isinstance(phaseOffset, torch.Tensor) else torch.from_numpy(np.asarray(phaseOffset))
+        offset = (off.to(dev, torch.float64) * (ngs.wavelength / (2 * np.pi))).to(torch.float32)
+    elif np.ndim(phaseOffset) == 0 and phaseOffset != 0:
+        offset = torch.full((tel.resolution, tel.resolution), float(phaseOffset) * ngs.wavelength / (2 * np.pi),
+                            dtype=torch.float32, device=dev)
+    intMat = torch.zeros((wfs.nSignal, nModes), dtype=torch.float64, device=dev)
+    for i0 in range(0, nModes, nMeasurements):
+        a = time.time()
+        i1 = min(nModes, i0 + nMeasurements)
+        push = scr[i0:i1] * stroke
+        pull = -push
+        if offset is not None:
+            push = push + offset
+            pull = pull + offset
+        sp = measure_opd(tel, wfs, push.contiguous())
+        sm = measure_opd(tel, wfs, pull.contiguous())
+        intMat[:, i0:i1] = 0.5 * (sp.double() - sm.double()) / stroke
+        if print_time:
+            torch.cuda.synchronize(dev)
+            print(str(i1) + '/' + str(nModes))
+            print('Time elapsed: ' + str(time.time() - a) + ' s')
+    ngs * tel
+    out = CalibrationVault(intMat, invert=invert, device=dev)
     out.D_dev = intMat
     return out

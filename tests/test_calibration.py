@@ -106,6 +106,40 @@ def test_interaction_matrix_matches_oracle():
     assert rel_l2(calib_pp.D, Dpp) < 1e-4
 
 
+@pytest.mark.gpu
+def test_interaction_matrix_from_phase_screen_matches_oracle():
+    # calibration/InteractionMatrix.py:172-259: push-pull columns of OPD maps [n, n, nModes], with and
+    # without a phase offset added to both maps, batch sizes that leave a ragged last batch
+    from oopao_amd import InteractionMatrixFromPhaseScreen
+    from oracle.dm import DMOracle
+    tel, ngs, atm, dm, wfs = _small_loop_system()
+    o = _oracle_pwfs(wfs, tel, ngs)
+    od = DMOracle(32, 8, 8, 0.35)
+    modes = od.modes()
+    rs = np.random.RandomState(4)
+    screens = np.stack([od.opd(modes, rs.randn(dm.nValidAct)) for _ in range(7)], axis=2)
+    screens /= np.abs(screens).max()
+    stroke = 15e-9
+    offset = 0.3 * rs.randn(32, 32)
+    k = ngs.wavelength / (2 * np.pi)
+    for off, nMeas in ((0, 3), (offset, 7), (offset, 1)):
+        calib = InteractionMatrixFromPhaseScreen(ngs, atm, tel, wfs, screens, stroke, phaseOffset=off,
+                                                 nMeasurements=nMeas, invert=False, print_time=False)
+        base = off * k if np.ndim(off) == 2 else 0
+        Dref = np.stack([0.5 * (o.measure_opd(screens[:, :, j] * stroke + base)[1]
+                                - o.measure_opd(-screens[:, :, j] * stroke + base)[1]) / stroke
+                         for j in range(7)], axis=1)
+        assert calib.D.shape == (wfs.nSignal, 7)
+        err = rel_l2(calib.D, Dref)
+        print('IM from phase screens, nMeasurements', nMeas, 'rel_l2', err)
+        assert err < 1e-4
+    one = InteractionMatrixFromPhaseScreen(ngs, atm, tel, wfs, screens[:, :, 2], stroke, invert=False,
+                                           print_time=False)
+    assert one.D.shape == (wfs.nSignal, 1)
+    ref_one = 0.5 * (o.measure_opd(screens[:, :, 2] * stroke)[1] - o.measure_opd(-screens[:, :, 2] * stroke)[1]) / stroke
+    assert rel_l2(one.D[:, 0], ref_one) < 1e-4
+
+
 def _oracle_layers(seed, tel, atm):
     """Oracle layers from the device atmosphere's current state and the same RandomState streams."""
     from oracle.atmosphere import LayerOracle
