@@ -177,6 +177,58 @@ class Atmosphere:
     def _set_B(self, lay):
         lay.B_dev = torch.from_numpy(np.ascontiguousarray(lay.B)).to(self.device)
 
+    def save_state(self, path):
+        """Write the evolving state of every layer to an .npz file: the extruded screen, its (min, max),
+        the sub-pixel shift buffer, the per-step ratio and the layer's RandomState. The reference has no
+        simulation checkpoint; a second initializeAtmosphere resets the layers (Atmosphere.py:246-271).
+        load_state on an atmosphere built with the same parameters resumes the same sequence of screens."""
+        if self.hasNotBeenInitialized:
+            raise OopaoError('The Atmosphere object needs to be initialised using the initialiseAtmosphere()')
+        out = {'nLayer': np.int64(self.nLayer), 'r0': np.float64(self.r0), 'L0': np.float64(self.L0),
+               'resolution': np.int64(self.telescope.resolution)}
+        for i in range(self.nLayer):
+            lay = getattr(self, 'layer_' + str(i + 1))
+            key, pos, has_gauss, cached = lay.randomState.get_state()[1:]
+            p = 'layer_%d_' % (i + 1)
+            out[p + 'M'] = np.int64(lay.M)
+            out[p + 'map'] = lay.map.cpu().numpy()
+            out[p + 'mm'] = lay.mm[:2].cpu().numpy()
+            out[p + 'buff'] = np.asarray(lay.buff, np.float64)
+            out[p + 'ratio'] = np.asarray(lay.ratio, np.float64)
+            out[p + 'notDoneOnce'] = np.bool_(lay.notDoneOnce)
+            out[p + 'shift'] = np.asarray(getattr(lay, 'shift', (0.0, 0.0)), np.float64)
+            out[p + 'rs_key'] = np.asarray(key, np.uint32)
+            out[p + 'rs_pos'] = np.int64(pos)
+            out[p + 'rs_gauss'] = np.asarray([has_gauss, cached], np.float64)
+        np.savez(path, **out)
+
+    def load_state(self, path):
+        """Restore what save_state wrote; the layer geometry and r0 must match this atmosphere's."""
+        if self.hasNotBeenInitialized:
+            raise OopaoError('The Atmosphere object needs to be initialised using the initialiseAtmosphere()')
+        with np.load(path) as f:
+            if int(f['nLayer']) != self.nLayer or int(f['resolution']) != self.telescope.resolution \
+                    or float(f['r0']) != float(self.r0) or float(f['L0']) != float(self.L0):
+                raise OopaoError('the saved atmosphere state does not match this atmosphere (nLayer, resolution, '
+                                 'r0, L0)')
+            for i in range(self.nLayer):
+                lay = getattr(self, 'layer_' + str(i + 1))
+                p = 'layer_%d_' % (i + 1)
+                if int(f[p + 'M']) != lay.M:
+                    raise OopaoError('the saved atmosphere state does not match layer %d (M)' % (i + 1))
+                lay.map.copy_(torch.from_numpy(f[p + 'map']))
+                lay.mm[:2].copy_(torch.from_numpy(f[p + 'mm']))
+                lay.buff = f[p + 'buff'].copy()
+                lay.ratio = f[p + 'ratio'].copy()
+                lay.notDoneOnce = bool(f[p + 'notDoneOnce'])
+                lay.shift = tuple(float(v) for v in f[p + 'shift'])
+                g = f[p + 'rs_gauss']
+                lay.randomState = RandomState()
+                lay.randomState.set_state(('MT19937', f[p + 'rs_key'], int(f[p + 'rs_pos']), int(g[0]), float(g[1])))
+        self._compute_opd()
+        if self.telescope.isPaired:
+            self.telescope.src ** self * self.telescope
+
     def generateNewPhaseScreen(self, seed=None):
         """Atmosphere.generateNewPhaseScreen (Atmosphere.py:779-810)."""
         if seed is None:

@@ -107,3 +107,35 @@ def test_property_setters():
     from oopao_amd.tools import OopaoError
     with pytest.raises(OopaoError):
         atm.fractionalR0 = [1.0]
+
+
+def test_save_load_state_resumes_the_same_screens(tmp_path):
+    # a checkpoint taken mid-run resumes bit for bit, on the same object and on a second atmosphere built
+    # with the same parameters (0.8 / 1.6 px per step: whole-pixel extrusions and the sub-pixel buffer)
+    tel, ngs, atm = _atm(48, 0.02, [10, 20], [30, 120], 7)
+    for _ in range(5):
+        atm.update()
+    path = str(tmp_path / 'atm_state.npz')
+    atm.save_state(path)
+    ref = []
+    for _ in range(7):
+        atm.update()
+        ref.append(atm._OPD_dev.clone())
+    atm.load_state(path)
+    for k in range(7):
+        atm.update()
+        assert torch_equal(atm._OPD_dev, ref[k]), k
+    tel2, ngs2, atm2 = _atm(48, 0.02, [10, 20], [30, 120], 99)
+    atm2.load_state(path)
+    for k in range(7):
+        atm2.update()
+        assert torch_equal(atm2._OPD_dev, ref[k]), k
+    tel3, ngs3, atm3 = _atm(40, 0.02, [10, 20], [30, 120], 7)
+    from oopao_amd.tools import OopaoError
+    with pytest.raises(OopaoError):
+        atm3.load_state(path)
+
+
+def torch_equal(a, b):
+    import torch
+    return bool(torch.equal(a, b))
