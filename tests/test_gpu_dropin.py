@@ -196,3 +196,27 @@ def test_modulation_and_light_ratio_setters(nTheta_user):
         err = rel_l2(wfs.signal, o.measure_opd(opd)[1])
         print('modulation', mod, 'lightRatio', lr, 'nTheta', wfs.nTheta, 'rel_l2', err)
         assert err <= 1e-5
+
+
+def test_error_paths_raise_oopao_error():
+    # DeformableMirror.py:683-685 exits the interpreter on a wrong coefficient count: the drop-in raises;
+    # Pyramid.py:667-670 raises on a source whose wavelength differs from the calibration wavelength;
+    # a phase-screen interaction matrix of the wrong map size is rejected before any device work
+    from oopao_amd import Source, InteractionMatrixFromPhaseScreen
+    from oopao_amd.tools import OopaoError
+    tel, ngs, dm, wfs = _config1_objects()
+    with pytest.raises(OopaoError):
+        dm.coefs = np.zeros(356)
+    with pytest.raises(OopaoError):
+        dm.coefs = 1.0
+    dm.coefs = 0
+    ngs ** tel * dm * wfs
+    s0 = wfs.signal.copy()
+    other = Source('K', 8, display_properties=False)
+    with pytest.raises(OopaoError):
+        other ** tel * dm * wfs
+    with pytest.raises(OopaoError):
+        InteractionMatrixFromPhaseScreen(ngs, None, tel, wfs, np.zeros((40, 40, 2)), 1e-8, print_time=False)
+    # the objects stay usable after the errors
+    ngs ** tel * dm * wfs
+    assert np.array_equal(wfs.signal, s0)
