@@ -16,7 +16,7 @@ from . import _lib
 from ._device import to_dev, to_host
 from .Detector import Detector
 from .pwfs_engine import PWFSEngine, SlopesEngine
-from .tools import OopaoError, warning
+from .tools import OopaoError, WorkspaceAllocationError, warning
 
 
 class Pyramid:
@@ -465,12 +465,23 @@ class Pyramid:
         scale = 2 * np.pi / self.src.wavelength
         B = opd.shape[0]
         bs = self._batch_size(B)
-        if bs >= B:
-            return self.engine.frames(opd.contiguous(), scale, amp, weighted=weighted)
-        out = torch.empty((B, self.engine.cam, self.engine.cam), dtype=torch.float32, device=self.device)
-        for j in range(0, B, bs):
-            self.engine.frames(opd[j:j + bs].contiguous(), scale, amp, out=out[j:j + bs], weighted=weighted)
-        return out
+        while True:
+            try:
+                if bs >= B:
+                    return self.engine.frames(opd.contiguous(), scale, amp, weighted=weighted)
+                out = torch.empty((B, self.engine.cam, self.engine.cam), dtype=torch.float32, device=self.device)
+                for j in range(0, B, bs):
+                    self.engine.frames(opd[j:j + bs].contiguous(), scale, amp, out=out[j:j + bs],
+                                       weighted=weighted)
+                return out
+            except WorkspaceAllocationError as e:
+                # the reference frees memory by measuring fewer screens at a time (Pyramid.py:713-740):
+                # halve the batch, keep the smaller size for later calls, and measure again
+                if bs == 1:
+                    raise
+                bs = max(1, min(bs, B) // 2)
+                self.max_batch_bytes = bs * max(1, self.engine.bytes_per_screen())
+                warning('%s; measuring %d screens at a time' % (e.string, bs))
 
     def wfs_measure(self, phase_in=None, integrate=True):
         """Pyramid.wfs_measure (Pyramid.py:666-833): all modulation points, 2-D or batched 3-D."""

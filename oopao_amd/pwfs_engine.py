@@ -25,6 +25,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from ._device import alloc_workspace
 from .tools import OopaoError
 
 
@@ -208,7 +209,7 @@ class PWFSEngine:
     def _scratch(self, nbytes):
         if self._ws is None or self._ws.numel() < nbytes:
             self._ws = None
-            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            self._ws = alloc_workspace(nbytes, self.device)
         return self._ws
 
     def _generic_nchunk(self, B):

@@ -11,6 +11,11 @@ class OopaoError(Exception):
         super().__init__(self.message)
 
 
+class WorkspaceAllocationError(OopaoError):
+    """A device workspace could not be allocated (out of device memory, or above the byte cap set in the
+    environment variable OOPAO_AMD_MAX_WORKSPACE_BYTES, which forces the failure from the host)."""
+
+
 def warning(string):
     """ANSI warning print (tools/tools.py:558-559)."""
     print('\033[01m\033[33m' + 'OOPAO Warning: \n' + string + '\033[0m')

@@ -220,3 +220,32 @@ def test_error_paths_raise_oopao_error():
     # the objects stay usable after the errors
     ngs ** tel * dm * wfs
     assert np.array_equal(wfs.signal, s0)
+
+
+def test_workspace_failure_halves_the_batch(monkeypatch):
+    # a workspace that cannot be allocated (forced from the host by the byte cap) makes the Pyramid
+    # measure fewer screens per call instead of failing; the frames equal the unconstrained ones
+    import torch
+    tel, ngs, dm, wfs = _config1_objects()
+    C = np.random.RandomState(6).normal(0, 40e-9, (357, 8))
+    dm.coefs = C
+    ngs ** tel * dm * wfs
+    ref = wfs.cam._frame.clone() if hasattr(wfs.cam, '_frame') else None
+    s_ref = wfs.signal.copy()
+    tel2, ngs2, dm2, wfs2 = _config1_objects()
+    per = wfs2.engine.bytes_per_screen()
+    monkeypatch.setenv('OOPAO_AMD_MAX_WORKSPACE_BYTES', str(int(2.5 * per)))
+    dm2.coefs = C
+    ngs2 ** tel2 * dm2 * wfs2
+    monkeypatch.delenv('OOPAO_AMD_MAX_WORKSPACE_BYTES')
+    assert wfs2.max_batch_bytes <= 2 * per
+    assert wfs2.signal.shape == (wfs2.nSignal, 8)
+    assert rel_l2(wfs2.signal, s_ref) <= 1e-6
+    if ref is not None:
+        assert torch.allclose(wfs2.cam._frame, ref, rtol=1e-5, atol=1e-6 * float(ref.abs().max()))
+    # a single screen that cannot get its workspace is an error, not a silent result
+    # (the Pyramid's own calibration measures one screen)
+    monkeypatch.setenv('OOPAO_AMD_MAX_WORKSPACE_BYTES', str(per // 2))
+    from oopao_amd.tools import WorkspaceAllocationError
+    with pytest.raises(WorkspaceAllocationError):
+        _config1_objects()
