@@ -20,7 +20,7 @@ from ._device import to_host
 from .Detector import Detector
 from .Pyramid import Pyramid
 from .pwfs_engine import PWFSEngine, SlopesEngine
-from .tools import OopaoError, warning
+from .tools import OopaoError, WorkspaceAllocationError, warning
 
 
 def bioedge_mask_set(N, n, n_initial, nSubap, zero_padding_factor, grey_width=0.0, grey_length=False,
@@ -327,13 +327,22 @@ class BioEdge(Pyramid):
         cam = self.cam.resolution
         out = torch.empty((B, 2 * cam, 2 * cam), dtype=torch.float32, device=self.device)
         bs = self._batch_size(B)
-        for j in range(0, B, bs):
-            o = opd[j:j + bs].contiguous()
-            for k, e in enumerate(self.engines):
-                r, c = divmod(k, 2)
-                out[j:j + bs, r * cam:(r + 1) * cam, c * cam:(c + 1) * cam] = e.frames(o, scale, amp,
-                                                                                      weighted=weighted)
-        return out
+        while True:
+            try:
+                for j in range(0, B, bs):
+                    o = opd[j:j + bs].contiguous()
+                    for k, e in enumerate(self.engines):
+                        r, c = divmod(k, 2)
+                        out[j:j + bs, r * cam:(r + 1) * cam, c * cam:(c + 1) * cam] = e.frames(o, scale, amp,
+                                                                                              weighted=weighted)
+                return out
+            except WorkspaceAllocationError as err:
+                # as Pyramid.frames_dev: fewer screens per call, remembered in max_batch_bytes
+                if bs == 1:
+                    raise
+                bs = max(1, min(bs, B) // 2)
+                self.max_batch_bytes = bs * max(1, self.engine.bytes_per_screen())
+                warning('%s; measuring %d screens at a time' % (err.string, bs))
 
     def _raw_dev(self, args, weighted):
         opd, wavelength, amp = args

@@ -88,3 +88,24 @@ def test_bioedge_grey_length_and_shifts():
     err = rel_l2(wfs.cam.frame, o.frame(phase))
     print('frame rel-L2 %.2e' % err)
     assert err < 1e-5
+
+
+def test_bioedge_workspace_failure_halves_the_batch(monkeypatch):
+    # a workspace above the host-side cap makes the Bi-O Edge measure fewer screens per call
+    import torch
+    from oopao_amd.BioEdge import BioEdge
+    tel, ngs = _setup()
+    kw = dict(modulation=2, grey_width=1, lightRatio=0.1, n_pix_separation=4, nTheta_user_defined=16,
+              postProcessing='slopesMaps')
+    maps = torch.from_numpy(np.stack([_opd(tel, 10 + k) for k in range(6)]).astype(np.float32)).cuda()
+    wfs = BioEdge(nSubap=8, telescope=tel, **kw)
+    ngs * tel * wfs
+    ref = wfs.frames_dev(maps * tel._pupil_dev)
+    wfs2 = BioEdge(nSubap=8, telescope=tel, **kw)
+    ngs * tel * wfs2
+    per = wfs2.engine.bytes_per_screen()
+    monkeypatch.setenv('OOPAO_AMD_MAX_WORKSPACE_BYTES', str(int(2.5 * per)))
+    got = wfs2.frames_dev(maps * tel._pupil_dev)
+    monkeypatch.delenv('OOPAO_AMD_MAX_WORKSPACE_BYTES')
+    assert wfs2.max_batch_bytes <= 2 * per
+    assert rel_l2(got.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
