@@ -30,7 +30,8 @@ from numpy.random import RandomState
 
 from . import _lib
 from ._device import ptr, stream_ptr, to_host
-from .phaseStats import ft_sh_phase_screen, ft_phase_screen, makeCovarianceMatrix
+from .phaseStats import (ft_sh_phase_screen, ft_phase_screen, makeCovarianceMatrix, extrusion_rings,
+                         extrusion_operators)
 from .tools import OopaoError, warning
 
 
@@ -134,15 +135,8 @@ class Atmosphere:
         res, M = lay.resolution, lay.resolution + self.n_extra_pixel
         lay.OPD_host = ft_sh_phase_screen(self.r0, self.L0, res, lay.D / res, seed=i_layer)
         lay.seed = i_layer
-        outer = np.ones((M, M))
-        outer[1:-1, 1:-1] = 0
-        inner = np.ones((M, M)) - outer
-        inner[1 + self.n_extra_pixel:-1 - self.n_extra_pixel, 1 + self.n_extra_pixel:-1 - self.n_extra_pixel] = 0
+        outer, inner, lay.innerZ, lay.outerZ = extrusion_rings(res, self.n_extra_pixel, lay.D)
         lay.outerMask, lay.innerMask = outer, inner
-        x = np.linspace(0, res + 1, res + 2) * lay.D / (res - 1)
-        u, v = np.meshgrid(x, x)
-        lay.innerZ = u[inner != 0] + 1j * v[inner != 0]
-        lay.outerZ = u[outer != 0] + 1j * v[outer != 0]
         # covariance matrices (Atmosphere.py:707-777); the ring geometry depends only on the layer
         # resolution, so layers of equal size share them
         cache = self.__dict__.setdefault('_cov_cache', {})
@@ -154,9 +148,7 @@ class Atmosphere:
                           makeCovarianceMatrix(lay.outerZ, lay.outerZ, self.L0, self.r0_def), np.linalg.pinv(ZZt))
         self.ZZt, self.ZXt, self.XXt, self.ZZt_inv = cache[key]
         lay.ZZt, lay.ZXt, lay.XXt, lay.ZZt_inv = self.ZZt, self.ZXt, self.XXt, self.ZZt_inv
-        c = (self.r0_def / self.r0) ** (5 / 3)
-        lay.A = np.matmul((lay.ZXt * c).T, lay.ZZt_inv / c)
-        lay.B = np.linalg.cholesky(lay.XXt * c - np.matmul(lay.A, lay.ZXt * c))
+        lay.A, lay.B = extrusion_operators(lay.ZZt, lay.ZXt, lay.XXt, lay.ZZt_inv, self.r0_def, self.r0)
         lay.notDoneOnce = True
         lay.buff = np.zeros(2)
         lay.ratio = np.zeros(2)
@@ -652,11 +644,9 @@ class Atmosphere:
             print('Updating the Atmosphere covariance matrices...')
             self.seeingArcsec = self.rad2arcsec * (self.wavelength / val)
             self.cn2 = (self.r0 ** (-5. / 3) / (0.423 * (2 * np.pi / self.wavelength) ** 2)) / np.max([1, np.max(self.altitude)])
-            c = (self.r0_def / self.r0) ** (5 / 3)
             for i in range(self.nLayer):
                 lay = getattr(self, 'layer_' + str(i + 1))
-                BBt = lay.XXt * c - np.matmul(lay.A, lay.ZXt * c)
-                lay.B = np.linalg.cholesky(BBt)
+                lay.B = extrusion_operators(lay.ZZt, lay.ZXt, lay.XXt, lay.ZZt_inv, self.r0_def, self.r0)[1]
                 self._set_B(lay)
 
     @property

@@ -143,33 +143,15 @@ struct Dft<8, DIR> {
 // Compile-time radix plans. Every radix must divide E so that each lane owns
 // whole butterflies. Larger radices first: fewer LDS exchanges.
 // ---------------------------------------------------------------------------
-// Odd radices go first: a radix-5 (or 3) stage at a small span writes with a
-// lane stride coprime to the 16 eight-byte LDS bank groups, and the power-of-
-// two stages that follow then run at spans >= 4/5 where their writes are
-// (nearly) conflict-free; power-of-two stages at spans 1 and 4 are 4-way
-// bank-conflicted.
-// (Measured on gfx950 at N = 1280: odd-first plans run 2x slower because spans
-// 5, 20, 80, 320 do not divide the 64-lane count, so every twiddle becomes a
-// per-butterfly load; kept as an option, off by default.)
-#ifndef OOPAO_ODD_FIRST
-#define OOPAO_ODD_FIRST 0
-#endif
+// (Measured on gfx950 at N = 1280: plans with the odd radices first run 2x slower, because spans 5,
+// 20, 80, 320 do not divide the 64-lane count and every twiddle becomes a per-butterfly load.)
 __host__ __device__ constexpr int pick_radix(int rem, int E) {
-#if OOPAO_ODD_FIRST
-  return (rem % 5 == 0 && E % 5 == 0) ? 5
-       : (rem % 3 == 0 && E % 3 == 0) ? 3
-       : (rem % 8 == 0 && E % 8 == 0) ? 8
-       : (rem % 4 == 0 && E % 4 == 0) ? 4
-       : (rem % 2 == 0 && E % 2 == 0) ? 2
-       : 0;
-#else
   return (rem % 8 == 0 && E % 8 == 0) ? 8
        : (rem % 4 == 0 && E % 4 == 0) ? 4
        : (rem % 5 == 0 && E % 5 == 0) ? 5
        : (rem % 3 == 0 && E % 3 == 0) ? 3
        : (rem % 2 == 0 && E % 2 == 0) ? 2
        : 0;
-#endif
 }
 
 __host__ __device__ constexpr bool plan_ok(int rem, int E) {
@@ -180,20 +162,10 @@ __host__ __device__ constexpr int plan_stages(int rem, int E) {
   return rem == 1 ? 0 : 1 + plan_stages(rem / pick_radix(rem, E), E);
 }
 
-// Optional exchange-buffer index padding (one float2 every 32). Off by default:
-// on gfx950 the unpadded layout lets every LDS address fold into the
-// instruction's immediate offset, which saved registers (occupancy 2 -> 4
-// waves/SIMD for the 1280-point columns pass) and ran 10-30% faster.
-#ifndef OOPAO_LDS_PAD
-#define OOPAO_LDS_PAD 0
-#endif
-#if OOPAO_LDS_PAD
-__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
-__host__ __device__ constexpr int lds_len(int N) { return N + N / 32 + 1; }
-#else
-__device__ __forceinline__ int lds_pad(int i) { return i; }
+// Exchange buffers are unpadded: on gfx950 every LDS address then folds into the instruction's
+// immediate offset, which saved registers (occupancy 2 -> 4 waves/SIMD for the 1280-point columns
+// pass) and ran 10-30 % faster than one float2 of padding every 32.
 __host__ __device__ constexpr int lds_len(int N) { return N; }
-#endif
 
 // XOR swizzle of the exchange-buffer index for 64-lane transforms: the low 4
 // bits (the 16 eight-byte bank groups of one 128-byte LDS row) are XORed with
@@ -201,9 +173,6 @@ __host__ __device__ constexpr int lds_len(int N) { return N; }
 // the span-1 and span-4 radix-4 writes (4-way conflicted unswizzled) become
 // conflict-free; the XOR term depends only on the lane, so per-butterfly
 // offsets (multiples of 64) remain immediate offsets.
-#ifndef OOPAO_LDS_SWZ
-#define OOPAO_LDS_SWZ 1
-#endif
 __device__ __forceinline__ int lds_swz(int a) {
   const int rw = (a >> 4) & 3;
   return a ^ (rw | (rw << 2));
@@ -251,7 +220,7 @@ __device__ __forceinline__ void stockham_stage(float2* v, float2* lds, const flo
   const float2* twp = launder(tw);
   // XOR-swizzled exchange slots for 64-lane transforms (see lds_swz); the
   // lane-dependent part is computed once per stage, butterflies add constants.
-  constexpr bool kSwz = SWZ && (OOPAO_LDS_SWZ != 0) && (TPT == 64) && ((64 % NS == 0) || (NS % 64 == 0));
+  constexpr bool kSwz = SWZ && (TPT == 64) && ((64 % NS == 0) || (NS % 64 == 0));
   int p0[R];
   if constexpr (kSwz) {
     const int base0 = (t / NS) * (NS * R) + (t % NS);
@@ -297,7 +266,7 @@ __device__ __forceinline__ void stockham_stage(float2* v, float2* lds, const flo
     } else {
       const int base = (j / NS) * (NS * R) + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lds_pad(base + r * NS)] = x[r];
+      for (int r = 0; r < R; ++r) lds[base + r * NS] = x[r];
     }
   }
   wave_lds_sync();
@@ -307,7 +276,7 @@ __device__ __forceinline__ void stockham_stage(float2* v, float2* lds, const flo
     for (int m = 0; m < E; ++m) v[m] = lds[q + 64 * m];
   } else {
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = lds[lds_pad(t + m * TPT)];
+    for (int m = 0; m < E; ++m) v[m] = lds[t + m * TPT];
   }
   wave_lds_sync();
 }

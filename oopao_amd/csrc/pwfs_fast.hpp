@@ -3,10 +3,6 @@
 #pragma once
 #include "fft_core.hpp"
 
-#ifndef OOPAO_TPT_1280
-#define OOPAO_TPT_1280 64
-#endif
-
 namespace oopao {
 
 // Lane count per transform for each supported padded length N (fast path: the
@@ -17,7 +13,7 @@ namespace oopao {
 // than the general path at that size.
 __host__ __device__ constexpr int fast_tpt(int N) {
   return N == 64 ? 16 : N == 128 ? 16 : N == 256 ? 32 : N == 320 ? 16 : N == 384 ? 16 : N == 512 ? 32
-       : N == 640 ? 16 : N == 768 ? 32 : N == 1024 ? 64 : N == 1280 ? OOPAO_TPT_1280 : N == 1536 ? 64 : N == 1920 ? 32 : N == 2048 ? 64 : 0;
+       : N == 640 ? 16 : N == 768 ? 32 : N == 1024 ? 64 : N == 1280 ? 64 : N == 1536 ? 64 : N == 1920 ? 32 : N == 2048 ? 64 : 0;
 }
 
 // Lane count of the rows pass. It may differ from the columns pass's (the intermediate layout does not

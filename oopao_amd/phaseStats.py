@@ -35,6 +35,33 @@ def makeCovarianceMatrix(rho1, rho2, L0, r0_def):
     return out
 
 
+def extrusion_rings(resolution, n_extra_pixel, D):
+    """Ring geometry of the Assemat extrusion for a layer of ``resolution`` px and diameter D
+    (Atmosphere.py:335-349): on the (resolution + n_extra_pixel)^2 support, the outer ring (the
+    border pixels that an extrusion writes) and the inner ring (the n_extra_pixel-wide band inside it
+    that conditions the draw), as masks and as complex coordinates u + i v on a pitch of
+    D / (resolution - 1). Returns (outer, inner, innerZ, outerZ)."""
+    M = resolution + n_extra_pixel
+    outer = np.ones((M, M))
+    outer[1:-1, 1:-1] = 0
+    inner = np.ones((M, M)) - outer
+    inner[1 + n_extra_pixel:-1 - n_extra_pixel, 1 + n_extra_pixel:-1 - n_extra_pixel] = 0
+    x = np.linspace(0, resolution + 1, resolution + 2) * D / (resolution - 1)
+    u, v = np.meshgrid(x, x)
+    return outer, inner, u[inner != 0] + 1j * v[inner != 0], u[outer != 0] + 1j * v[outer != 0]
+
+
+def extrusion_operators(ZZt, ZXt, XXt, ZZt_inv, r0_def, r0):
+    """Extrusion operators at Fried parameter r0 from the covariances computed at r0_def
+    (Atmosphere.py:358-363, 1027-1035): with c = (r0_def / r0)^(5/3), A = (c ZXt)^T (ZZt_inv / c)
+    (independent of r0) and B = chol(c XXt - A c ZXt), so that an extruded ring X = A Z + B xi has
+    the von Karman covariance c XXt jointly with the inner ring Z. Returns (A, B)."""
+    c = (r0_def / r0) ** (5 / 3)
+    A = np.matmul((ZXt * c).T, ZZt_inv / c)
+    B = np.linalg.cholesky(XXt * c - np.matmul(A, ZXt * c))
+    return A, B
+
+
 def ift2(G):
     """fftshift(fft2(fftshift(G))) (phaseStats.py:135-150)."""
     return np.fft.fftshift(np.fft.fft2(np.fft.fftshift(G)))

@@ -1,0 +1,127 @@
+"""GPU: the pins of tests/test_oracle_pins.py applied to the HIP path, and CalibrationVault on the device.
+
+* the slopes kernel (oopao_pwfs_slopes, Pyramid.py:835-854) on a hand-built frame: exact values;
+* a saturated tip / tilt through the drop-in Pyramid's frames (Pyramid.py:612-757): the two lit
+  quadrants, mean normalised slope -1 along the axis, and the oracle's slopes within 1e-5;
+* the valid-pixel threshold margin of the drop-in's float32 calibration frame at config 2;
+* CalibrationVault's device routes (rocSOLVER SVD, Gram + eigh) against numpy's float64 SVD
+  (calibration/CalibrationVault.py:45-70).
+"""
+import numpy as np
+import pytest
+import torch
+
+from tests.helpers import rel_l2, config1_system
+from tests.test_oracle_pins import SAT, hand_frame
+
+pytestmark = pytest.mark.gpu
+
+
+def _config1_pyramid():
+    from oopao_amd import Telescope, Source, Pyramid
+    tel = Telescope(80, 8, 1e-3, centralObstruction=0)
+    ngs = Source('I3', 8, display_properties=False)
+    ngs * tel
+    wfs = Pyramid(20, tel, 3, 0.5, n_pix_separation=20, n_pix_edge=10, nTheta_user_defined=48)
+    return tel, ngs, wfs
+
+
+def test_slopes_kernel_on_hand_built_frame():
+    from oopao_amd.pwfs_engine import SlopesEngine
+    from oracle import pwfs as opw
+    tel, ngs, wfs = _config1_pyramid()
+    g = opw.Geometry(80, 20, 20, 10)
+    fr, valid = hand_frame(g)
+    quad, w = wfs._quadrant_origins()
+    eng = SlopesEngine(wfs.device, wfs.cam.resolution, w, quad)
+    eng.set_valid(valid)
+    frames = torch.from_numpy(np.stack([fr, 3.0 * fr]).astype(np.float32)).to(wfs.device)
+    s2d, s = eng.slopes(frames)
+    s2d = s2d.cpu().numpy().astype(np.float64)
+    for b in range(2):                       # the second frame: same slopes at 3x the flux
+        assert np.allclose(s2d[b, :w][valid], 0.2, rtol=0, atol=1e-7)
+        assert np.allclose(s2d[b, w:][valid], -0.6, rtol=0, atol=1e-7)
+        assert np.all(s2d[b, :w][~valid] == 0) and np.all(s2d[b, w:][~valid] == 0)
+    ref = opw.slopes_maps(fr, valid, 0, g)[1]
+    assert rel_l2(s[0].cpu().numpy(), ref) <= 1e-7
+
+
+def test_saturated_tip_tilt_through_dropin():
+    from oopao_amd.pwfs_engine import SlopesEngine
+    from oracle import pwfs as opw
+    tel, ngs, wfs = _config1_pyramid()
+    o = config1_system()
+    assert np.array_equal(wfs.validI4Q, o.validI4Q)
+    quad, w = wfs._quadrant_origins()
+    eng = SlopesEngine(wfs.device, wfs.cam.resolution, w, quad)
+    eng.set_valid(wfs.validI4Q)                  # reference slopes zero: raw normalised slopes
+    v = wfs.validI4Q
+    lam = ngs.wavelength
+    for axis, grid, lit in (('tip', o.tip, (2, 3)), ('tilt', o.tilt, (3, 4))):
+        opd = torch.from_numpy((SAT * grid * lam / (2 * np.pi)).astype(np.float32)).to(wfs.device)[None]
+        frames = wfs.frames_dev(opd)
+        s2d, s = eng.slopes(frames)
+        fr = frames[0].double().cpu().numpy()
+        q = np.array([opw.grab_quadrant(fr, k, o.geom)[v].sum() for k in (1, 2, 3, 4)])
+        q /= q.sum()
+        assert all(q[k - 1] > 0.49 for k in lit), (axis, q)
+        g2d = s2d[0].double().cpu().numpy()
+        sx, sy = g2d[:w][v].mean(), g2d[w:][v].mean()
+        along, across = (sx, sy) if axis == 'tip' else (sy, sx)
+        print('%s: mean slope along %.6f across %.2e, lit fractions %s' % (axis, along, across, np.round(q, 5)))
+        assert abs(along + 1) < 0.02 and abs(across) < 1e-5
+        ref2d = opw.slopes_maps(o.frame(SAT * grid), v, 0, o.geom)[0]
+        assert rel_l2(g2d, ref2d) <= 1e-5
+
+
+def test_valid_threshold_margin_config2():
+    # the drop-in selects validI4Q on its float32 calibration frame (Pyramid.py:541-589); at config 2
+    # the nearest pixel sits 1.1e-3 (relative to max I4Q) from the threshold in the float64 oracle
+    from oopao_amd import Telescope, Source, Pyramid
+    from tests.helpers import config2_system
+    tel = Telescope(320, 8, 1e-3)
+    ngs = Source('I3', 8, display_properties=False)
+    ngs * tel
+    wfs = Pyramid(80, tel, 3, 0.5, n_pix_separation=80, n_pix_edge=40, nTheta_user_defined=72)
+    r = wfs.I4Q / wfs.I4Q.max()
+    margin = float(np.min(np.abs(r - 0.5)))
+    o = config2_system(n_jobs=16)
+    try:
+        r_ref = o.I4Q / o.I4Q.max()
+        dev = float(np.max(np.abs(r - r_ref)))
+        print('config 2: threshold margin %.3e (float32 frame), max |ratio - float64 ratio| %.2e' % (margin, dev))
+        assert margin > 1e-4 and dev < 1e-5
+        assert np.array_equal(wfs.validI4Q, o.validI4Q)
+    finally:
+        if o._pool is not None:
+            o._pool.shutdown()
+
+
+def _im_like(m, k, cond, seed):
+    rs = np.random.RandomState(seed)
+    r = min(m, k)
+    Ua = np.linalg.qr(rs.randn(m, r))[0]
+    Va = np.linalg.qr(rs.randn(k, r))[0]
+    return Ua @ np.diag(np.logspace(0, -np.log10(cond), r)) @ Va.T
+
+
+@pytest.mark.parametrize('method', ['svd', 'gram', 'auto'])
+@pytest.mark.parametrize('shape', [(632, 1000), (2000, 1000)])
+def test_calibration_vault_on_device(method, shape):
+    from oopao_amd.calibration.CalibrationVault import CalibrationVault
+    D = _im_like(*shape, cond=8e3, seed=2)
+    cv = CalibrationVault(D, method=method, nTrunc=10)
+    assert cv.M_dev.is_cuda
+    U, s, Vh = np.linalg.svd(D, full_matrices=False)
+    pinv = np.linalg.pinv(D)
+    n = len(s) - 10
+    mtrunc = Vh[:n].T @ np.diag(1 / s[:n]) @ U[:, :n].T      # CalibrationVault.py:60-66
+    # tolerance: the SVD route at float64 rounding; the Gram route ~eps cond^2 = 1.4e-8 relative
+    tol = 1e-10 if method == 'svd' else 1e-7
+    em = np.abs(cv.M - pinv).max() / np.abs(pinv).max()
+    et = np.abs(cv.Mtrunc - mtrunc).max() / np.abs(mtrunc).max()
+    print('%s %s: M %.2e Mtrunc %.2e' % (method, shape, em, et))
+    assert em <= tol and et <= tol
+    assert np.allclose(cv.eigenValues, s, rtol=1e-9)
+    assert np.isclose(cv.cond, s[0] / s[-11], rtol=1e-9)
+    assert np.allclose(D @ cv.M @ D, D, atol=1e-9)
