@@ -144,6 +144,16 @@ int oopao_pwfs_slopes(const float* frames, int B, int cam, const unsigned char* 
                       const int* quad, const float* ref2d, float units, const int* vidx, int nsig,
                       int nvalid, void* workspace, size_t ws_bytes, float* sig2d, float* sig, void* stream);
 
+/* Allocation-free variant of oopao_pwfs_slopes (same arguments without the workspace, same
+ * results bit for bit). No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops). The partial sums live in a static
+ * device array of 65536 doubles (512 KiB per device, held until the process ends): calls with
+ * B * ceil(w*w / 256) above 65536 fail with ERR_WORKSPACE. Calls share that array on a device:
+ * calls that overlap on one device (other streams, other threads) race. */
+int oopao_pwfs_slopes_static(const float* frames, int B, int cam, const unsigned char* valid, int w,
+                             const int* quad, const float* ref2d, float units, const int* vidx, int nsig,
+                             int nvalid, float* sig2d, float* sig, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Deformable mirror (replaces DeformableMirror.modesComputation and the coefs
  * setter's modes @ coefs: DeformableMirror.py:572-600, 651-686)

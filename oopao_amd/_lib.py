@@ -30,6 +30,8 @@ SIGNATURES = {
     'oopao_pwfs_slopes_workspace_bytes': (c_int, [c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_slopes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_float,
                                   c_void_p, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),
+    'oopao_pwfs_slopes_static': (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_float,
+                                         c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     'oopao_pwfs_intermediate_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t),
                                               ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_rows': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,

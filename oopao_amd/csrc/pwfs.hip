@@ -437,6 +437,58 @@ static bool fast_supported(int n, int N, int bin) {
   return N == 4 * n && bin == 4 && fast_tpt(N) != 0;
 }
 
+static int check_slopes_args(const float* frames, int B, int cam, const unsigned char* valid, int w, const int* quad,
+                             const float* ref2d, const int* vidx, int nsig, int nvalid, const float* sig2d,
+                             const float* sig) {
+  OOPAO_CHECK_ARG(frames && valid && quad && ref2d && vidx && sig2d && sig, "null pointer argument");
+  OOPAO_CHECK_ARG(B > 0 && cam > 0 && w > 0 && nsig >= 0 && nvalid > 0,
+                  "invalid sizes B=%d cam=%d w=%d nsig=%d nvalid=%d", B, cam, w, nsig, nvalid);
+  OOPAO_CHECK_ARG(B <= 65535, "too many frames in one call: %d", B);
+  return OK;
+}
+
+// slopesMaps of B frames with the per-block valid-I4Q sums in part[B][nblk]
+static int launch_slopes(const float* frames, int B, int cam, const unsigned char* valid, int w, const int* quad,
+                         const float* ref2d, float units, const int* vidx, int nsig, int nvalid, double* part,
+                         float* sig2d, float* sig, hipStream_t st) {
+  const int nblk = (w * w + 255) / 256;
+  hipLaunchKernelGGL(k_slopes_partial, dim3(nblk, B), dim3(256), 0, st, frames, cam, valid, w, quad, nblk, part);
+  OOPAO_LAUNCH_CHECK();
+  const int nout = 2 * w * w + nsig;
+  hipLaunchKernelGGL(k_slopes_finish, dim3((nout + 255) / 256, B), dim3(256), 0, st, frames, cam, valid, w, quad,
+                     ref2d, units, vidx, nsig, nvalid, nblk, part, sig2d, sig);
+  OOPAO_LAUNCH_CHECK();
+  return OK;
+}
+
+// Static scratch of oopao_pwfs_slopes_static: the per-block partial sums of the valid I4Q
+// (k_slopes_partial writes every entry it covers before k_slopes_finish reads it, so nothing needs
+// clearing between calls). 64 Ki doubles = 512 KiB per device, held from the library's first use on a
+// device until the process ends. In host code `scratch` only names the variable: its device address,
+// one per device, comes from hipGetSymbolAddress.
+constexpr int kSlopesScratch = 65536;
+static __device__ __align__(256) double scratch[kSlopesScratch];
+
+static int slopes_scratch(double** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double* ptrs[kMaxDevices] = {};
+  int dev = 0;
+  OOPAO_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device %d outside the %d devices the static scratch supports", dev, kMaxDevices);
+    return ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (ptrs[dev] == nullptr) {
+    void* p = nullptr;
+    OOPAO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));   // with device `dev` current
+    ptrs[dev] = static_cast<double*>(p);
+  }
+  *out = ptrs[dev];
+  return OK;
+}
+
 }  // namespace oopao
 
 using namespace oopao;
@@ -594,24 +646,35 @@ int oopao_pwfs_slopes_workspace_bytes(int B, int w, size_t* bytes) {
 int oopao_pwfs_slopes(const float* frames, int B, int cam, const unsigned char* valid, int w,
                       const int* quad, const float* ref2d, float units, const int* vidx, int nsig,
                       int nvalid, void* workspace, size_t ws_bytes, float* sig2d, float* sig, void* stream) {
-  OOPAO_CHECK_ARG(frames && valid && quad && ref2d && vidx && sig2d && sig && workspace, "null pointer argument");
-  OOPAO_CHECK_ARG(B > 0 && cam > 0 && w > 0 && nsig >= 0 && nvalid > 0,
-                  "invalid sizes B=%d cam=%d w=%d nsig=%d nvalid=%d", B, cam, w, nsig, nvalid);
-  OOPAO_CHECK_ARG(B <= 65535, "too many frames in one call: %d", B);
+  OOPAO_CHECK_ARG(workspace != nullptr, "null pointer argument");
+  int rc = check_slopes_args(frames, B, cam, valid, w, quad, ref2d, vidx, nsig, nvalid, sig2d, sig);
+  if (rc) return rc;
   const int nblk = (w * w + 255) / 256;
   if (ws_bytes < (size_t)B * nblk * sizeof(double)) {
     set_error("workspace too small: %zu < %zu bytes", ws_bytes, (size_t)B * nblk * sizeof(double));
     return ERR_WORKSPACE;
   }
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  double* part = reinterpret_cast<double*>(workspace);
-  hipLaunchKernelGGL(k_slopes_partial, dim3(nblk, B), dim3(256), 0, st, frames, cam, valid, w, quad, nblk, part);
-  OOPAO_LAUNCH_CHECK();
-  const int nout = 2 * w * w + nsig;
-  hipLaunchKernelGGL(k_slopes_finish, dim3((nout + 255) / 256, B), dim3(256), 0, st, frames, cam, valid, w, quad,
-                     ref2d, units, vidx, nsig, nvalid, nblk, part, sig2d, sig);
-  OOPAO_LAUNCH_CHECK();
-  return OK;
+  return launch_slopes(frames, B, cam, valid, w, quad, ref2d, units, vidx, nsig, nvalid,
+                       reinterpret_cast<double*>(workspace), sig2d, sig, reinterpret_cast<hipStream_t>(stream));
+}
+
+int oopao_pwfs_slopes_static(const float* frames, int B, int cam, const unsigned char* valid, int w,
+                             const int* quad, const float* ref2d, float units, const int* vidx, int nsig,
+                             int nvalid, float* sig2d, float* sig, void* stream) {
+  int rc = check_slopes_args(frames, B, cam, valid, w, quad, ref2d, vidx, nsig, nvalid, sig2d, sig);
+  if (rc) return rc;
+  const long need = (long)B * ((w * w + 255) / 256);
+  if (need > kSlopesScratch) {
+    set_error("oopao_pwfs_slopes_static: %d frames of %d x %d quadrant pixels need %ld partial sums, more than "
+              "the %d of the static scratch; use oopao_pwfs_slopes with a workspace", B, w, w, need,
+              kSlopesScratch);
+    return ERR_WORKSPACE;
+  }
+  double* part = nullptr;
+  rc = slopes_scratch(&part);
+  if (rc) return rc;
+  return launch_slopes(frames, B, cam, valid, w, quad, ref2d, units, vidx, nsig, nvalid, part, sig2d, sig,
+                       reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"
