@@ -6,7 +6,9 @@ n_pix_separation=80, n_pix_edge=40, nTheta_user_defined=72) -> 1280^2 padded
 fields, 320^2 detector, slopesMaps signal. Input: a seeded synthetic OPD map
 (smooth random screen, fixed seed) resident in HBM before the timed region;
 output: the slope vector. The timed step runs the rows pass, the columns pass
-(+ chunk reduction) and the slopes kernel on one stream.
+(+ chunk reduction) and the slopes kernel on one stream, for the detector columns
+the slopesMaps signal reads (the quadrant bands, half of the 320 columns); the
+whole 320x320 frame (the same kernels over every column) is timed beside it.
 
 Prints one JSON line. ``--gpus N`` with N > 1 exits: this build runs on one GPU.
 """
@@ -23,7 +25,9 @@ sys.path.insert(0, ROOT)
 
 # config-2 geometry
 N_RES, N_SUBAP, SEP, EDGE, MOD, T_POINTS, DIAM, TS = 320, 80, 80, 40, 3, 72, 8.0, 1e-3
-VALU_PEAK_TFLOPS = 78.6    # FP32 vector FMA, non-packed (256 CU x 64 lanes x 2 x 2.4 GHz)
+FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 peak (packed FP32 issue)
+VALU_PEAK_TFLOPS = 78.6    # FP32 vector FMA without packed instructions (256 CU x 64 lanes x 2 x 2.4 GHz); the
+                           # build disables packed FP32 (-packed-fp32-ops), so this is the issue ceiling
 HBM_PEAK_GBS = 8000.0
 BF16_MFMA_PEAK_TFLOPS = 2500.0   # dense BF16 matrix-core peak
 
@@ -39,9 +43,21 @@ def synthetic_opd(n, seed=10, rms_nm=150.0):
     return scr
 
 
-def cpu_baseline(validI4Q, opd, frames=3, warmup=1):
+def _median_fps(fn, frames, warmup):
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(frames):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return 1.0 / float(np.median(ts))
+
+
+def cpu_baseline(validI4Q, opd, frames=5, warmup=2):
     """Reference-faithful numpy restatement (complex128, scipy fft2 / numpy ifft2, thread pool of
-    6 or 8 workers over modulation points like the reference's joblib loop) on host cores."""
+    6 or 8 workers over modulation points like the reference's joblib loop) on host cores: the median
+    of ``frames`` frames after ``warmup`` warm-up frames."""
     from oracle import pwfs as opw
     ncpu = os.cpu_count() or 1
     n_jobs = 8 if ncpu > 16 else 6
@@ -50,34 +66,21 @@ def cpu_baseline(validI4Q, opd, frames=3, warmup=1):
     o = opw.PyramidOracle(N_RES, N_SUBAP, MOD, 0.5, pup, flux, 0.8e-6, n_pix_separation=SEP,
                           n_pix_edge=EDGE, nTheta_user_defined=T_POINTS, precision='fp64', n_jobs=n_jobs,
                           validI4Q=validI4Q)
-    for _ in range(warmup):
-        o.measure_opd(opd)
-    t0 = time.perf_counter()
-    for _ in range(frames):
-        o.measure_opd(opd)
-    dt = time.perf_counter() - t0
+    fps = _median_fps(lambda: o.measure_opd(opd), frames, warmup)
     # the same oracle over every core of the process's affinity set
     n_all = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else ncpu
     if o._pool is not None:
         o._pool.shutdown()
     o.n_jobs = n_all
     o._pool = None
-    o.measure_opd(opd)
-    t1 = time.perf_counter()
-    for _ in range(frames):
-        o.measure_opd(opd)
-    dt_all = time.perf_counter() - t1
+    fps_all = _median_fps(lambda: o.measure_opd(opd), frames, 1)
     if o._pool is not None:
         o._pool.shutdown()
     # complex64 variant of the same restatement (the reference with precision 32), reference thread count
     o32 = opw.PyramidOracle(N_RES, N_SUBAP, MOD, 0.5, pup, flux, 0.8e-6, n_pix_separation=SEP,
                             n_pix_edge=EDGE, nTheta_user_defined=T_POINTS, precision='fp32', n_jobs=n_jobs,
                             validI4Q=validI4Q)
-    o32.measure_opd(opd)
-    t2 = time.perf_counter()
-    for _ in range(frames):
-        o32.measure_opd(opd)
-    dt32 = time.perf_counter() - t2
+    fps32 = _median_fps(lambda: o32.measure_opd(opd), frames, 1)
     if o32._pool is not None:
         o32._pool.shutdown()
     model = ''
@@ -87,11 +90,11 @@ def cpu_baseline(validI4Q, opd, frames=3, warmup=1):
     except OSError:
         pass
     env = {k: os.environ.get(k) for k in ('OMP_NUM_THREADS', 'OPENBLAS_NUM_THREADS', 'MKL_NUM_THREADS')}
-    return dict(value=frames / dt, unit='frames/s', cores=n_jobs, kind='port',
-                sample=f'{frames} full config-2 frames ({T_POINTS} points, 2 x 1280^2 complex128 FFTs each) '
-                       f'after {warmup} warm-up, {n_jobs} threads; host reports {ncpu} CPUs',
-                all_cores={'value': frames / dt_all, 'threads': n_all},
-                fp32={'value': frames / dt32, 'threads': n_jobs},
+    return dict(value=fps, unit='frames/s', cores=n_jobs, kind='port',
+                sample=f'median of {frames} full config-2 frames ({T_POINTS} points, 2 x 1280^2 complex128 FFTs '
+                       f'each) after {warmup} warm-up frames, {n_jobs} threads; host reports {ncpu} CPUs',
+                all_cores={'value': fps_all, 'threads': n_all},
+                fp32={'value': fps32, 'threads': n_jobs},
                 host={'cpu_count': ncpu, 'affinity': n_all, 'model': model, 'thread_env': env})
 
 
@@ -514,16 +517,17 @@ def contraction_leg(tel, dev, n_modes=1000):
     return out
 
 
-def measured_traffic(kernels):
-    """HBM bytes per launch of the given kernels from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by the copy-kernel calibration)."""
+def measured_traffic(kernels, workload):
+    """HBM bytes per launch of the given kernels of one workload ('slopes_only' or 'full_frame') from
+    the committed rocprofv3 --pmc summary of this bench (FETCH_SIZE and WRITE_SIZE in KiB per dispatch,
+    scaled by the copy-kernel calibration)."""
     path = os.path.join(ROOT, 'profiles', 'pwfs_config2_traffic.json')
     try:
         with open(path) as f:
             d = json.load(f)
         tot = 0.0
         for k in kernels:
-            e = d['kernels'][k]
+            e = d['workloads'][workload][k]
             tot += (e['fetch_kib'] * d['fetch_scale'] + e['write_kib'] * d['write_scale']) * 1024
         return int(tot)
     except (OSError, KeyError, ValueError):
@@ -535,7 +539,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=200)
     ap.add_argument('--warmup', type=int, default=10)
-    ap.add_argument('--cpu-frames', type=int, default=3)
+    ap.add_argument('--cpu-frames', type=int, default=5)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--batch', type=int, default=8, help='screens per call for the batched form')
     ap.add_argument('--no-extra', action='store_true',
@@ -574,70 +578,97 @@ def main():
     frame = torch.empty((1, eng.cam, eng.cam), dtype=torch.float32, device=dev)
     sig2d = torch.empty((1, 2 * se.w, se.w), dtype=torch.float32, device=dev)
     sig = torch.empty((1, se.nsig), dtype=torch.float32, device=dev)
+    # the slopes read only the quadrant columns of the detector (Pyramid.py:838-853, 946-961)
+    bands = wfs.slope_bands()
+    n_cols = sum(b - a for a, b in bands)
 
-    def step(ev=None):
-        if ev is not None:
-            ev[0].record()
-        eng.rows(opd, scale, amp, buf)
-        if ev is not None:
-            ev[1].record()
-        eng.cols(buf, frame)
-        if ev is not None:
-            ev[2].record()
-        se.slopes(frame, sig2d, sig)
+    def make_step(bd):
+        def step(ev=None):
+            if ev is not None:
+                ev[0].record()
+            eng.rows(opd, scale, amp, buf, bd)
+            if ev is not None:
+                ev[1].record()
+            eng.cols(buf, frame, bd)
+            if ev is not None:
+                ev[2].record()
+            se.slopes(frame, sig2d, sig)
+        return step
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
+    def timed(step, K):
+        """ms per step over K steps (bracketed by synchronisations) and the mean rows / columns
+        pass durations from HIP events on the kernels' stream."""
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
+        t0 = time.perf_counter()
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record()
+        for k in range(K):
+            step(evs[k])
+        end.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return (start.elapsed_time(end) / K, float(np.mean([e[0].elapsed_time(e[1]) for e in evs])),
+                float(np.mean([e[1].elapsed_time(e[2]) for e in evs])), wall)
+
     K = args.steps
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
-    t0 = time.perf_counter()
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    start.record()
-    for k in range(K):
-        step(evs[k])
-    end.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = start.elapsed_time(end) / K
-    rows_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
-    cols_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    step = make_step(bands)
+    ms_step, rows_ms, cols_ms, wall = timed(step, K)
     fps = 1000.0 / ms_step
+    sig_slopes_only = sig.clone()
+    # the whole frame: every detector column (same slopes bit for bit)
+    step_full = make_step(None)
+    ms_full, rows_full, cols_full, _ = timed(step_full, K)
+    full_equal = bool(torch.equal(sig, sig_slopes_only))
 
     # algorithmic work of the dominant kernel (columns pass + chunk reduction), per launch
     n, N, T, P = N_RES, eng.N, eng.T, eng.P
     log2N = np.log2(N)
-    cols_flops = N * T * 2 * 5 * N * log2N            # one pruned forward + one inverse N-FFT per column & point
-    # the per-point intermediates read once, the chunk partials written and read by the reduction,
-    # the mask vector and the frame written
-    cols_bytes = T * n * N * 8 + buf['nchunk'] * eng.cam ** 2 * 4 * 2 + N * 8 + eng.cam ** 2 * 4
-    rows_flops = P * n * 2 * 5 * N * log2N
+
+    def cols_work(ncol):
+        # one pruned forward + one inverse N-point FFT per pupil-plane column (4 per detector column) and
+        # point; bytes: the intermediate's columns read once, the chunk partials written and read by the
+        # reduction, the mask vector and the frame written
+        flops = 4 * ncol * T * 2 * 5 * N * log2N
+        nbytes = T * n * 4 * ncol * 8 + buf['nchunk'] * eng.cam * ncol * 4 * 2 + N * 8 + eng.cam ** 2 * 4
+        return flops, nbytes
+    cols_flops, cols_bytes = cols_work(n_cols)
     achieved_tf = cols_flops / (cols_ms * 1e-3) / 1e12
     achieved_gbs = cols_bytes / (cols_ms * 1e-3) / 1e9
+    full_flops, full_bytes = cols_work(eng.cam)
+    full_tf = full_flops / (cols_full * 1e-3) / 1e12
 
-    # batched form: B screens per call
+    # batched form: B screens per call (slopes only)
     B = args.batch
     opdB = opd.expand(B, N_RES, N_RES).contiguous()
     bufB = eng.buffers(B)
     frameB = torch.empty((B, eng.cam, eng.cam), dtype=torch.float32, device=dev)
     sig2dB = torch.empty((B, 2 * se.w, se.w), dtype=torch.float32, device=dev)
     sigB = torch.empty((B, se.nsig), dtype=torch.float32, device=dev)
+
+    def stepB():
+        eng.rows(opdB, scale, amp, bufB, bands)
+        eng.cols(bufB, frameB, bands)
+        se.slopes(frameB, sig2dB, sigB)
     for _ in range(3):
-        eng.rows(opdB, scale, amp, bufB); eng.cols(bufB, frameB); se.slopes(frameB, sig2dB, sigB)
+        stepB()
     torch.cuda.synchronize()
     s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     KB = max(3, K // B)
     s0.record()
     for _ in range(KB):
-        eng.rows(opdB, scale, amp, bufB); eng.cols(bufB, frameB); se.slopes(frameB, sig2dB, sigB)
+        stepB()
     s1.record()
     torch.cuda.synchronize()
     batched_fps = B * KB * 1000.0 / s0.elapsed_time(s1)
 
-    # the single-screen frame captured once in a HIP graph (the library's calls are stream-ordered and
-    # allocate nothing) and replayed K times; its slopes must equal the eager ones bit for bit
+    # the single-screen slopes-only frame captured once in a HIP graph (the library's calls are
+    # stream-ordered and allocate nothing) and replayed K times; its slopes must equal the eager ones
+    # bit for bit
     step()
     torch.cuda.synchronize()
     sig_eager = sig.clone()
@@ -697,17 +728,31 @@ def main():
         'vs_baseline': None,
         'dtype': 'fp32/complex64',
         'data': 'synthetic seeded OPD screen (150 nm RMS, seed 10), resident in HBM',
-        'config': {'workload': 'config-2 modulated PWFS: n=320, N=1280, T=72, 320x320 detector, slopesMaps',
+        'config': {'workload': 'config-2 modulated PWFS, OPD -> slopesMaps signal: n=320, N=1280, T=72, the %d '
+                               'detector columns of the quadrants computed (slopes-only frame; the whole 320x320 '
+                               'frame is full_frame)' % n_cols,
                    'n': N_RES, 'N': int(N), 'nTheta': int(T), 'nSubap': N_SUBAP, 'batch': 1,
-                   'nchunk': buf['nchunk'], 'pts_per_task': buf['ppt'], 'tip_tables': int(P)},
+                   'nchunk': buf['nchunk'], 'pts_per_task': buf['ppt'], 'tip_tables': int(P),
+                   'column_bands': [list(b) for b in bands]},
         'kernels_ms': {'rows_pass': round(rows_ms, 5), 'cols_pass_plus_reduce': round(cols_ms, 5)},
-        'roofline': {'bound': 'valu', 'kernel': 'k_cols_sep<1280,64> + k_reduce_chunks',
-                     'achieved': round(achieved_tf, 3), 'peak': VALU_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': round(achieved_tf / VALU_PEAK_TFLOPS, 4),
-                     'traffic': measured_traffic(['k_cols_sep<1280,64>', 'k_reduce_chunks']),
+        'roofline': {'bound': 'valu', 'kernel': 'k_cols_sep<1280,64> + k_reduce_chunks (slopes-only columns)',
+                     'achieved': round(achieved_tf, 3), 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': round(achieved_tf / FP32_PEAK_TFLOPS, 4),
+                     'frac_of_nonpacked_peak': round(achieved_tf / VALU_PEAK_TFLOPS, 4),
+                     'peak_note': 'peak = published MI355X FP32 157.3 TFLOP/s; the build disables packed FP32, '
+                                  'whose non-packed FMA ceiling is 78.6 TFLOP/s',
+                     'traffic': measured_traffic(['k_cols_sep<1280,64>', 'k_reduce_chunks'], 'slopes_only'),
+                     'traffic_source': 'profiles/pwfs_config2_traffic.json (rocprofv3 --pmc FETCH_SIZE / '
+                                       'WRITE_SIZE of this bench, copy-calibrated)',
                      'flops_per_launch': cols_flops, 'bytes_per_launch': cols_bytes,
                      'hbm_achieved_GBs': round(achieved_gbs, 1),
                      'hbm_frac': round(achieved_gbs / HBM_PEAK_GBS, 4)},
+        'full_frame': {'value': round(1000.0 / ms_full, 2), 'unit': 'frames/s', 'ms_per_step': round(ms_full, 5),
+                       'kernels_ms': {'rows_pass': round(rows_full, 5), 'cols_pass_plus_reduce': round(cols_full, 5)},
+                       'cols_achieved_TFLOPs': round(full_tf, 3),
+                       'cols_frac': round(full_tf / FP32_PEAK_TFLOPS, 4),
+                       'traffic': measured_traffic(['k_cols_sep<1280,64>', 'k_reduce_chunks'], 'full_frame'),
+                       'slopes_equal_slopes_only': full_equal},
         'batched': {'batch': B, 'value': round(batched_fps, 2), 'unit': 'frames/s'},
         'graph': {'value': round(1000.0 / graph_ms, 2), 'unit': 'frames/s', 'ms_per_step': round(graph_ms, 5),
                   'replays': K, 'slopes_equal_eager': graph_match},

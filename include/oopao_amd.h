@@ -36,7 +36,7 @@ const char* oopao_last_error(void);
  * ------------------------------------------------------------------------- */
 
 /* 1 if the separable fast path is compiled for this geometry: N = 4 n,
- * 4 x 4 detector binning, N in {64,128,256,320,512,640,1024,1280,1536,2048}. */
+ * 4 x 4 detector binning, N in {64,128,256,320,384,512,640,768,1024,1280,1536,1920,2048}. */
 int oopao_pwfs_supported(int n, int N, int bin);
 
 /* Workspace needed by oopao_pwfs_frames for B screens of T modulation points
@@ -60,11 +60,17 @@ int oopao_pwfs_workspace_bytes(int n, int N, int T, int B, int nchunk, size_t* b
  *          sum-binned (tools/tools.py:464-483)
  * pts_per_task: tip tables per rows-pass task; nchunk: point chunks of the
  * columns pass (both are tuning knobs, any valid value gives the same result
- * up to float summation order within a chunk). */
+ * up to float summation order within a chunk).
+ * bands: null for the whole frame, or a host array {a0, a1, b0, b1} of detector
+ * columns (0 <= a0 <= a1 <= b0 <= b1 <= N/4): only the columns in [a0, a1) and
+ * [b0, b1) are computed (bit for bit as in the whole frame), the others are
+ * written as zero. The slopesMaps signal reads only the quadrant columns
+ * [c-e-w, c-e) and [c+e, c+e+w) (Pyramid.py:838-853, 946-961), so a
+ * slopes-only measurement passes those and does half the work at config 2. */
 int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
                       const void* ty, const void* mx, const void* my, int n, int N, int T, int B, int bin,
-                      int nchunk, int pts_per_task, void* workspace, size_t ws_bytes, float* frames,
-                      void* stream);
+                      int nchunk, int pts_per_task, const int* bands, void* workspace, size_t ws_bytes,
+                      float* frames, void* stream);
 
 /* The two passes of oopao_pwfs_frames as separate calls (same arguments), for
  * callers that own the intermediate buffers or time the passes:
@@ -74,14 +80,17 @@ int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const voi
  *   cols pass : per detector column, IFFT_N(my . FFT_N(pad(column))) for every
  *               point, |.|^2 summed over the points of each of `nchunk` chunks
  *               and binned 4x4 into part[B][nchunk][cam][cam], then reduced over
- *               chunks into frames[B][cam][cam]. */
+ *               chunks into frames[B][cam][cam].
+ * inter_bytes / part_bytes: sizes of the caller's buffers, checked against
+ * oopao_pwfs_intermediate_bytes (ERR_WORKSPACE when smaller); bands as in
+ * oopao_pwfs_frames (the same bands for both passes). */
 int oopao_pwfs_intermediate_bytes(int n, int N, int T, int B, int nchunk, size_t* inter_bytes,
                                   size_t* part_bytes);
 int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
-                    const void* ty, const void* mx, int n, int N, int T, int B, int pts_per_task, void* inter,
-                    void* stream);
-int oopao_pwfs_cols(const void* inter, const void* my, int n, int N, int T, int B, int nchunk, float* part,
-                    float* frames, void* stream);
+                    const void* ty, const void* mx, int n, int N, int T, int B, int pts_per_task, const int* bands,
+                    void* inter, size_t inter_bytes, void* stream);
+int oopao_pwfs_cols(const void* inter, size_t inter_bytes, const void* my, int n, int N, int T, int B, int nchunk,
+                    const int* bands, float* part, size_t part_bytes, float* frames, void* stream);
 
 /* Modulation focal-plane camera (Pyramid.py:638-642 modulation_camera_em,
  * 1183-1190 intensity crop): for B screens,

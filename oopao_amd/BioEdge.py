@@ -19,7 +19,7 @@ import torch
 from ._device import to_host
 from .Detector import Detector
 from .Pyramid import Pyramid
-from .pwfs_engine import PWFSEngine, SlopesEngine
+from .pwfs_engine import PWFSEngine, SlopesEngine, Workspace
 from .tools import OopaoError, WorkspaceAllocationError, warning
 
 
@@ -134,7 +134,9 @@ class BioEdge(Pyramid):
         cam = self.cam.resolution
         if self.resolution % cam:
             raise OopaoError('detector size %d does not divide N=%d' % (cam, self.resolution))
-        self.engines = [PWFSEngine(n, self.resolution, cam, self.device) for _ in range(4)]
+        # the four masks run one after another on one stream: one shared workspace
+        ws = Workspace(self.device)
+        self.engines = [PWFSEngine(n, self.resolution, cam, self.device, workspace=ws) for _ in range(4)]
         self.engine = self.engines[0]
         self._last_opd = None
         self._last_weighted = False
@@ -319,10 +321,17 @@ class BioEdge(Pyramid):
         """sqrt(0.5) * sqrt(src.intensity / nTheta) (BioEdge.py:514-520, 567)."""
         return (np.sqrt(0.5) * torch.sqrt(self.src._intensity / self.nTheta)).contiguous()
 
-    def frames_dev(self, opd, weighted=False):
-        """Detector frames [B, 2 cam, 2 cam]: the four masks' binned images tiled 2 x 2."""
-        amp = self._amp()
-        scale = 2 * np.pi / self.src.wavelength
+    def slope_bands(self):
+        """The tiled four-mask frame is always computed whole."""
+        return None
+
+    def frames_dev(self, opd, weighted=False, bands=None, amp=None, scale=None):
+        """Detector frames [B, 2 cam, 2 cam]: the four masks' binned images tiled 2 x 2 (``bands`` must
+        be None: the Bi-O Edge frame is always whole)."""
+        if bands is not None:
+            raise OopaoError('column bands are not supported by the Bi-O Edge sensor')
+        amp = self._amp() if amp is None else amp
+        scale = 2 * np.pi / self.src.wavelength if scale is None else scale
         B = opd.shape[0]
         cam = self.cam.resolution
         out = torch.empty((B, 2 * cam, 2 * cam), dtype=torch.float32, device=self.device)
@@ -342,6 +351,8 @@ class BioEdge(Pyramid):
                     raise
                 bs = max(1, min(bs, B) // 2)
                 self.max_batch_bytes = bs * max(1, self.engine.bytes_per_screen())
+                self.engine.workspace.release()      # shared by the four engines
+                torch.cuda.empty_cache()
                 warning('%s; measuring %d screens at a time' % (err.string, bs))
 
     def _raw_dev(self, args, weighted):

@@ -14,6 +14,18 @@ from ._device import default_device, to_host
 from .tools import OopaoError, warning
 
 
+class LazyFrame:
+    """A detector frame computed on first use and kept."""
+
+    def __init__(self, fn):
+        self._fn, self._value = fn, None
+
+    def __call__(self):
+        if self._fn is not None:
+            self._value, self._fn = self._fn(), None
+        return self._value
+
+
 class Detector:
     def __init__(self, nRes: int = None, integrationTime: float = None, bits: int = None,
                  output_precision: int = None, FWC: int = None, gain: int = 1, sensor: str = 'CCD',
@@ -43,6 +55,7 @@ class Detector:
         self.log_scale = log_scale
         self._frame = None if nRes is None else torch.zeros((nRes, nRes), dtype=torch.float32,
                                                               device=self.device)
+        self._lazy = None
         self.saturation = 0
         self.tag = 'detector'
         self.buffer_frame = []
@@ -64,15 +77,56 @@ class Detector:
 
     @property
     def frame(self):
-        f = to_host(self._frame)
+        f = to_host(self._resolved())
         if f is not None and getattr(self, '_integer_output', False):
             f = f.astype(self.output_precision)
         return f
 
     @frame.setter
     def frame(self, val):
+        self._lazy = None
         self._frame = None if val is None else torch.as_tensor(np.asarray(val), dtype=torch.float32,
                                                                 device=self.device)
+
+    @property
+    def perfect_frame(self):
+        self._resolved()
+        return self._perfect_frame
+
+    @perfect_frame.setter
+    def perfect_frame(self, val):
+        self._perfect_frame = val
+
+    # A slopes-only measurement (Pyramid.wfs_measure) integrates a frame whose columns outside the
+    # slopesMaps quadrants are zero, and hands the detector a function that computes the whole frame;
+    # reading `frame` (or `perfect_frame`) runs it once. Only allowed when the readout leaves the frame
+    # unchanged, so that the whole frame is exactly what integrate() would have produced.
+    def identity_readout(self):
+        """True when integrate + readout return the input frame unchanged (no noise, no integration
+        over several frames, no saturation, binning, gain, quantisation, log scale or background)."""
+        return (not self.photonNoise and self.backgroundNoise is not True and self.QE == 1
+                and self.integrationTime is None and self.darkCurrent == 0 and self.FWC is None
+                and self.sensor != 'EMCCD' and self.binning == 1 and self.readoutNoise == 0
+                and not (self.sensor in ('CCD', 'CMOS') and self.gain != 1) and self.bits is None
+                and not self.log_scale and self.backgroundMap is None and not self._integer_output)
+
+    def _defer_full(self, fn):
+        self._lazy = LazyFrame(fn)
+
+    def _resolved(self):
+        """The current frame, computing the whole frame first if the last one was slopes-only."""
+        if self._lazy is not None:
+            lazy, self._lazy = self._lazy, None
+            self._frame = lazy()
+            self._perfect_frame = self._frame
+        return self._frame
+
+    def _frame_getter(self):
+        """A function returning the current whole frame later (after other frames were integrated)."""
+        if self._lazy is not None:
+            return self._lazy
+        f = self._frame
+        return lambda: f
 
     def relay(self, src):
         """Detector.relay (Detector.py:148-171): image of the source through its telescope."""
@@ -92,6 +146,7 @@ class Detector:
 
     def integrate(self, frame):
         """Detector.integrate + readout (Detector.py:273-354) for a device frame (or a batch)."""
+        self._lazy = None
         self.perfect_frame = frame
         f = frame
         if self.photonNoise != 0:
@@ -177,10 +232,10 @@ class Detector:
         with this frame, the next ones compute the optical gains."""
         if getattr(obj, 'tag', None) == 'GSC':
             if obj.calibration_ready is False:
-                obj.calibration(self._frame)
+                obj.calibration(self._resolved())
                 obj.detector_properties = {'resolution': f"{'Resolution [px]':<25s}|{int(self.resolution):^9d}"}
             else:
-                obj.compute_optical_gains(self._frame)
+                obj.compute_optical_gains(self._resolved())
         else:
             raise OopaoError(f'Coupled object should be a "GSC" but is {getattr(obj, "tag", type(obj).__name__)}')
 

@@ -458,21 +458,32 @@ class Pyramid:
         per = max(1, self.engine.bytes_per_screen())
         return max(1, min(B, int(self.max_batch_bytes // per)))
 
-    def frames_dev(self, opd, weighted=False):
+    def slope_bands(self):
+        """Detector-column intervals the slopesMaps signal reads (the quadrants of grabQuadrant,
+        Pyramid.py:838-853, 946-961), for a slopes-only frame; None when a measurement needs the whole
+        frame (another post-processing mode, the general path, detector binning)."""
+        if (self.postProcessing != 'slopesMaps' or getattr(self, 'engine', None) is None
+                or self.engine.mode != 'fast' or self.binning != 1):
+            return None
+        quad, w = self._quadrant_origins()
+        return [(c, c + w) for c in sorted(set(quad[4:]))]
+
+    def frames_dev(self, opd, weighted=False, bands=None, amp=None, scale=None):
         """Detector frames for device OPD maps [B, n, n] (metres) of the current source. ``weighted``
-        applies weight_vector to the modulation points (the 2-D measurement, Pyramid.py:750-755)."""
-        amp = self._amp()
-        scale = 2 * np.pi / self.src.wavelength
+        applies weight_vector to the modulation points (the 2-D measurement, Pyramid.py:750-755);
+        ``bands`` restricts the frames to those detector columns (see slope_bands)."""
+        amp = self._amp() if amp is None else amp
+        scale = 2 * np.pi / self.src.wavelength if scale is None else scale
         B = opd.shape[0]
         bs = self._batch_size(B)
         while True:
             try:
                 if bs >= B:
-                    return self.engine.frames(opd.contiguous(), scale, amp, weighted=weighted)
+                    return self.engine.frames(opd.contiguous(), scale, amp, weighted=weighted, bands=bands)
                 out = torch.empty((B, self.engine.cam, self.engine.cam), dtype=torch.float32, device=self.device)
                 for j in range(0, B, bs):
                     self.engine.frames(opd[j:j + bs].contiguous(), scale, amp, out=out[j:j + bs],
-                                       weighted=weighted)
+                                       weighted=weighted, bands=bands)
                 return out
             except WorkspaceAllocationError as e:
                 # the reference frees memory by measuring fewer screens at a time (Pyramid.py:713-740):
@@ -496,8 +507,16 @@ class Pyramid:
         opd = self.src._OPD
         batched = opd.dim() == 3
         weighted = (not batched) and self._weight_vector is not None
-        frames = self.frames_dev(opd if batched else opd[None], weighted=weighted)
-        self._last_opd = (opd if batched else opd[None], self.src.wavelength, self._amp())
+        opd_b = opd if batched else opd[None]
+        amp = self._amp()
+        scale = 2 * np.pi / self.src.wavelength
+        # slope-only frame: when the signal is all this measurement produces and the camera readout is
+        # an identity, compute only the quadrant columns; cam.frame computes the whole frame when read
+        bands = None
+        if integrate and self.isInitialized and self.isCalibrated and self.cam.identity_readout():
+            bands = self.slope_bands()
+        frames = self.frames_dev(opd_b, weighted=weighted, bands=bands, amp=amp, scale=scale)
+        self._last_opd = (opd_b, self.src.wavelength, amp)
         self._last_weighted = weighted
         self._raw_cache = None
         if self.binning != 1:
@@ -506,6 +525,13 @@ class Pyramid:
             frames = frames.reshape(-1, c // b, b, c // b, b).sum(dim=(2, 4))
         self.cam._integrated_time += self.telescope.samplingTime
         self.cam.integrate(frames if batched else frames[0])
+        if bands is not None:
+            opd_keep = opd_b.clone()
+
+            def whole_frame():
+                f = self.frames_dev(opd_keep, weighted=weighted, amp=amp, scale=scale)
+                return f if batched else f[0]
+            self.cam._defer_full(whole_frame)
         if integrate and self.isInitialized and self.isCalibrated:
             s2d, s = self._signal_processing_dev()
             self._signal_2D, self._signal = s2d, s
@@ -726,12 +752,20 @@ class Pyramid:
             self.wfs_measure()
             sig2d.append(self._signal_2D)
             sig.append(self._signal)
-            frames.append(self.cam._frame)
+            frames.append(self.cam._frame_getter())
         if len(src_list) > 1:
             self._signal_2D = torch.stack(sig2d) if sig2d[0] is not None else None
             self._signal = torch.stack(sig) if sig[0] is not None else None
         self._stacked_sources = len(src_list) > 1
-        self.frames = frames
+        self._frame_getters = frames
+
+    @property
+    def frames(self):
+        """Camera frames of the sources of the last relay (Pyramid.py:1170-1175), host float64."""
+        getters = getattr(self, '_frame_getters', None)
+        if not getters:
+            return None
+        return np.squeeze(np.array([to_host(g()) for g in getters]))
 
     def focal_intensity_dev(self, window):
         """Sum over modulation points (and over the screens of a batched measurement) of the focal-plane
@@ -766,7 +800,7 @@ class Pyramid:
                                 'the binning value requested:' + str(self.binning) + '! -- Ignoring the binning.')
                 obj.integrate(frame)
                 return
-            obj.integrate(self.cam._frame)
+            obj.integrate(self.cam._resolved())
         else:
             raise OopaoError('Error light propagated to the wrong type of object')
 

@@ -25,8 +25,8 @@ SIGNATURES = {
     'oopao_pwfs_supported': (c_int, [c_int, c_int, c_int]),
     'oopao_pwfs_workspace_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_frames': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
-                                  c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_size_t,
-                                  c_void_p, c_void_p]),
+                                  c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                  c_size_t, c_void_p, c_void_p]),
     'oopao_pwfs_slopes_workspace_bytes': (c_int, [c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_slopes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_float,
                                   c_void_p, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),
@@ -35,9 +35,9 @@ SIGNATURES = {
     'oopao_pwfs_intermediate_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t),
                                               ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_rows': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,
-                                c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
-    'oopao_pwfs_cols': (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
-                                c_void_p]),
+                                c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    'oopao_pwfs_cols': (c_int, [c_void_p, c_size_t, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p,
+                                c_void_p, c_size_t, c_void_p, c_void_p]),
     'oopao_pwfs_focal_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_focal': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int,
                                  c_int, c_int, c_int, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p]),

@@ -66,9 +66,11 @@ class PWFSAutotuner:
         s2d = torch.empty((B, 2 * se.w, se.w), dtype=torch.float32, device=wfs.device)
         s = torch.empty((B, se.nsig), dtype=torch.float32, device=wfs.device)
 
+        bands = wfs.slope_bands()
+
         def run():
-            e.rows(opd[:B], scale, amp, buf)
-            e.cols(buf, frames)
+            e.rows(opd[:B], scale, amp, buf, bands)
+            e.cols(buf, frames, bands)
             se.slopes(frames, s2d, s)
         return _event_ms(run, reps) / B
 
@@ -161,6 +163,7 @@ class CalibrationSystem:
                 self.bufs[b2] = e.buffers(b2)
                 se.slopes(self.frames[:b2], self.s2d[:b2], self.s[:b2])   # grows the slopes workspace now
         self.neg_pupil = (-self.pupil).contiguous()
+        self.bands = self.wfs.slope_bands()     # only the slopes are kept: quadrant columns of the frames
         self.half_over_stroke = 0.5 / self.stroke
 
     def record(self):
@@ -173,8 +176,8 @@ class CalibrationSystem:
             torch.mul(opd, self.pupil, out=self.opd2[:b])
             torch.mul(opd, self.neg_pupil, out=self.opd2[b:2 * b])
             buf = self.bufs[2 * b]
-            e.rows(self.opd2[:2 * b], self.scale, self.amp, buf)
-            e.cols(buf, self.frames[:2 * b])
+            e.rows(self.opd2[:2 * b], self.scale, self.amp, buf, self.bands)
+            e.cols(buf, self.frames[:2 * b], self.bands)
             se.slopes(self.frames[:2 * b], self.s2d[:2 * b], self.s[:2 * b])
             torch.sub(self.s[:b], self.s[b:2 * b], out=self.D[i0:i1])
             self.D[i0:i1].mul_(self.half_over_stroke)

@@ -60,6 +60,7 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
     scale = 2 * np.pi / ngs.wavelength
     eng, se = wfs.engine, wfs.slopes_engine
     buf = eng.buffers(1)
+    bands = wfs.slope_bands()         # the loop reads only the slopes: quadrant columns of the frame
     if strehl_every:
         from .science import psf_dev, strehl_dev
         amp_psf = (torch.sqrt(ngs._intensity) * pupil).float().contiguous()
@@ -84,8 +85,8 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
         if strehl_every and i % strehl_every == 0:
             psf = psf_dev(amp_psf, opd, psf_scale, tel.resolution, psf_sampling, None, dev)
             sr[i // strehl_every] = strehl_dev(psf, psf_ref, tel.resolution, psf_sampling, as_tensor=True)
-        eng.rows(opd[None], scale, amp, buf)
-        eng.cols(buf, frame)
+        eng.rows(opd[None], scale, amp, buf, bands)
+        eng.cols(buf, frame, bands)
         se.slopes(frame, s2d, s_cur)
         if record_slopes:
             slopes[i] = s_cur[0]

@@ -32,6 +32,22 @@
 
 namespace oopao {
 
+// Detector columns a frame computes: [a0, a1) and [b0, b1), 0 <= a0 <= a1 <= b0 <= b1 <= cam. The
+// slopesMaps signal reads only the quadrant bands of the camera (Pyramid.py:838-853, 946-961: columns
+// [c-e-w, c-e) and [c+e, c+e+w)), so a slopes-only frame transforms those detector columns (4 pupil-
+// plane columns each) and no others; the full frame is the band [0, cam).
+struct Bands {
+  int a0, a1, b0, b1;
+};
+__device__ __forceinline__ bool in_bands(int g, Bands bd) {
+  return (g >= bd.a0 && g < bd.a1) || (g >= bd.b0 && g < bd.b1);
+}
+// the gi-th computed detector column
+__device__ __forceinline__ int band_col(int gi, Bands bd) {
+  const int na = bd.a1 - bd.a0;
+  return gi < na ? bd.a0 + gi : bd.b0 + (gi - na);
+}
+
 // Pupil-plane field u[b][y][x] = amp[y][x] exp(i scale opd[b][y][x]), once per
 // screen (the reference re-forms it per modulation point, Pyramid.py:619-620).
 __global__ void k_field(const float* __restrict__ opd, float scale, const float* __restrict__ amp,
@@ -50,14 +66,16 @@ __global__ void k_field(const float* __restrict__ opd, float scale, const float*
 // Rows pass. One lane group (TPT lanes) per pupil row y; the row's field is
 // loaded once and reused for a chunk of tip tables tx_p; each transformed row
 // is written once per point of the table's group, times that point's tilt
-// phasor. FWD_ONLY stops after the forward transform (focal-plane field rows,
-// no mask).
+// phasor. With BANDS only the pupil-plane columns of the detector columns in
+// `bd` are written (a separate instantiation: the whole-frame kernel carries no
+// column tests). FWD_ONLY stops after the forward transform (focal-plane field
+// rows, no mask).
 // ---------------------------------------------------------------------------
-template <int N, int TPT, bool FWD_ONLY = false>
+template <int N, int TPT, bool FWD_ONLY = false, bool BANDS = false>
 __global__ void __launch_bounds__(64 * kWaves)
 k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, const int* __restrict__ pstart,
            const float2* __restrict__ ty, const float2* __restrict__ mx, const float2* __restrict__ tw,
-           float2* __restrict__ ws, int B, int P, int T, int pts_per_task) {
+           float2* __restrict__ ws, int B, int P, int T, int pts_per_task, Bands bd) {
   constexpr int n = N / 4;
   constexpr int E = N / TPT;
   constexpr int G = 64 / TPT;
@@ -103,10 +121,15 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
       for (int k = k0; k < k1; ++k) {
         const float2 tyv = ty[(size_t)k * n + y];
         float2* dst = ws + ((size_t)b * T + k) * (size_t)(N / 4) * n * 4;
+        // band limits through an opaque zero: the column tests stay here instead of being hoisted out
+        // of both loops as E lane masks (which cost a wave per SIMD of occupancy)
+        int z = 0;
+        asm volatile("" : "+s"(z));
+        const Bands bk{bd.a0 + z, bd.a1 + z, bd.b0 + z, bd.b1 + z};
 #pragma unroll
         for (int m = 0; m < E; ++m) {
           const int X = t + TPT * m;
-          dst[((size_t)(X >> 2) * n + y) * 4 + (X & 3)] = cmul(v[m], tyv);
+          if (!BANDS || in_bands(X >> 2, bk)) dst[((size_t)(X >> 2) * n + y) * 4 + (X & 3)] = cmul(v[m], tyv);
         }
       }
     }
@@ -123,12 +146,13 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
 //    L1/L2 and shared; the column sums are combined through LDS.
 //  * narrower transforms: one wavefront per (b, chunk, g), its 64/TPT lane
 //    groups taking the columns.
+// Only the detector columns in `bd` are computed (one task per such column).
 // Output: part[b][chunk][g][r] (detector column-major partial sums).
 // ---------------------------------------------------------------------------
 template <int N, int TPT>
 __global__ void __launch_bounds__(64 * kWaves, fast_cols_min_waves(N))
 k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
-           const float2* __restrict__ tw, float* __restrict__ part, int B, int T, int nchunk) {
+           const float2* __restrict__ tw, float* __restrict__ part, int B, int T, int nchunk, Bands bd) {
   constexpr int n = N / 4;
   constexpr int E = N / TPT;
   constexpr int G = 64 / TPT;
@@ -140,9 +164,10 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   const int lane = threadIdx.x & 63;
   const int h = lane / TPT;
   const int t = lane % TPT;
+  const int ng = (bd.a1 - bd.a0) + (bd.b1 - bd.b0);
   int task = kWaveCols ? (int)blockIdx.x : (int)blockIdx.x * kWaves + wave;
-  const int g = task % NG;
-  task /= NG;
+  const int g = band_col(task % ng, bd);
+  task /= ng;
   const int chunk = task % nchunk;
   const int b = task / nchunk;
   if (b >= B) return;  // uniform over the wavefront (and over the workgroup when kWaveCols)
@@ -265,8 +290,9 @@ __global__ void k_sum_chunks(const float* __restrict__ part, float* __restrict__
 }
 
 // frame[b][r][g] = sum_k part[b][k][g][r]  (fixed summation order)
+// (columns outside the bands `bd` are set to zero)
 __global__ void k_reduce_chunks(const float* __restrict__ part, float* __restrict__ frame, int B,
-                                int nchunk, int cam) {
+                                int nchunk, int cam, Bands bd) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t total = (size_t)B * cam * cam;
   if (idx >= total) return;
@@ -274,7 +300,8 @@ __global__ void k_reduce_chunks(const float* __restrict__ part, float* __restric
   const int g = (int)((idx / cam) % cam);
   const int b = (int)(idx / ((size_t)cam * cam));
   float s = 0.f;
-  for (int k = 0; k < nchunk; ++k) s += part[(((size_t)b * nchunk + k) * cam + g) * cam + r];
+  if (in_bands(g, bd))
+    for (int k = 0; k < nchunk; ++k) s += part[(((size_t)b * nchunk + k) * cam + g) * cam + r];
   frame[((size_t)b * cam + r) * cam + g] = s;
 }
 
@@ -365,7 +392,7 @@ static size_t ws_part_bytes(int N, int B, int nchunk) {
 
 template <int N, bool FWD_ONLY = false>
 static int launch_rows(const float* opd, float scale, const float* amp, const float2* tx, int P, const int* pstart,
-                       const float2* ty, const float2* mx, int T, int B, int pts_per_task, float2* inter,
+                       const float2* ty, const float2* mx, int T, int B, int pts_per_task, Bands bd, float2* inter,
                        hipStream_t st) {
   constexpr int TPT = fast_tpt_rows(N);
   constexpr int n = N / 4;
@@ -384,30 +411,36 @@ static int launch_rows(const float* opd, float scale, const float* amp, const fl
     OOPAO_LAUNCH_CHECK();
   }
   dim3 grid((unsigned)(nrb * npc * B));
-  hipLaunchKernelGGL((k_rows_sep<N, TPT, FWD_ONLY>), grid, dim3(64 * kWaves), 0, st, field, tx, pstart, ty, mx, tw,
-                     inter, B, P, T, pts_per_task);
+  const bool all = bd.a0 == 0 && bd.a1 == n && bd.b0 == n && bd.b1 == n;
+  if (FWD_ONLY || all)
+    hipLaunchKernelGGL((k_rows_sep<N, TPT, FWD_ONLY, false>), grid, dim3(64 * kWaves), 0, st, field, tx, pstart, ty,
+                       mx, tw, inter, B, P, T, pts_per_task, bd);
+  else
+    hipLaunchKernelGGL((k_rows_sep<N, TPT, false, true>), grid, dim3(64 * kWaves), 0, st, field, tx, pstart, ty, mx,
+                       tw, inter, B, P, T, pts_per_task, bd);
   OOPAO_LAUNCH_CHECK();
   return OK;
 }
 
 template <int N>
-static int launch_cols(const float2* inter, const float2* my, int T, int B, int nchunk, float* part,
+static int launch_cols(const float2* inter, const float2* my, int T, int B, int nchunk, Bands bd, float* part,
                        float* frames, hipStream_t st) {
   constexpr int TPT = fast_tpt(N);
   const float2* tw = nullptr;
   int rc = twiddles(N, &tw);
   if (rc) return rc;
   {
-    const long tasks = (long)B * nchunk * (N / 4);
+    const long tasks = (long)B * nchunk * ((bd.a1 - bd.a0) + (bd.b1 - bd.b0));
     dim3 grid((unsigned)(TPT == 64 ? tasks : (tasks + kWaves - 1) / kWaves));
-    hipLaunchKernelGGL((k_cols_sep<N, TPT>), grid, dim3(64 * kWaves), 0, st, inter, my, tw, part, B, T, nchunk);
+    hipLaunchKernelGGL((k_cols_sep<N, TPT>), grid, dim3(64 * kWaves), 0, st, inter, my, tw, part, B, T, nchunk,
+                       bd);
     OOPAO_LAUNCH_CHECK();
   }
   {
     const int cam = N / 4;
     const size_t total = (size_t)B * cam * cam;
     hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, part, frames,
-                       B, nchunk, cam);
+                       B, nchunk, cam, bd);
     OOPAO_LAUNCH_CHECK();
   }
   return OK;
@@ -435,6 +468,21 @@ static int launch_focal(const float2* inter, int T, int B, int nchunk, int crop,
 
 static bool fast_supported(int n, int N, int bin) {
   return N == 4 * n && bin == 4 && fast_tpt(N) != 0;
+}
+
+// Column bands from the C-ABI argument: null = every detector column.
+static int read_bands(const int* bands, int cam, Bands* out) {
+  if (bands == nullptr) {
+    *out = Bands{0, cam, cam, cam};
+    return OK;
+  }
+  const Bands b{bands[0], bands[1], bands[2], bands[3]};
+  OOPAO_CHECK_ARG(0 <= b.a0 && b.a0 <= b.a1 && b.a1 <= b.b0 && b.b0 <= b.b1 && b.b1 <= cam &&
+                      (b.a1 - b.a0) + (b.b1 - b.b0) > 0,
+                  "invalid column bands [%d, %d) [%d, %d) for %d detector columns (need 0 <= a0 <= a1 <= b0 <= b1 "
+                  "<= cam and at least one column)", b.a0, b.a1, b.b0, b.b1, cam);
+  *out = b;
+  return OK;
 }
 
 static int check_slopes_args(const float* frames, int B, int cam, const unsigned char* valid, int w, const int* quad,
@@ -525,13 +573,20 @@ int oopao_pwfs_intermediate_bytes(int n, int N, int T, int B, int nchunk, size_t
 }
 
 int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
-                    const void* ty, const void* mx, int n, int N, int T, int B, int pts_per_task, void* inter,
-                    void* stream) {
+                    const void* ty, const void* mx, int n, int N, int T, int B, int pts_per_task, const int* bands,
+                    void* inter, size_t inter_bytes, void* stream) {
   OOPAO_CHECK_ARG(opd && amp && tx && pstart && ty && mx && inter, "null pointer argument");
   OOPAO_CHECK_ARG(P > 0 && P <= T && B > 0 && pts_per_task > 0, "invalid P=%d T=%d B=%d pts_per_task=%d", P, T, B,
                   pts_per_task);
   int rc = check_geometry(n, N, 4);
   if (rc) return rc;
+  Bands bd;
+  rc = read_bands(bands, N / 4, &bd);
+  if (rc) return rc;
+  if (inter_bytes < ws_inter_bytes(n, N, T, B)) {
+    set_error("intermediate buffer too small: %zu < %zu bytes", inter_bytes, ws_inter_bytes(n, N, T, B));
+    return ERR_WORKSPACE;
+  }
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const float2* txp = reinterpret_cast<const float2*>(tx);
   const float2* typ = reinterpret_cast<const float2*>(ty);
@@ -540,7 +595,7 @@ int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void*
   switch (N) {
 #define OOPAO_CASE(NN) \
   case NN:             \
-    return launch_rows<NN>(opd, scale, amp, txp, P, pstart, typ, mxp, T, B, pts_per_task, ip, st);
+    return launch_rows<NN>(opd, scale, amp, txp, P, pstart, typ, mxp, T, B, pts_per_task, bd, ip, st);
     OOPAO_FAST_SIZES(OOPAO_CASE)
 #undef OOPAO_CASE
     default:
@@ -550,19 +605,27 @@ int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void*
   return ERR_UNSUPPORTED;
 }
 
-int oopao_pwfs_cols(const void* inter, const void* my, int n, int N, int T, int B, int nchunk, float* part,
-                    float* frames, void* stream) {
+int oopao_pwfs_cols(const void* inter, size_t inter_bytes, const void* my, int n, int N, int T, int B, int nchunk,
+                    const int* bands, float* part, size_t part_bytes, float* frames, void* stream) {
   OOPAO_CHECK_ARG(inter && my && part && frames, "null pointer argument");
   OOPAO_CHECK_ARG(T > 0 && B > 0 && nchunk > 0 && nchunk <= T, "invalid T=%d B=%d nchunk=%d", T, B, nchunk);
   int rc = check_geometry(n, N, 4);
   if (rc) return rc;
+  Bands bd;
+  rc = read_bands(bands, N / 4, &bd);
+  if (rc) return rc;
+  if (inter_bytes < ws_inter_bytes(n, N, T, B) || part_bytes < ws_part_bytes(N, B, nchunk)) {
+    set_error("buffers too small: intermediate %zu < %zu or partial frames %zu < %zu bytes", inter_bytes,
+              ws_inter_bytes(n, N, T, B), part_bytes, ws_part_bytes(N, B, nchunk));
+    return ERR_WORKSPACE;
+  }
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const float2* ip = reinterpret_cast<const float2*>(inter);
   const float2* myp = reinterpret_cast<const float2*>(my);
   switch (N) {
 #define OOPAO_CASE(NN) \
   case NN:             \
-    return launch_cols<NN>(ip, myp, T, B, nchunk, part, frames, st);
+    return launch_cols<NN>(ip, myp, T, B, nchunk, bd, part, frames, st);
     OOPAO_FAST_SIZES(OOPAO_CASE)
 #undef OOPAO_CASE
     default:
@@ -574,23 +637,23 @@ int oopao_pwfs_cols(const void* inter, const void* my, int n, int N, int T, int 
 
 int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
                       const void* ty, const void* mx, const void* my, int n, int N, int T, int B, int bin, int nchunk,
-                      int pts_per_task, void* workspace, size_t ws_bytes, float* frames, void* stream) {
+                      int pts_per_task, const int* bands, void* workspace, size_t ws_bytes, float* frames,
+                      void* stream) {
   OOPAO_CHECK_ARG(opd && amp && tx && pstart && ty && mx && my && workspace && frames, "null pointer argument");
   OOPAO_CHECK_ARG(P > 0 && P <= T && B > 0 && nchunk > 0 && nchunk <= T && pts_per_task > 0,
                   "invalid P=%d T=%d B=%d nchunk=%d pts_per_task=%d", P, T, B, nchunk, pts_per_task);
   int rc = check_geometry(n, N, bin);
   if (rc) return rc;
-  size_t need = ws_inter_bytes(n, N, T, B) + ws_part_bytes(N, B, nchunk);
-  if (ws_bytes < need) {
-    set_error("workspace too small: %zu < %zu bytes", ws_bytes, need);
+  const size_t ib = ws_inter_bytes(n, N, T, B), pb = ws_part_bytes(N, B, nchunk);
+  if (ws_bytes < ib + pb) {
+    set_error("workspace too small: %zu < %zu bytes", ws_bytes, ib + pb);
     return ERR_WORKSPACE;
   }
   char* w = reinterpret_cast<char*>(workspace);
-  void* inter = w;
-  float* part = reinterpret_cast<float*>(w + ws_inter_bytes(n, N, T, B));
-  rc = oopao_pwfs_rows(opd, scale, amp, tx, P, pstart, ty, mx, n, N, T, B, pts_per_task, inter, stream);
+  float* part = reinterpret_cast<float*>(w + ib);
+  rc = oopao_pwfs_rows(opd, scale, amp, tx, P, pstart, ty, mx, n, N, T, B, pts_per_task, bands, w, ib, stream);
   if (rc) return rc;
-  return oopao_pwfs_cols(inter, my, n, N, T, B, nchunk, part, frames, stream);
+  return oopao_pwfs_cols(w, ib, my, n, N, T, B, nchunk, bands, part, pb, frames, stream);
 }
 
 int oopao_pwfs_focal_bytes(int n, int N, int T, int B, int nchunk, int crop, size_t* bytes) {
@@ -624,7 +687,8 @@ int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void
   switch (N) {
 #define OOPAO_CASE(NN)                                                                                     \
   case NN:                                                                                                 \
-    rc = launch_rows<NN, true>(opd, scale, amp, txp, P, pstart, typ, txp, T, B, pts_per_task, inter, st);  \
+    rc = launch_rows<NN, true>(opd, scale, amp, txp, P, pstart, typ, txp, T, B, pts_per_task,              \
+                               Bands{0, NN / 4, NN / 4, NN / 4}, inter, st);                              \
     if (rc) return rc;                                                                                     \
     return launch_focal<NN>(inter, T, B, nchunk, crop, part, out, st);
     OOPAO_FAST_SIZES(OOPAO_CASE)

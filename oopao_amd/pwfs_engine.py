@@ -79,6 +79,25 @@ def tip_groups(path, tol=1e-12):
     return order, pidx, np.asarray(reps)
 
 
+def column_bands(bands, cam):
+    """ctypes int[4] {a0, a1, b0, b1} for the C-ABI (None: every detector column). ``bands`` is a
+    sequence of (start, stop) detector-column intervals; they are sorted and merged."""
+    if bands is None:
+        return None
+    iv = sorted((int(a), int(b)) for a, b in bands if int(b) > int(a))
+    merged = []
+    for a, b in iv:
+        if merged and a <= merged[-1][1]:
+            merged[-1] = (merged[-1][0], max(merged[-1][1], b))
+        else:
+            merged.append((a, b))
+    if not merged or len(merged) > 2 or merged[0][0] < 0 or merged[-1][1] > cam:
+        raise OopaoError('column bands %s: need one or two intervals inside [0, %d)' % (list(bands), cam))
+    if len(merged) == 1:
+        merged.append((cam, cam))
+    return (ctypes.c_int * 4)(merged[0][0], merged[0][1], merged[1][0], merged[1][1])
+
+
 def mask_vectors(a, b, N, n):
     """my[N], mx[N] complex128 from the separable mask phase (see module docstring)."""
     c0 = N // 2 - n // 2
@@ -87,10 +106,32 @@ def mask_vectors(a, b, N, n):
     return np.exp(1j * a) * shift, np.exp(1j * b) * shift
 
 
+class Workspace:
+    """Device scratch that grows on demand. Engines whose calls run one after another on one stream
+    (the four masks of a Bi-O Edge sensor) share one, so their peak memory is one workspace."""
+
+    def __init__(self, device):
+        self.device = device
+        self.buf = None
+
+    def get(self, nbytes):
+        if self.buf is None or self.buf.numel() < nbytes:
+            self.buf = None
+            self.buf = alloc_workspace(nbytes, self.device)
+        return self.buf
+
+    def release(self):
+        self.buf = None
+
+    @property
+    def nbytes(self):
+        return 0 if self.buf is None else self.buf.numel()
+
+
 class PWFSEngine:
     """Device-resident tables, workspace and the HIP calls for one PWFS geometry."""
 
-    def __init__(self, n, N, cam, device=None, nchunk=None, pts_per_task=None):
+    def __init__(self, n, N, cam, device=None, nchunk=None, pts_per_task=None, workspace=None):
         self.device = torch.device('cuda', torch.cuda.current_device()) if device is None else torch.device(device)
         self.n, self.N, self.cam = n, N, cam
         self.bin = N // cam if cam and N % cam == 0 else 0
@@ -102,7 +143,7 @@ class PWFSEngine:
         self.mode = 'fast' if self.fast_geometry else 'generic'
         self.nchunk_user = nchunk
         self.pts_per_task_user = pts_per_task
-        self._ws = None
+        self.workspace = Workspace(self.device) if workspace is None else workspace
         self.T = 0
         self.maskT = None
         self._fullT = None
@@ -207,10 +248,7 @@ class PWFSEngine:
         return opd
 
     def _scratch(self, nbytes):
-        if self._ws is None or self._ws.numel() < nbytes:
-            self._ws = None
-            self._ws = alloc_workspace(nbytes, self.device)
-        return self._ws
+        return self.workspace.get(nbytes)
 
     def _generic_nchunk(self, B):
         """Point chunks of the general path: enough (screen, chunk, row) tasks to fill the GPU."""
@@ -229,10 +267,12 @@ class PWFSEngine:
                       ctypes.byref(nb))
         return nb.value
 
-    def frames(self, opd, scale, amp, out=None, raw=None, weighted=False):
+    def frames(self, opd, scale, amp, out=None, raw=None, weighted=False, bands=None):
         """Detector frames [B, cam, cam] float32 for OPD/phase maps opd [B, n, n] (phase = opd*scale).
         ``raw`` (a float32 [B, N, N] tensor) also receives the unbinned frames (general path);
-        ``weighted`` uses the per-point weights of set_weights."""
+        ``weighted`` uses the per-point weights of set_weights. ``bands`` (fast path): detector-column
+        intervals [(start, stop), ...] to compute, the other columns are returned as zero (the
+        slopesMaps signal needs only the quadrant columns); ignored on the general path."""
         opd = self._check(opd, amp)
         B = opd.shape[0]
         if out is None:
@@ -246,7 +286,8 @@ class PWFSEngine:
         ws = self._scratch(nb.value)
         _lib.call('oopao_pwfs_frames', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
                   _ptr(self.pstart), _ptr(ty), _ptr(self.mx), _ptr(self.my), self.n, self.N, self.T, B, self.bin,
-                  nchunk, ppt, _ptr(ws), ctypes.c_size_t(nb.value), _ptr(out), _stream_ptr(self.device))
+                  nchunk, ppt, column_bands(bands, self.cam), _ptr(ws), ctypes.c_size_t(nb.value), _ptr(out),
+                  _stream_ptr(self.device))
         return out
 
     def _full_maskT(self):
@@ -344,16 +385,25 @@ class PWFSEngine:
                   ctypes.byref(pb))
         inter = torch.empty(ib.value, dtype=torch.uint8, device=self.device)
         part = torch.empty(pb.value // 4, dtype=torch.float32, device=self.device)
-        return dict(B=B, nchunk=nchunk, ppt=ppt, inter=inter, part=part)
+        return dict(B=B, T=self.T, nchunk=nchunk, ppt=ppt, inter=inter, part=part)
 
-    def rows(self, opd, scale, amp, buf):
+    def _check_buffers(self, buf):
+        if buf['T'] != self.T:
+            raise OopaoError('PWFS buffers were sized for %d modulation points, the sensor has %d'
+                             % (buf['T'], self.T))
+
+    def rows(self, opd, scale, amp, buf, bands=None):
+        self._check_buffers(buf)
         _lib.call('oopao_pwfs_rows', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
                   _ptr(self.pstart), _ptr(self.ty_s), _ptr(self.mx), self.n, self.N, self.T, buf['B'], buf['ppt'],
-                  _ptr(buf['inter']), _stream_ptr(self.device))
+                  column_bands(bands, self.cam), _ptr(buf['inter']), ctypes.c_size_t(buf['inter'].numel()),
+                  _stream_ptr(self.device))
 
-    def cols(self, buf, out):
-        _lib.call('oopao_pwfs_cols', _ptr(buf['inter']), _ptr(self.my), self.n, self.N, self.T, buf['B'],
-                  buf['nchunk'], _ptr(buf['part']), _ptr(out), _stream_ptr(self.device))
+    def cols(self, buf, out, bands=None):
+        self._check_buffers(buf)
+        _lib.call('oopao_pwfs_cols', _ptr(buf['inter']), ctypes.c_size_t(buf['inter'].numel()), _ptr(self.my),
+                  self.n, self.N, self.T, buf['B'], buf['nchunk'], column_bands(bands, self.cam), _ptr(buf['part']),
+                  ctypes.c_size_t(4 * buf['part'].numel()), _ptr(out), _stream_ptr(self.device))
 
 
 class SlopesEngine:
