@@ -9,7 +9,8 @@
  *
  * Conventions
  *  - All array pointers are device pointers (HIP), caller-allocated; the library
- *    allocates nothing per call (a per-(device, N) twiddle table is built once).
+ *    allocates nothing per call (a per-(device, N) twiddle table is built once,
+ *    ahead of use with oopao_prepare_tables).
  *  - Complex arrays are interleaved float pairs (complex64).
  *  - Every call is enqueued on `stream` (a hipStream_t, NULL = default stream)
  *    and returns immediately.
@@ -28,6 +29,13 @@ extern "C" {
 
 /* Message for the last failed call on this thread. */
 const char* oopao_last_error(void);
+
+/* Build the device twiddle table exp(-2 pi i m / N) of transform length N on the
+ * current device (once per device and N; kept until the process ends). The
+ * transform entry points build it on their first call for a new N with a
+ * synchronous allocation and copy, which a stream under graph capture does not
+ * allow: call this first for every N a captured call will use. */
+int oopao_prepare_tables(int N);
 
 /* ---------------------------------------------------------------------------
  * Pyramid WFS (replaces Pyramid.pyramid_transform + the modulation loop and
@@ -295,6 +303,18 @@ int oopao_atm_add_row(const double* map, double* tmp, int M, int sx, int sy, con
 
 /* mm[0] = min(map[0..count)), mm[1] = max; mm holds 4 doubles (mm[2..3] are working space). */
 int oopao_atm_minmax(const double* map, int count, double* mm, void* stream);
+
+/* Boiling (Atmosphere.apply_boiling, Atmosphere.py:387-408): for the layer's extended screen map
+ * [M][M] float64 (M = res + 2), map = alpha map + beta phi, phi the phase screen of
+ * ft_sh_phase_screen (nsub = 12) or ft_phase_screen (nsub = 0) (phaseStats.py:153-271):
+ *   g   [2 M^2]   the RandomState(boiling_seed) normal draws (real parts, then imaginary parts)
+ *   amp [M^2]     sqrt(PSD) df of the FFT grid (PSD[M/2][M/2] = 0)
+ *   W   [M][M]    complex128 centred DFT matrix (W C W^T = fftshift(fft2(fftshift(C))))
+ *   sub [nsub][4] (Re c, Im c, fx, fy) of the subharmonic terms, xc [M] their grid coordinates.
+ * Workspace of oopao_atm_boil_bytes(M) bytes. */
+int oopao_atm_boil_bytes(int M, size_t* bytes);
+int oopao_atm_boil(double* map, int M, const double* g, const double* amp, const void* W, const double* sub,
+                   int nsub, const double* xc, double alpha, double beta, void* ws, size_t ws_bytes, void* stream);
 
 /* out[y][x] (+)= w * clip(cubic(map, row y + r0 - ty, col x + c0 - tx)) for an
  * n x n window of the M x M map: Catmull-Rom cubic convolution, 0 outside the

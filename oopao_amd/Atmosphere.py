@@ -19,8 +19,10 @@ star the cone-effect magnification (bilinear, about the footprint centre),
 on the GPU (oopao_bilinear_affine). With angular_spectrum_propagation the source
 field is propagated through the layers by the angular spectrum (oopao_asm_propagate,
 Atmosphere.py:644-706, 938-972), giving the scintillation (intensity) and the phase
-(wrapped, geometric, or geometric guide + wrapped residual). Boiling raises
-OopaoError.
+(wrapped, geometric, or geometric guide + wrapped residual). Boiling layers
+(t_boiling, Atmosphere.py:376-408) mix their extended screen with a fresh
+phase screen each step on the GPU (oopao_atm_boil), from the layer's host
+RandomState(boiling_seed) draws in the reference's order.
 """
 import ctypes
 
@@ -31,7 +33,7 @@ from numpy.random import RandomState
 from . import _lib
 from ._device import ptr, stream_ptr, to_host
 from .phaseStats import (ft_sh_phase_screen, ft_phase_screen, makeCovarianceMatrix, extrusion_rings,
-                         extrusion_operators)
+                         extrusion_operators, centred_dft_matrix, ft_amplitude, subharmonic_terms)
 from .tools import OopaoError, warning
 
 
@@ -70,9 +72,7 @@ class Atmosphere:
         self.nLayer = len(fractionalR0)
         self.windSpeed = windSpeed
         self.windDirection = windDirection
-        if t_boiling is not None and any(v is not None for v in np.atleast_1d(t_boiling)):
-            raise OopaoError('boiling layers are not supported by oopao_amd')
-        self.t_boiling = [None] * self.nLayer
+        self.t_boiling = t_boiling
         self.n_extra_pixel = 2
         self.V0 = (np.sum(np.asarray(self.fractionalR0) * np.asarray(self.windSpeed) ** (5 / 3))) ** (3 / 5)
         self.tau0 = 0.31 * self.r0 / self.V0
@@ -131,7 +131,8 @@ class Atmosphere:
         lay.D = lay.resolution * tel.D / tel.resolution
         lay.pixel_size = lay.D / lay.resolution
         lay.n_extra_pixel = self.n_extra_pixel
-        lay.t_boiling = None
+        lay.t_boiling = self.t_boiling[i_layer]
+        lay.boiling_seed = 100000 + i_layer * 100000      # Atmosphere.py:329-333
         res, M = lay.resolution, lay.resolution + self.n_extra_pixel
         lay.OPD_host = ft_sh_phase_screen(self.r0, self.L0, res, lay.D / res, seed=i_layer)
         lay.seed = i_layer
@@ -171,13 +172,15 @@ class Atmosphere:
 
     def save_state(self, path):
         """Write the evolving state of every layer to an .npz file: the extruded screen, its (min, max),
-        the sub-pixel shift buffer, the per-step ratio and the layer's RandomState. The reference has no
+        the sub-pixel shift buffer, the per-step ratio, the layer's RandomState and boiling seed, and the
+        parameters load_state checks. The reference has no
         simulation checkpoint; a second initializeAtmosphere resets the layers (Atmosphere.py:246-271).
         load_state on an atmosphere built with the same parameters resumes the same sequence of screens."""
         if self.hasNotBeenInitialized:
             raise OopaoError('The Atmosphere object needs to be initialised using the initialiseAtmosphere()')
         out = {'nLayer': np.int64(self.nLayer), 'r0': np.float64(self.r0), 'L0': np.float64(self.L0),
-               'resolution': np.int64(self.telescope.resolution)}
+               'resolution': np.int64(self.telescope.resolution),
+               'samplingTime': np.float64(self.telescope.samplingTime)}
         for i in range(self.nLayer):
             lay = getattr(self, 'layer_' + str(i + 1))
             key, pos, has_gauss, cached = lay.randomState.get_state()[1:]
@@ -192,10 +195,23 @@ class Atmosphere:
             out[p + 'rs_key'] = np.asarray(key, np.uint32)
             out[p + 'rs_pos'] = np.int64(pos)
             out[p + 'rs_gauss'] = np.asarray([has_gauss, cached], np.float64)
+            for name, val in self._layer_params(i).items():
+                out[p + name] = np.float64(val)
+            out[p + 'boiling_seed'] = np.int64(lay.boiling_seed)
         np.savez(path, **out)
 
+    def _layer_params(self, i):
+        """Parameters that fix layer i's per-step evolution (wind shift per step, statistics, boiling)."""
+        lay = getattr(self, 'layer_' + str(i + 1))
+        t_b = getattr(lay, 't_boiling', None)
+        return {'vX': lay.vX, 'vY': lay.vY, 'pixel_size': lay.pixel_size, 'altitude': lay.altitude,
+                'fractionalR0': self.fractionalR0[i], 't_boiling': np.nan if t_b is None else t_b}
+
     def load_state(self, path):
-        """Restore what save_state wrote; the layer geometry and r0 must match this atmosphere's."""
+        """Restore what save_state wrote. The atmosphere must have the saved one's parameters: number of
+        layers, resolution, r0, L0, sampling time, and per layer the wind (speed and direction), pixel
+        size, altitude, fractional r0 and boiling time; otherwise the resumed screens would follow other
+        statistics, and OopaoError names what differs."""
         if self.hasNotBeenInitialized:
             raise OopaoError('The Atmosphere object needs to be initialised using the initialiseAtmosphere()')
         with np.load(path) as f:
@@ -203,11 +219,27 @@ class Atmosphere:
                     or float(f['r0']) != float(self.r0) or float(f['L0']) != float(self.L0):
                 raise OopaoError('the saved atmosphere state does not match this atmosphere (nLayer, resolution, '
                                  'r0, L0)')
+            if 'samplingTime' in f and float(f['samplingTime']) != float(self.telescope.samplingTime):
+                raise OopaoError('the saved atmosphere state was taken at another sampling time (%g s, here %g s)'
+                                 % (float(f['samplingTime']), self.telescope.samplingTime))
+            for i in range(self.nLayer):
+                p = 'layer_%d_' % (i + 1)
+                for name, val in self._layer_params(i).items():
+                    if p + name not in f:
+                        continue
+                    saved = float(f[p + name])
+                    same = (np.isnan(saved) and np.isnan(val)) or (not np.isnan(saved) and not np.isnan(val)
+                                                                  and np.isclose(saved, val, rtol=1e-12, atol=0))
+                    if not same:
+                        raise OopaoError('the saved atmosphere state does not match layer %d: %s saved %r, here %r'
+                                         % (i + 1, name, saved, float(val)))
             for i in range(self.nLayer):
                 lay = getattr(self, 'layer_' + str(i + 1))
                 p = 'layer_%d_' % (i + 1)
                 if int(f[p + 'M']) != lay.M:
                     raise OopaoError('the saved atmosphere state does not match layer %d (M)' % (i + 1))
+                if p + 'boiling_seed' in f:
+                    lay.boiling_seed = int(f[p + 'boiling_seed'])
                 lay.map.copy_(torch.from_numpy(f[p + 'map']))
                 lay.mm[:2].copy_(torch.from_numpy(f[p + 'mm']))
                 lay.buff = f[p + 'buff'].copy()
@@ -245,6 +277,7 @@ class Atmosphere:
             lay.notDoneOnce = True
             lay.buff = np.zeros(2)
             lay.shift = (0.0, 0.0)
+            lay.boiling_seed = 100000 + seed + i * 100000      # Atmosphere.py:797-799
         self._compute_opd()
         if self.telescope.isPaired:
             self.telescope.src ** self * self.telescope
@@ -262,30 +295,111 @@ class Atmosphere:
     _NOISE_RING = 8
 
     def _stage_noise(self, lay, draw):
-        """Host draw -> device noise vector without stalling the host: the draw is written into one of a
-        ring of pinned buffers and copied asynchronously on the stream; a buffer is rewritten only after
-        the event of its previous copy has completed."""
-        ring = getattr(lay, '_xi_ring', None)
-        if ring is None or ring[0].numel() != draw.size:
-            ring = [torch.empty(draw.size, dtype=torch.float64, pin_memory=True) for _ in range(self._NOISE_RING)]
-            lay._xi_ring = ring
-            lay._xi_events = [None] * self._NOISE_RING
-            lay._xi_k = 0
-        k = lay._xi_k
-        lay._xi_k = (k + 1) % self._NOISE_RING
-        ev = lay._xi_events[k]
-        if ev is not None:
-            ev.synchronize()
-        ring[k].numpy()[:] = draw
-        lay.xi.copy_(ring[k], non_blocking=True)
+        self._stage(lay, 'xi', draw, lay.xi)
+
+    def _stage(self, lay, name, draw, dst):
+        """Host draw -> device vector ``dst`` without stalling the host: the draw is written into one of a
+        ring of pinned buffers (one ring per layer and name) and copied asynchronously on the stream; a
+        buffer is rewritten only after the event of its previous copy has completed."""
+        rings = lay.__dict__.setdefault('_rings', {})
+        r = rings.get(name)
+        if r is None or r['buf'][0].numel() != draw.size:
+            r = dict(buf=[torch.empty(draw.size, dtype=torch.float64, pin_memory=True)
+                          for _ in range(self._NOISE_RING)], ev=[None] * self._NOISE_RING, k=0)
+            rings[name] = r
+        k = r['k']
+        r['k'] = (k + 1) % self._NOISE_RING
+        if r['ev'][k] is not None:
+            r['ev'][k].synchronize()
+        r['buf'][k].numpy()[:] = draw
+        dst.copy_(r['buf'][k], non_blocking=True)
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream(self.device))
-        lay._xi_events[k] = ev
+        r['ev'][k] = ev
+
+    # ------------------------------------------------------------ boiling
+    def get_t_boiling(self, val):
+        """Atmosphere.get_t_boiling (Atmosphere.py:376-385): one boiling time (or None) per layer."""
+        if val is None:
+            return [None] * self.nLayer
+        if np.isscalar(val):
+            return [val] * self.nLayer
+        val = list(val)
+        if len(val) != self.nLayer:
+            raise OopaoError('Wrong value for t_boiling! Provide either a scalar '
+                             '(applied to each layer) or a list with a value per layer.')
+        return val
+
+    @property
+    def t_boiling(self):
+        return self._t_boiling
+
+    @t_boiling.setter
+    def t_boiling(self, val):
+        self._t_boiling = self.get_t_boiling(val)
+        if self.hasNotBeenInitialized is False:
+            print('Updating the boiling time-constant(s)...')
+            for i in range(self.nLayer):
+                getattr(self, 'layer_' + str(i + 1)).t_boiling = self._t_boiling[i]
+
+    def _boil_tables(self, lay):
+        """Device tables of the boiling screen on the layer's (res + 2)^2 support: the FFT-grid amplitudes
+        sqrt(PSD) df, the centred DFT matrix, the subharmonic grid coordinates and the workspace."""
+        M, pixel = lay.M, lay.D / lay.resolution
+        key = (M, float(pixel), float(self.r0), float(self.L0))
+        cache = self.__dict__.setdefault('_boil_cache', {})
+        t = cache.get(key)
+        if t is None:
+            dev = self.device
+            nb = ctypes.c_size_t(0)
+            _lib.call('oopao_atm_boil_bytes', M, ctypes.byref(nb))
+            t = dict(amp=torch.from_numpy(ft_amplitude(self.r0, self.L0, M, pixel).reshape(-1)).to(dev),
+                     W=torch.from_numpy(centred_dft_matrix(M)).to(dev),
+                     xc=torch.from_numpy(np.arange(-M / 2, M / 2) * pixel).to(dev),
+                     ws=torch.empty(nb.value, dtype=torch.uint8, device=dev),
+                     g=torch.empty(2 * M * M + 48, dtype=torch.float64, device=dev))
+            cache[key] = t
+        return t
+
+    def _apply_boiling(self, lay):
+        """Atmosphere.apply_boiling (Atmosphere.py:387-408) on the device: mapShift = alpha mapShift +
+        beta phi with alpha = exp(-dt / t_boiling), beta = sqrt(1 - alpha^2) and phi the screen of seed
+        boiling_seed + 1 (ft_sh_phase_screen in mode 2, else ft_phase_screen) on the (res + 2)^2 support."""
+        t_b = getattr(lay, 't_boiling', None)
+        if t_b is None or not np.isfinite(t_b) or t_b <= 0:
+            return False
+        alpha = float(np.exp(-self.telescope.samplingTime / t_b))
+        if alpha >= 1.0 - 1e-6:
+            return False
+        beta = float(np.sqrt(max(0.0, 1.0 - alpha ** 2)))
+        lay.boiling_seed += 1
+        M = lay.M
+        tb = self._boil_tables(lay)
+        # one draw of the FFT-grid coefficients: real parts then imaginary parts (phaseStats.py:192); the
+        # subharmonics restart the same seed (a second RandomState, phaseStats.py:225), i.e. reuse the
+        # first 54 of these numbers
+        g = RandomState(lay.boiling_seed).normal(size=2 * M * M)
+        nsub = 0
+        staged = g
+        if self.mode == 2:
+            sub, _ = subharmonic_terms(self.r0, self.L0, M, lay.D / lay.resolution, g[:54])
+            nsub = sub.shape[0]
+            staged = np.concatenate((g, sub.reshape(-1)))
+        dst = tb['g'][:staged.size]
+        self._stage(lay, 'boil', staged, dst)
+        _lib.call('oopao_atm_boil', ptr(lay.map), M, ptr(dst), ptr(tb['amp']), ptr(tb['W']),
+                  ctypes.c_void_p(dst.data_ptr() + 8 * 2 * M * M) if nsub else None, nsub, ptr(tb['xc']),
+                  ctypes.c_double(alpha), ctypes.c_double(beta), ptr(tb['ws']), ctypes.c_size_t(tb['ws'].numel()),
+                  stream_ptr(self.device))
+        _lib.call('oopao_atm_minmax', ptr(lay.map), M * M, ptr(lay.mm), stream_ptr(self.device))
+        return True
 
     def updateLayer(self, lay, shift=None):
-        """Atmosphere.updateLayer (Atmosphere.py:447-516)."""
+        """Atmosphere.updateLayer (Atmosphere.py:447-516); boiling first (Atmosphere.py:454-457), so the
+        decorrelation accumulates in the support and composes with the wind shift."""
+        self._apply_boiling(lay)
         if lay.vX == 0 and lay.vY == 0 and shift is None:
-            return
+            return      # no wind: the (boiled) support's interior, read with a zero shift
         if lay.notDoneOnce:
             lay.notDoneOnce = False
             lay.ratio = np.zeros(2)
@@ -477,6 +591,7 @@ class Atmosphere:
             fx, fy = np.meshgrid(f, f)
             Hh = np.exp(-1j * np.pi * wvl * distance * (fx ** 2 + fy ** 2))
             H = torch.from_numpy(Hh.astype(np.complex64)).to(self.device)
+            _lib.prepare_tables(n, self.device)
             self._transfer[key] = H
         return H
 
@@ -768,4 +883,7 @@ class Atmosphere:
         s = '\n------------ Atmosphere ------------\n'
         s += f"{'r0 @500nm [m]':<25s}|{self.r0:^9.3f}\n{'L0 [m]':<25s}|{self.L0:^9.1f}\n"
         s += f"{'Layers':<25s}|{self.nLayer:^9d}\n"
+        for i, t_b in enumerate(self.t_boiling):
+            evo = 'Frozen Flow' if t_b is None else f'Boiling: t = {t_b:.4g} s'
+            s += f"{'Layer %d' % (i + 1):<25s}|{evo:^24s}\n"
         return s

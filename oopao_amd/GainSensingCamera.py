@@ -58,6 +58,7 @@ class GainSensingCamera:
             raise OopaoError(f'Output size {N} should be higher than input size {n}')
         if (N - n) % 2:
             raise OopaoError('mask size minus basis size must be even (symmetric zero padding)')
+        _lib.prepare_tables(N, self.device)
         if not _lib.lib().oopao_fft2_supported(N):
             raise OopaoError('no FFT plan for N=%d' % N)
         self.N, self.n = N, n

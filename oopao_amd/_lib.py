@@ -22,6 +22,7 @@ c_double = ctypes.c_double
 # name -> (restype, argtypes); the single source of the binding's signatures
 SIGNATURES = {
     'oopao_last_error': (ctypes.c_char_p, []),
+    'oopao_prepare_tables': (c_int, [c_int]),
     'oopao_pwfs_supported': (c_int, [c_int, c_int, c_int]),
     'oopao_pwfs_workspace_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_frames': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
@@ -60,6 +61,9 @@ SIGNATURES = {
     'oopao_atm_add_row':(c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'oopao_atm_minmax': (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    'oopao_atm_boil_bytes': (c_int, [c_int, ctypes.POINTER(c_size_t)]),
+    'oopao_atm_boil': (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double,
+                               c_double, c_void_p, c_size_t, c_void_p]),
     'oopao_atm_layer_opd': (c_int, [c_void_p, c_int, c_void_p, c_double, c_double, c_int, c_int, c_double, c_void_p,
                                     c_int, c_int, c_void_p]),
     'oopao_bilinear_affine': (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_double, c_double, c_double,
@@ -119,3 +123,13 @@ def check(rc):
 
 def call(name, *args):
     check(getattr(lib(), name)(*args))
+
+
+def prepare_tables(N, device):
+    """Build the twiddle table of length N on ``device`` now (not inside a later, maybe captured, call)."""
+    import torch
+    device = torch.device(device)
+    if device.type != 'cuda':
+        return
+    with torch.cuda.device(device):
+        call('oopao_prepare_tables', int(N))

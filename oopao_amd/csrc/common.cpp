@@ -47,3 +47,9 @@ int twiddles(int N, const float2** out) {
 }  // namespace oopao
 
 extern "C" const char* oopao_last_error(void) { return oopao::get_error(); }
+
+extern "C" int oopao_prepare_tables(int N) {
+  OOPAO_CHECK_ARG(N > 0 && N <= 4096, "invalid transform length N=%d", N);
+  const float2* tw = nullptr;
+  return oopao::twiddles(N, &tw);
+}

@@ -63,7 +63,9 @@ const char* get_error();
   } while (0)
 
 // Device table tw[m] = exp(-2 pi i m / N), m = 0..N-1, one per (device, N),
-// built once from double-precision values and kept for the process lifetime.
+// built once from double-precision values and kept for the process lifetime
+// (allocated and copied synchronously on first use: oopao_prepare_tables builds
+// it ahead, e.g. before a graph capture).
 int twiddles(int N, const float2** out);
 
 }  // namespace oopao

@@ -62,6 +62,54 @@ def extrusion_operators(ZZt, ZXt, XXt, ZZt_inv, r0_def, r0):
     return A, B
 
 
+def centred_dft_matrix(M):
+    """W [M, M] complex128 with W C W^T = fftshift(fft2(fftshift(C))) (the ift2 of phaseStats.py:135-150):
+    W = P F P for the DFT F and the fftshift permutation P (shift by h = M // 2), i.e.
+    W[k, l] = exp(-2 pi i ((k - h) mod M)((l + h) mod M) / M), the exponent reduced exactly mod M."""
+    h = M // 2
+    k = np.arange(M)
+    r = np.outer((k - h) % M, (k + h) % M) % M
+    return np.exp(-2j * np.pi * r / M)
+
+
+def ft_amplitude(r0, L0, N, delta, l0=1e-10):
+    """sqrt(PSD) df of ft_phase_screen's FFT grid (phaseStats.py:176-195), PSD[N/2, N/2] = 0: the
+    screen's Fourier coefficients are (g_re + i g_im) times this."""
+    delta = float(delta)
+    del_f = 1. / (N * delta)
+    fx = np.arange(-N / 2., N / 2.) * del_f
+    fx, fy = np.meshgrid(fx, fx)
+    f = np.sqrt(fx ** 2 + fy ** 2)
+    fm = 5.92 / l0 / (2 * np.pi)
+    psd = 0.023 * r0 ** (-5. / 3.) * np.exp(-1 * ((f / fm) ** 2)) / (((f ** 2) + (1. / L0) ** 2) ** (11. / 6))
+    psd[int(N / 2), int(N / 2)] = 0
+    return np.sqrt(psd) * del_f
+
+
+def subharmonic_terms(r0, L0, resolution, pixel_size, draws, l0=1e-10):
+    """The 12 subharmonic terms of ft_sh_phase_screen (phaseStats.py:236-266) as rows (Re c, Im c, fx, fy)
+    of c exp(2 pi i (fx x + fy y)), and the grid coordinates x = y; ``draws`` are the first 54 normal draws
+    of RandomState(seed) (per level: 9 real then 9 imaginary parts, row-major 3 x 3)."""
+    pixel_size = float(pixel_size)
+    D = resolution * pixel_size
+    g = np.asarray(draws, np.float64)[:54].reshape(3, 2, 3, 3)
+    rows = []
+    for p in range(1, 4):
+        del_f = 1 / (3 ** p * D)
+        fxv = np.arange(-1, 2) * del_f
+        fx, fy = np.meshgrid(fxv, fxv)
+        f = np.sqrt(fx ** 2 + fy ** 2)
+        fm = 5.92 / l0 / (2 * np.pi)
+        psd = 0.023 * r0 ** (-5. / 3) * np.exp(-1 * (f / fm) ** 2) / ((f ** 2 + (1. / L0) ** 2) ** (11. / 6))
+        psd[1, 1] = 0
+        cn = (g[p - 1, 0] + 1j * g[p - 1, 1]) * np.sqrt(psd) * del_f
+        for i in range(0, 2):
+            for j in range(0, 2):
+                rows.append((cn[i, j].real, cn[i, j].imag, fx[i, j], fy[i, j]))
+    coords = np.arange(-resolution / 2, resolution / 2) * pixel_size
+    return np.asarray(rows, np.float64), coords
+
+
 def ift2(G):
     """fftshift(fft2(fftshift(G))) (phaseStats.py:135-150)."""
     return np.fft.fftshift(np.fft.fft2(np.fft.fftshift(G)))

@@ -141,6 +141,7 @@ class PWFSEngine:
             raise OopaoError('PWFS geometry n=%d N=%d binning=%d is not supported by the HIP path '
                              '(needs n <= N <= 2048 and a detector size dividing N)' % (n, N, self.bin))
         self.mode = 'fast' if self.fast_geometry else 'generic'
+        _lib.prepare_tables(N, self.device)
         self.nchunk_user = nchunk
         self.pts_per_task_user = pts_per_task
         self.workspace = Workspace(self.device) if workspace is None else workspace

@@ -52,6 +52,7 @@ class _FocalEngine:
         if not _lib.lib().oopao_pwfs_generic_supported(n, N, 1):
             raise OopaoError('padded size N=%d outside the supported range' % N)
         self.tab = torch.from_numpy(np.ascontiguousarray(tab.astype(np.complex64)[None, :])).to(device)
+        _lib.prepare_tables(N, device)
         nb = ctypes.c_size_t(0)
         _lib.call('oopao_pwfs_generic_focal_bytes', n, N, 1, 1, 1, img, ctypes.byref(nb))
         self.nb = nb.value

@@ -12,7 +12,11 @@ Test infrastructure only (see ``oracle/__init__.py``).
 * LayerOracle: updateLayer / add_row (Atmosphere.py:410-421, 447-516) and the
   pupil-footprint crop of fill_OPD_support (589-622) for an on-axis source at
   infinity, in float64, consuming the layer's RandomState exactly like the
-  reference (one normal(size=nOuter) draw per extruded ring).
+  reference (one normal(size=nOuter) draw per extruded ring); with a boiling
+  time, apply_boiling (387-408) first: the AR(1) mix of mapShift with a fresh
+  ft_sh_phase_screen / ft_phase_screen of seed boiling_seed + 1.
+* ft_phase_screen, ft_sh_phase_screen: phaseStats.py:153-271 (numpy FFTs, the
+  reference's RandomState draw order: the subharmonics restart the seed).
 * warp_bilinear: scikit-image's order-1 warp (mode 'constant', cval 0) for an
   affine map y = a r + by, x = a c + bx: the four floor/ceil neighbours of the
   mapped coordinate, any neighbour outside the image counted as cval. Pinned by
@@ -67,6 +71,44 @@ def warp_shift(img, tx, ty, cval=0.0, clip=True):
     return out
 
 
+def ft_phase_screen(r0, L0, N, delta, seed, l0=1e-10):
+    """phaseStats.ft_phase_screen (phaseStats.py:153-198), radians at 500 nm."""
+    rs = np.random.RandomState(seed)
+    del_f = 1. / (N * delta)
+    fx = np.arange(-N / 2., N / 2.) * del_f
+    fx, fy = np.meshgrid(fx, fx)
+    f = np.sqrt(fx ** 2 + fy ** 2)
+    fm = 5.92 / l0 / (2 * np.pi)
+    psd = 0.023 * r0 ** (-5. / 3.) * np.exp(-1 * ((f / fm) ** 2)) / (((f ** 2) + (1. / L0) ** 2) ** (11. / 6))
+    psd[int(N / 2), int(N / 2)] = 0
+    cn = (rs.normal(size=(N, N)) + 1j * rs.normal(size=(N, N))) * np.sqrt(psd) * del_f
+    return np.fft.fftshift(np.fft.fft2(np.fft.fftshift(cn))).real
+
+
+def ft_sh_phase_screen(r0, L0, resolution, pixel_size, seed, l0=1e-10):
+    """phaseStats.ft_sh_phase_screen (phaseStats.py:201-271): FFT screen + 3 levels of the 2 x 2
+    subharmonics of a 3 x 3 grid, the subharmonic draws from a second RandomState(seed)."""
+    rs = np.random.RandomState(seed)
+    D = resolution * pixel_size
+    hi = ft_phase_screen(r0, L0, resolution, pixel_size, seed, l0)
+    coords = np.arange(-resolution / 2, resolution / 2) * pixel_size
+    x, y = np.meshgrid(coords, coords)
+    lo = np.zeros((resolution, resolution), complex)
+    for p in range(1, 4):
+        del_f = 1 / (3 ** p * D)
+        fx = np.arange(-1, 2) * del_f
+        fx, fy = np.meshgrid(fx, fx)
+        f = np.sqrt(fx ** 2 + fy ** 2)
+        fm = 5.92 / l0 / (2 * np.pi)
+        psd = 0.023 * r0 ** (-5. / 3) * np.exp(-1 * (f / fm) ** 2) / ((f ** 2 + (1. / L0) ** 2) ** (11. / 6))
+        psd[1, 1] = 0
+        cn = (rs.normal(size=(3, 3)) + 1j * rs.normal(size=(3, 3))) * np.sqrt(psd) * del_f
+        for i in range(0, 2):
+            for j in range(0, 2):
+                lo += cn[i, j] * np.exp(1j * 2 * np.pi * (fx[i, j] * x + fy[i, j] * y))
+    return (lo.real - lo.real.mean()) + hi
+
+
 class LayerOracle:
     """One frozen-flow layer: mapShift [(res+2)^2] float64, A, B, RandomState (Atmosphere.py:281-374, 410-516)."""
 
@@ -81,6 +123,28 @@ class LayerOracle:
         self.buff = np.zeros(2)
         self.res = self.mapShift.shape[0] - 2
         self.OPD = self.mapShift[1:-1, 1:-1].copy()
+        self.boiling = None
+
+    def set_boiling(self, t_boiling, sampling_time, r0, L0, pixel_size, boiling_seed, mode=2):
+        """Boiling state of the layer (Atmosphere.py:329-333, 387-408)."""
+        self.boiling = dict(t=t_boiling, dt=sampling_time, r0=r0, L0=L0, pixel=pixel_size, mode=mode)
+        self.boiling_seed = boiling_seed
+
+    def apply_boiling(self):
+        """Atmosphere.apply_boiling (Atmosphere.py:387-408): mapShift = alpha mapShift + beta phi."""
+        b = self.boiling
+        if b is None or b['t'] is None or not np.isfinite(b['t']) or b['t'] <= 0:
+            return False
+        alpha = float(np.exp(-b['dt'] / b['t']))
+        if alpha >= 1.0 - 1e-6:
+            return False
+        beta = float(np.sqrt(max(0.0, 1.0 - alpha ** 2)))
+        self.boiling_seed += 1
+        M = self.mapShift.shape[0]
+        gen = ft_sh_phase_screen if b['mode'] == 2 else ft_phase_screen
+        phi = gen(b['r0'], b['L0'], M, b['pixel'], seed=self.boiling_seed)
+        self.mapShift = alpha * self.mapShift + beta * phi
+        return True
 
     def add_row(self, step):
         """Atmosphere.add_row (Atmosphere.py:410-421)."""
@@ -93,8 +157,14 @@ class LayerOracle:
         return one
 
     def update(self):
-        """Atmosphere.updateLayer (Atmosphere.py:447-516) for a layer with non-zero wind."""
+        """Atmosphere.updateLayer (Atmosphere.py:447-516): boiling first, then the frozen-flow shift (a
+        layer without wind delivers the boiled support's interior)."""
+        boiled = self.apply_boiling()
         ratio = self.ratio
+        if ratio[0] == 0 and ratio[1] == 0:
+            if boiled:
+                self.OPD = self.mapShift[1:-1, 1:-1].copy()
+            return self.OPD
         tmp = np.abs(ratio)
         tmp[np.isinf(tmp)] = 0
         nScreens = tmp.astype('int')

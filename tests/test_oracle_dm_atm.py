@@ -3,6 +3,7 @@ host-side phase-screen code (oopao_amd/phaseStats.py)."""
 import math
 
 import numpy as np
+import pytest
 from numpy.random import RandomState
 
 from oracle.dm import DMOracle
@@ -143,3 +144,65 @@ def test_covariance_continuity_at_zero_lag():
     assert np.isclose(c[0, 1] / c[0, 0], 1.0, rtol=1e-5)
     # von Karman phase variance closed form 0.0863 (L0/r0)^(5/3) (phaseStats.variance)
     assert np.isclose(c[0, 0], 0.0863 * (L0 / r0) ** (5 / 3), rtol=1e-3)
+
+
+@pytest.mark.parametrize('M', [10, 11, 34, 326])
+def test_centred_dft_matrix(M):
+    # W C W^T = fftshift(fft2(fftshift(C))) (phaseStats.py:135-150) for even and odd supports
+    from oopao_amd.phaseStats import centred_dft_matrix
+    rs = np.random.RandomState(M)
+    C = rs.randn(M, M) + 1j * rs.randn(M, M)
+    W = centred_dft_matrix(M)
+    ref = np.fft.fftshift(np.fft.fft2(np.fft.fftshift(C)))
+    assert np.max(np.abs(W @ C @ W.T - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize('M', [34, 35])
+def test_boiling_screen_pieces(M):
+    # the pieces the device boiling screen is built from reproduce the host generators exactly:
+    # FFT-grid amplitude x the seed's draws -> ft_phase_screen; the 12 staged subharmonic terms from the
+    # first 54 draws -> ft_sh - ft; and the oracle's restatement equals the product's host code
+    from oopao_amd.phaseStats import (centred_dft_matrix, ft_amplitude, subharmonic_terms, ft_phase_screen,
+                                      ft_sh_phase_screen)
+    from oracle import atmosphere as oat
+    r0, L0, pix, seed = 0.15, 25.0, 0.1, 100017
+    g = np.random.RandomState(seed).normal(size=2 * M * M)
+    cn = (g[:M * M] + 1j * g[M * M:]).reshape(M, M) * ft_amplitude(r0, L0, M, pix)
+    W = centred_dft_matrix(M)
+    hi = (W @ cn @ W.T).real
+    ft = ft_phase_screen(r0, L0, M, pix, seed=seed)
+    assert np.max(np.abs(hi - ft)) <= 1e-12 * np.max(np.abs(ft))
+    sub, xc = subharmonic_terms(r0, L0, M, pix, g[:54])
+    x, y = np.meshgrid(xc, xc)
+    lo = sum(c[0] * np.cos(2 * np.pi * (c[2] * x + c[3] * y)) - c[1] * np.sin(2 * np.pi * (c[2] * x + c[3] * y))
+             for c in sub)
+    sh = ft_sh_phase_screen(r0, L0, M, pix, seed=seed)
+    assert np.max(np.abs((lo - lo.mean()) + ft - sh)) <= 1e-12 * np.max(np.abs(sh))
+    assert np.allclose(oat.ft_sh_phase_screen(r0, L0, M, pix, seed), sh, rtol=0, atol=1e-12 * np.max(np.abs(sh)))
+    assert np.allclose(oat.ft_phase_screen(r0, L0, M, pix, seed), ft, rtol=0, atol=1e-12 * np.max(np.abs(ft)))
+
+
+def test_boiling_oracle_ar1_statistics():
+    # Atmosphere.apply_boiling (Atmosphere.py:387-408) is the AR(1) process m_{k+1} = a m_k + b phi_k with
+    # a^2 + b^2 = 1 and independent screens phi_k: the support keeps the screens' variance and successive
+    # supports have correlation a. Estimated over 400 steps of a windless layer (piston removed).
+    from oracle import atmosphere as oat
+    M, pix, r0, L0, dt, tb = 34, 0.25, 0.15, 25.0, 1e-3, 2e-3
+    a = np.exp(-dt / tb)
+    m0 = oat.ft_sh_phase_screen(r0, L0, M, pix, seed=5)
+    lay = oat.LayerOracle(m0, np.zeros((1, 1)), np.zeros((1, 1)), np.zeros((M, M)), np.zeros((M, M)),
+                          np.random.RandomState(0), (0.0, 0.0))
+    lay.set_boiling(tb, dt, r0, L0, pix, 100000, mode=2)
+    num = den = v0 = v1 = 0.0
+    prev = lay.mapShift - lay.mapShift.mean()
+    for k in range(400):
+        lay.update()
+        cur = lay.mapShift - lay.mapShift.mean()
+        num += np.sum(cur * prev)
+        den += np.sum(prev * prev)
+        v1 += np.sum(cur * cur)
+        prev = cur
+    assert abs(num / den - a) < 0.05
+    assert abs(v1 / den - 1) < 0.15
+    assert lay.boiling_seed == 100400
+    assert np.array_equal(lay.OPD, lay.mapShift[1:-1, 1:-1])
