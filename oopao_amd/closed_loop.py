@@ -9,7 +9,8 @@ closed_loop/run_cl.py:176-217): per iteration
 with the OPD sum, PWFS frame, slopes, reconstructor GEMV and integrator update
 kept on the device (no host round trip per iteration); with
 ``reconstructor_precision='bf16'`` the reconstructor is held in BF16 (half the
-bytes per iteration, FP32 accumulation). Records the
+bytes per iteration, FP32 accumulation), with ``'bf16x3'`` as split BF16 (hi + lo,
+the reconstructor to 2^-16, the float32 loop's accuracy). Records the
 turbulence and residual wavefront error in the pupil (nm RMS) per iteration,
 and, with ``strehl_every`` > 0, the Strehl ratio of the source's image behind
 the DM (ngs*tel*dm*cam with strehlMeter against the diffraction-limited image,
@@ -36,12 +37,13 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
                          % (tuple(R.shape), dm.nValidAct, wfs.nSignal))
     if frame_delay not in (1, 2):
         raise ValueError('frame_delay must be 1 or 2, got %r' % (frame_delay,))
-    if reconstructor_precision not in ('fp32', 'bf16'):
-        raise ValueError("reconstructor_precision must be 'fp32' or 'bf16', got %r" % (reconstructor_precision,))
+    if reconstructor_precision not in ('fp32', 'bf16', 'bf16x3'):
+        raise ValueError("reconstructor_precision must be 'fp32', 'bf16' or 'bf16x3', got %r"
+                         % (reconstructor_precision,))
     recon_bf16 = None
-    if reconstructor_precision == 'bf16':
+    if reconstructor_precision != 'fp32':
         from .contract import Contraction, PackedBF16
-        recon_bf16 = Contraction(PackedBF16(R, split=False))
+        recon_bf16 = Contraction(PackedBF16(R, split=reconstructor_precision == 'bf16x3'))
     pupil = tel._pupil_dev
     pupil_f = (pupil > 0).float().contiguous()
     npix = int((pupil > 0).sum())
@@ -93,7 +95,7 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
         sig = s_cur[0] if frame_delay == 1 else s_prev[0]
         # coefs <- coefs - gain * R @ sig   (R [nAct, nSig] row-major GEMV, float32 or BF16 matrix)
         if recon_bf16 is not None:
-            recon_bf16.gemv(sig, y0=coefs, alpha=-gain, beta=1.0, out=new_coefs)
+            recon_bf16.gemv(sig, y0=coefs, alpha=-gain, beta=1.0, out=new_coefs, precision=reconstructor_precision)
         else:
             _lib.call('oopao_gemv', ptr(R), ptr(sig), ptr(new_coefs), ptr(coefs), nAct, nSig, float(-gain), 1.0,
                       stream_ptr(dev))

@@ -113,8 +113,14 @@ class Contraction:
                   stream_ptr(self.A.device))
         return out
 
-    def gemv(self, x, y0=None, alpha=1.0, beta=0.0, out=None):
-        """y = alpha * A_hi @ x + beta * y0 (BF16 matrix, FP32 vector and accumulation)."""
+    def gemv(self, x, y0=None, alpha=1.0, beta=0.0, out=None, precision='bf16'):
+        """y = alpha * A @ x + beta * y0 with the BF16 matrix and an FP32 vector and accumulation:
+        ``precision='bf16'`` multiplies A_hi (A to 2^-8), ``'bf16x3'`` (a split operand) adds A_lo @ x in
+        a second pass (A = hi + lo to 2^-16; the same bytes as a float32 matrix)."""
+        if precision not in PRECISIONS:
+            raise OopaoError('precision must be one of %s, got %r' % (sorted(PRECISIONS), precision))
+        if precision == 'bf16x3' and self.A.lo is None:
+            raise OopaoError("precision 'bf16x3' needs a split operand (PackedBF16(..., split=True))")
         x = x.to(self.A.device, torch.float32).contiguous()
         if x.numel() != self.A.K:
             raise OopaoError('contraction length mismatch: matrix has K=%d, vector has %d' % (self.A.K, x.numel()))
@@ -123,7 +129,16 @@ class Contraction:
             _lib.call('oopao_gemv_bf16_workspace_bytes', self.A.I, self.A.K, ctypes.byref(nb))
             self._gemv_ws = torch.empty(nb.value, dtype=torch.uint8, device=self.A.device)
         y = out if out is not None else torch.empty(self.A.I, dtype=torch.float32, device=self.A.device)
-        _lib.call('oopao_gemv_bf16', ptr(self.A.hi), self.A.I, self.A.K, ptr(x), ptr(y),
+        first = y
+        if precision == 'bf16x3':
+            if getattr(self, '_gemv_tmp', None) is None:
+                self._gemv_tmp = torch.empty(self.A.I, dtype=torch.float32, device=self.A.device)
+            first = self._gemv_tmp
+        _lib.call('oopao_gemv_bf16', ptr(self.A.hi), self.A.I, self.A.K, ptr(x), ptr(first),
                   None if y0 is None else ptr(y0), float(alpha), float(beta), ptr(self._gemv_ws),
                   self._gemv_ws.numel(), stream_ptr(self.A.device))
+        if precision == 'bf16x3':
+            # y = alpha * A_lo @ x + (alpha * A_hi @ x + beta * y0)
+            _lib.call('oopao_gemv_bf16', ptr(self.A.lo), self.A.I, self.A.K, ptr(x), ptr(y), ptr(first),
+                      float(alpha), 1.0, ptr(self._gemv_ws), self._gemv_ws.numel(), stream_ptr(self.A.device))
         return y

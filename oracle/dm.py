@@ -50,6 +50,15 @@ class DMOracle:
             out[:, k] = G.reshape(n * n)
         return out
 
+    def modes_rows(self, pixels):
+        """Rows of the influence-function matrix for flat pupil pixels p = row * n + column, float64
+        [len(pixels), nValidAct], from the Gaussian formula (without forming the whole matrix)."""
+        p = np.asarray(pixels)
+        a = 1 / (2 * self.c ** 2)
+        X = self.grid[p % self.n][:, None]
+        Y = self.grid[p // self.n][:, None]
+        return np.exp(-(a * (X - self.u0x[None, :]) ** 2 + a * (Y - self.u0y[None, :]) ** 2))
+
     def opd(self, modes, coefs):
         """coefs setter (DeformableMirror.py:651-686): OPD = modes @ coefs, reshaped [n,n] or [n,n,B]."""
         coefs = np.asarray(coefs)

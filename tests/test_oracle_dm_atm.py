@@ -206,3 +206,9 @@ def test_boiling_oracle_ar1_statistics():
     assert abs(v1 / den - 1) < 0.15
     assert lay.boiling_seed == 100400
     assert np.array_equal(lay.OPD, lay.mapShift[1:-1, 1:-1])
+
+
+def test_dm_oracle_rows_equal_modes():
+    od = DMOracle(24, 8, 6, 0.35, central_obstruction=0.3)
+    rows = np.array([0, 5, 100, 287, 24 * 24 - 1])
+    assert np.array_equal(od.modes_rows(rows), od.modes()[rows])
