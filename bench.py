@@ -123,29 +123,36 @@ def closed_loop_leg(tel, ngs, wfs, dev, n_iter=1000, n_modes=1000):
     atm.initializeAtmosphere(tel)
     atm.generateNewPhaseScreen(10)
     run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=5)          # warm-up
+    atm.generateNewPhaseScreen(10)           # every timed loop starts from the same screens
     dm.coefs = np.zeros(dm.nValidAct)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     out = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=n_iter, strehl_every=10)
     dt = time.perf_counter() - t0
-    # the same loop (same screens, same start) with the reconstructor held in BF16 on the matrix-core
-    # GEMV path; its residual trace is compared with the float32 loop's
-    atm.generateNewPhaseScreen(10)
-    dm.coefs = np.zeros(dm.nValidAct)
-    run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=5, reconstructor_precision='bf16')
-    atm.generateNewPhaseScreen(10)
-    dm.coefs = np.zeros(dm.nValidAct)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    out16 = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=n_iter, strehl_every=10,
-                            reconstructor_precision='bf16')
-    dt16 = time.perf_counter() - t0
+    # the same loop (same screens, same start) with the reconstructor held in BF16 (plain, and split
+    # BF16 = hi + lo) on the matrix-core GEMV path; each residual trace is compared with the float32 loop's
     half = n_iter // 2
-    r32, r16 = out['wfe_residual'], out16['wfe_residual']
-    bf16 = {'value': round(n_iter / dt16, 2), 'unit': 'iterations/s',
-            'wfe_residual_nm': round(float(np.mean(r16[half:])), 1),
-            'strehl_pct_mean_second_half': round(float(np.mean(out16['strehl'][half // 10:])), 2),
-            'residual_trace_max_rel_diff_vs_fp32': float(np.max(np.abs(r16 - r32) / r32))}
+    r32 = out['wfe_residual']
+    forms = {}
+    for prec in ('bf16', 'bf16x3'):
+        atm.generateNewPhaseScreen(10)
+        dm.coefs = np.zeros(dm.nValidAct)
+        run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=5, reconstructor_precision=prec)
+        atm.generateNewPhaseScreen(10)
+        dm.coefs = np.zeros(dm.nValidAct)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        o = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=n_iter, strehl_every=10,
+                            reconstructor_precision=prec)
+        dtp = time.perf_counter() - t0
+        r = o['wfe_residual']
+        forms[prec] = {'value': round(n_iter / dtp, 2), 'unit': 'iterations/s',
+                       'wfe_residual_nm': round(float(np.mean(r[half:])), 1),
+                       'strehl_pct_mean_second_half': round(float(np.mean(o['strehl'][half // 10:])), 2),
+                       'residual_trace_max_rel_diff_vs_fp32': float(np.max(np.abs(r - r32) / r32)),
+                       'residual_trace_max_rel_diff_vs_fp32_first_30': float(np.max(np.abs(r[:30] - r32[:30])
+                                                                                 / r32[:30]))}
+    bf16 = forms['bf16']
     return {'value': round(n_iter / dt, 2), 'unit': 'iterations/s', 'iterations': n_iter, 'nAct': int(dm.nValidAct),
             'nSignal': int(wfs.nSignal), 'modes': n_modes, 'dm_route': 'dense' if dm.use_dense else 'separable',
             'kl_seconds': round(t_kl, 2), 'im_seconds': round(t_im, 2), 'im_cond': round(float(calib.cond), 1),
@@ -153,7 +160,7 @@ def closed_loop_leg(tel, ngs, wfs, dev, n_iter=1000, n_modes=1000):
             'wfe_residual_nm': round(float(np.mean(out['wfe_residual'][half:])), 1),
             'strehl_pct_mean_second_half': round(float(np.mean(out['strehl'][half // 10:])), 2),
             'note': 'wall clock of the loop incl. atmosphere update (host RNG draws) and Strehl every 10 it.',
-            'bf16_reconstructor': bf16}
+            'bf16_reconstructor': bf16, 'split_bf16_reconstructor': forms['bf16x3']}
 
 
 def _wall_ms(fn, reps=10):
@@ -325,15 +332,21 @@ def im_batched_leg(n_modes=1000):
     bc.run_eager()
     torch.cuda.synchronize()
     t_eager = time.perf_counter() - t0
-    bc.capture()
-    bc.run()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    bc.run()
-    e1.record()
-    torch.cuda.synchronize()
-    t_graph = e0.elapsed_time(e1) / 1e3
+    D_eager = [D.clone() for D in bc.run_eager()]
+
+    def graph_time(concurrent):
+        bc.capture(concurrent=concurrent)
+        bc.run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        Ds = bc.run()
+        e1.record()
+        torch.cuda.synchronize()
+        same = all(torch.equal(D, E) for D, E in zip(Ds, D_eager))
+        return e0.elapsed_time(e1) / 1e3, same
+    t_serial, same_serial = graph_time(False)
+    t_graph, same_graph = graph_time(True)
     t0 = time.perf_counter()
     vaults = bc.vaults()
     torch.cuda.synchronize()
@@ -345,6 +358,10 @@ def im_batched_leg(n_modes=1000):
            for cs, t, cv in zip(bc.systems, bc.tuning, vaults)]
     return {'value': round(modes / t_graph, 1), 'unit': 'modes/s (push-pull, 4 sensors, one graph launch)',
             'modes': modes, 'graph_s': round(t_graph, 4), 'eager_s': round(t_eager, 4),
+            'graph_serial_s': round(t_serial, 4), 'graph_speedup_vs_eager': round(t_eager / t_graph, 3),
+            'graph_equals_eager': bool(same_graph and same_serial),
+            'graph_note': 'one branch per system in the graph (forked streams); graph_serial_s records the '
+                          'systems one after another',
             'autotune_s': round(t_tune, 2), 'svd_pinv_f64_s': round(t_svd, 3), 'systems': per}
 
 

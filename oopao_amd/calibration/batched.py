@@ -212,8 +212,12 @@ class BatchedCalibration:
             cs.record()
         return [cs.interaction_matrix for cs in self.systems]
 
-    def capture(self):
-        """Warm up on a side stream, then record every batch of every system into one graph."""
+    def capture(self, concurrent=True):
+        """Warm up on a side stream, then record every batch of every system into one graph. With
+        ``concurrent`` each system is recorded on its own stream, forked from the capture stream and
+        joined back into it, so the graph holds one independent branch per system and the smaller
+        sensors' kernels run beside the larger ones' (the systems share no buffers; every stream has the
+        default priority)."""
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
@@ -221,9 +225,20 @@ class BatchedCalibration:
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
+        if concurrent and len(self.systems) > 1:
+            branches = [torch.cuda.Stream() for _ in self.systems]
         with torch.cuda.graph(self.graph):
-            for cs in self.systems:
-                cs.record()
+            if concurrent and len(self.systems) > 1:
+                cap = torch.cuda.current_stream()
+                for st, cs in zip(branches, self.systems):
+                    st.wait_stream(cap)
+                    with torch.cuda.stream(st):
+                        cs.record()
+                for st in branches:
+                    cap.wait_stream(st)
+            else:
+                for cs in self.systems:
+                    cs.record()
         return self.graph
 
     def run(self):
