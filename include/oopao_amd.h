@@ -100,6 +100,23 @@ int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void*
 int oopao_pwfs_cols(const void* inter, size_t inter_bytes, const void* my, int n, int N, int T, int B, int nchunk,
                     const int* bands, float* part, size_t part_bytes, float* frames, void* stream);
 
+/* Unbinned modulated frames on the separable path (the reference's raw_data,
+ * Pyramid.py:751-757): raw[B][N][N] float32 = sum over points of
+ * |ifft2(fft2(E) mask)|^2, same inputs as oopao_pwfs_frames; workspace of
+ * oopao_pwfs_raw_bytes bytes. */
+int oopao_pwfs_raw_bytes(int n, int N, int T, int B, int nchunk, size_t* bytes);
+int oopao_pwfs_raw(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
+                   const void* ty, const void* mx, const void* my, int n, int N, int T, int B, int nchunk,
+                   int pts_per_task, void* workspace, size_t ws_bytes, float* raw, void* stream);
+
+/* Detector integration by area (the reference's sum binning, tools/tools.py:464-483,
+ * extended to non-integer factors): frames[b][R][C] = sum_{Y,X} w(R,Y) w(C,X) raw[b][Y][X]
+ * with w(R, Y) the overlap of raw pixel [Y, Y+1) with detector pixel [R s, (R+1) s),
+ * s = N / cam (a plain sum over s x s pixels when cam divides N). No counterpart
+ * for a non-integer s in the reference, whose Pyramid refuses such a sampling
+ * (Pyramid.py:216-217). */
+int oopao_bin_area(const float* raw, int B, int N, int cam, float* frames, void* stream);
+
 /* Modulation focal-plane camera (Pyramid.py:638-642 modulation_camera_em,
  * 1183-1190 intensity crop): for B screens,
  *   out[b][crop][crop] = sum_k |fft2(pad(E_b tx_p(k) ty_k))|^2 / N^2

@@ -27,14 +27,18 @@ class Pyramid:
                  old_mask: bool = False, rooftop: float = 0, theta_rotation: float = 0, delta_theta: float = 0.,
                  user_modulation_path: list = None, pupilSeparationRatio: float = None, edgePixel: int = None,
                  zeroPadding: int = None, nchunk: int = None, pts_per_task: int = None,
-                 max_batch_bytes: float = 8e9):
+                 max_batch_bytes: float = 8e9, fractional_sampling: bool = False):
         self.gpu_available = False          # batching stays enabled in InteractionMatrix
         self.nJobs = 1
         self.precision = np.float32
         self.precision_complex = np.complex64
         self.telescope = telescope
         self.device = telescope.device
-        if (self.telescope.resolution / nSubap) % 2 != 0 and self.telescope.resolution / nSubap != 1:
+        # fractional_sampling (no counterpart in the reference, which raises here): a non-integer number of
+        # pupil pixels per subaperture, the detector integrating the unbinned frame by area
+        self.fractional_sampling = bool(fractional_sampling)
+        if (self.telescope.resolution / nSubap) % 2 != 0 and self.telescope.resolution / nSubap != 1 \
+                and not self.fractional_sampling:
             raise OopaoError('The resolution should be an even number and be a multiple of 2**i where i>=2')
         if self.telescope.src is None:
             raise OopaoError('The telescope was not coupled to any source object! Make sure to couple it with '
@@ -78,7 +82,11 @@ class Pyramid:
             raise OopaoError('The use of the zeroPadding property has been deprecated')
         n = self.telescope.resolution
         # geometry (Pyramid.py:276-289)
-        self.resolution = int((self.nSubap * 2 + self.n_pix_separation + self.n_pix_edge * 2) * n / self.nSubap)
+        full = (self.nSubap * 2 + self.n_pix_separation + self.n_pix_edge * 2) * n / self.nSubap
+        if self.fractional_sampling and abs(full - round(full)) > 1e-9 * full:
+            raise OopaoError('fractional_sampling needs an integer padded resolution; '
+                             '(2 nSubap + n_pix_separation + 2 n_pix_edge) n / nSubap = %g' % full)
+        self.resolution = int(round(full)) if self.fractional_sampling else int(full)
         self.zeroPaddingFactor = self.resolution / n
         self.zeroPadding = (self.resolution - n) // 2
         self.tag = 'pyramid'
@@ -463,7 +471,7 @@ class Pyramid:
         Pyramid.py:838-853, 946-961), for a slopes-only frame; None when a measurement needs the whole
         frame (another post-processing mode, the general path, detector binning)."""
         if (self.postProcessing != 'slopesMaps' or getattr(self, 'engine', None) is None
-                or self.engine.mode != 'fast' or self.binning != 1):
+                or self.engine.mode != 'fast' or self.engine.fractional or self.binning != 1):
             return None
         quad, w = self._quadrant_origins()
         return [(c, c + w) for c in sorted(set(quad[4:]))]
@@ -829,7 +837,8 @@ class Pyramid:
     @property
     def raw_data(self):
         """Unbinned modulated frame [N, N] of the last measurement (of its last screen when batched),
-        Pyramid.py:751-757, 820-821; computed on demand by the general HIP path."""
+        Pyramid.py:751-757, 820-821; computed on demand (the separable path's unbinned frame, or the
+        general path)."""
         if self._last_opd is None:
             return None
         if self._raw_cache is None:

@@ -39,6 +39,10 @@ SIGNATURES = {
                                 c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
     'oopao_pwfs_cols': (c_int, [c_void_p, c_size_t, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p,
                                 c_void_p, c_size_t, c_void_p, c_void_p]),
+    'oopao_pwfs_raw_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
+    'oopao_pwfs_raw': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                               c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p]),
+    'oopao_bin_area': (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
     'oopao_pwfs_focal_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_focal': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int,
                                  c_int, c_int, c_int, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p]),

@@ -278,6 +278,101 @@ k_cols_focal(const float2* __restrict__ ws, const float2* __restrict__ tw, float
   }
 }
 
+// ---------------------------------------------------------------------------
+// Unbinned frame (the reference's raw_data, Pyramid.py:751-757) on the separable path. One lane group
+// per (screen b, point chunk, pupil-plane column X): the column's transform pair for every point of the
+// chunk, |E|^2 summed in registers, written unbinned: part[b][chunk][X][Y].
+// ---------------------------------------------------------------------------
+template <int N, int TPT>
+__global__ void __launch_bounds__(64 * kWaves)
+k_cols_raw(const float2* __restrict__ ws, const float2* __restrict__ my, const float2* __restrict__ tw,
+           float* __restrict__ part, int B, int T, int nchunk) {
+  constexpr int n = N / 4;
+  constexpr int E = N / TPT;
+  constexpr int G = 64 / TPT;
+  constexpr int NG = N / 4;
+  __shared__ float2 lds[kWaves * G][lds_len(N)];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int h = lane / TPT;
+  const int t = lane % TPT;
+  const long total = (long)B * nchunk * N;
+  long task = ((long)blockIdx.x * kWaves + wave) * G + h;
+  const bool active = task < total;
+  if (!active) task = total - 1;      // the lane group repeats a valid column and writes nothing
+  const int X = (int)(task % N);
+  const int chunk = (int)((task / N) % nchunk);
+  const int b = (int)(task / ((long)N * nchunk));
+  const int i0 = (int)(((long)chunk * T) / nchunk);
+  const int i1 = (int)(((long)(chunk + 1) * T) / nchunk);
+  float2* xl = lds[wave * G + h];
+  float acc[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) acc[m] = 0.f;
+  for (int i = i0; i < i1; ++i) {
+    const float2* tile = ws + (((size_t)b * T + i) * NG + (X >> 2)) * (size_t)n * 4;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + (X & 3)];
+    fft_fwd_pruned<N, TPT, 4, true>(v, xl, tw, t);
+#pragma unroll
+    for (int m = 0; m < E; ++m) v[m] = cmul(v[m], launder(my)[t + TPT * m]);
+    fft_full<N, TPT, +1, true>(v, xl, tw, t);
+#pragma unroll
+    for (int m = 0; m < E; ++m) acc[m] += cabs2(v[m]);
+  }
+  if (!active) return;
+  float* dst = part + (((size_t)b * nchunk + chunk) * N + X) * (size_t)N;
+#pragma unroll
+  for (int m = 0; m < E; ++m) dst[t + TPT * m] = acc[m];
+}
+
+// raw[b][Y][X] = sum_k part[b][k][X][Y] (fixed order over chunks), through a 32 x 32 LDS tile so that
+// both the column reads and the row writes are coalesced
+__global__ void __launch_bounds__(256) k_raw_reduce(const float* __restrict__ part, float* __restrict__ raw, int N,
+                                                    int nchunk) {
+  __shared__ float tile[32][33];
+  const int b = blockIdx.z;
+  const int X0 = blockIdx.y * 32, Y0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int X = X0 + r, Y = Y0 + tx;
+    float s = 0.f;
+    if (X < N && Y < N)
+      for (int k = 0; k < nchunk; ++k) s += part[(((size_t)b * nchunk + k) * N + X) * (size_t)N + Y];
+    tile[r][tx] = s;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int Y = Y0 + r, X = X0 + tx;
+    if (X < N && Y < N) raw[((size_t)b * N + Y) * N + X] = tile[tx][r];
+  }
+}
+
+// Detector integration by area: detector pixel R covers raw pixels [R s, (R + 1) s), s = N / cam, and
+// takes each raw pixel's flux in proportion to the overlap (a sum over whole pixels when cam divides N,
+// the reference's set_binning, tools/tools.py:464-483). frames[b][R][C] = sum w(R, Y) w(C, X) raw[b][Y][X].
+__global__ void __launch_bounds__(256) k_bin_area(const float* __restrict__ raw, int N, int cam, double s,
+                                                  float* __restrict__ frames, int B) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)B * cam * cam) return;
+  const int C = (int)(idx % cam);
+  const int R = (int)((idx / cam) % cam);
+  const int b = (int)(idx / ((long)cam * cam));
+  const double r0 = R * s, r1 = (R + 1) * s, c0 = C * s, c1 = (C + 1) * s;
+  const int y0 = (int)floor(r0), y1 = min(N, (int)ceil(r1));
+  const int x0 = (int)floor(c0), x1 = min(N, (int)ceil(c1));
+  const float* f = raw + (size_t)b * N * N;
+  double acc = 0.0;
+  for (int Y = y0; Y < y1; ++Y) {
+    const double wy = fmin((double)(Y + 1), r1) - fmax((double)Y, r0);
+    double row = 0.0;
+    for (int X = x0; X < x1; ++X) row += (fmin((double)(X + 1), c1) - fmax((double)X, c0)) * (double)f[(size_t)Y * N + X];
+    acc += wy * row;
+  }
+  frames[idx] = (float)acc;
+}
+
 // out[b][p] = sum_k part[b][k][p], p < count  (fixed summation order)
 __global__ void k_sum_chunks(const float* __restrict__ part, float* __restrict__ out, int B, int nchunk, int count) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -464,6 +559,28 @@ static int launch_focal(const float2* inter, int T, int B, int nchunk, int crop,
                      crop * crop);
   OOPAO_LAUNCH_CHECK();
   return OK;
+}
+
+template <int N>
+static int launch_raw(const float2* inter, const float2* my, int T, int B, int nchunk, float* part, float* raw,
+                      hipStream_t st) {
+  constexpr int TPT = fast_tpt(N);
+  constexpr int G = 64 / TPT;
+  const float2* tw = nullptr;
+  int rc = twiddles(N, &tw);
+  if (rc) return rc;
+  const long groups = (long)B * nchunk * N;
+  const long waves = (groups + G - 1) / G;
+  hipLaunchKernelGGL((k_cols_raw<N, TPT>), dim3((unsigned)((waves + kWaves - 1) / kWaves)), dim3(64 * kWaves), 0,
+                     st, inter, my, tw, part, B, T, nchunk);
+  OOPAO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_raw_reduce, dim3((N + 31) / 32, (N + 31) / 32, B), dim3(256), 0, st, part, raw, N, nchunk);
+  OOPAO_LAUNCH_CHECK();
+  return OK;
+}
+
+static size_t ws_raw_part_bytes(int N, int B, int nchunk) {
+  return align256((size_t)B * nchunk * (size_t)N * N * sizeof(float));
 }
 
 static bool fast_supported(int n, int N, int bin) {
@@ -698,6 +815,57 @@ int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void
   }
   set_error("unsupported padded length N=%d", N);
   return ERR_UNSUPPORTED;
+}
+
+int oopao_pwfs_raw_bytes(int n, int N, int T, int B, int nchunk, size_t* bytes) {
+  OOPAO_CHECK_ARG(bytes != nullptr, "bytes is null");
+  OOPAO_CHECK_ARG(n > 0 && N > 0 && T > 0 && B > 0 && nchunk > 0 && nchunk <= T,
+                  "invalid sizes n=%d N=%d T=%d B=%d nchunk=%d", n, N, T, B, nchunk);
+  *bytes = ws_inter_bytes(n, N, T, B) + ws_raw_part_bytes(N, B, nchunk);
+  return OK;
+}
+
+int oopao_pwfs_raw(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
+                   const void* ty, const void* mx, const void* my, int n, int N, int T, int B, int nchunk,
+                   int pts_per_task, void* workspace, size_t ws_bytes, float* raw, void* stream) {
+  OOPAO_CHECK_ARG(opd && amp && tx && pstart && ty && mx && my && workspace && raw, "null pointer argument");
+  OOPAO_CHECK_ARG(P > 0 && P <= T && B > 0 && B <= 65535 && nchunk > 0 && nchunk <= T && pts_per_task > 0,
+                  "invalid P=%d T=%d B=%d nchunk=%d pts_per_task=%d", P, T, B, nchunk, pts_per_task);
+  int rc = check_geometry(n, N, 4);
+  if (rc) return rc;
+  const size_t ib = ws_inter_bytes(n, N, T, B), pb = ws_raw_part_bytes(N, B, nchunk);
+  if (ws_bytes < ib + pb) {
+    set_error("workspace too small: %zu < %zu bytes", ws_bytes, ib + pb);
+    return ERR_WORKSPACE;
+  }
+  char* w = reinterpret_cast<char*>(workspace);
+  rc = oopao_pwfs_rows(opd, scale, amp, tx, P, pstart, ty, mx, n, N, T, B, pts_per_task, nullptr, w, ib, stream);
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const float2* ip = reinterpret_cast<const float2*>(w);
+  const float2* myp = reinterpret_cast<const float2*>(my);
+  float* part = reinterpret_cast<float*>(w + ib);
+  switch (N) {
+#define OOPAO_CASE(NN) \
+  case NN:             \
+    return launch_raw<NN>(ip, myp, T, B, nchunk, part, raw, st);
+    OOPAO_FAST_SIZES(OOPAO_CASE)
+#undef OOPAO_CASE
+    default:
+      break;
+  }
+  set_error("unsupported padded length N=%d", N);
+  return ERR_UNSUPPORTED;
+}
+
+int oopao_bin_area(const float* raw, int B, int N, int cam, float* frames, void* stream) {
+  OOPAO_CHECK_ARG(raw && frames, "null pointer argument");
+  OOPAO_CHECK_ARG(B > 0 && N > 0 && cam > 0 && cam <= N, "invalid sizes B=%d N=%d cam=%d", B, N, cam);
+  const long total = (long)B * cam * cam;
+  hipLaunchKernelGGL(k_bin_area, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), raw, N, cam, (double)N / (double)cam, frames, B);
+  OOPAO_LAUNCH_CHECK();
+  return OK;
 }
 
 int oopao_pwfs_slopes_workspace_bytes(int B, int w, size_t* bytes) {

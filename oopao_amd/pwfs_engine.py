@@ -137,10 +137,14 @@ class PWFSEngine:
         self.bin = N // cam if cam and N % cam == 0 else 0
         L = _lib.lib()
         self.fast_geometry = bool(L.oopao_pwfs_supported(n, N, self.bin))
-        if not self.fast_geometry and not L.oopao_pwfs_generic_supported(n, N, self.bin):
+        # a detector that does not divide N (non-integer pixels per subaperture): the separable transforms
+        # give the unbinned frame, integrated onto the detector by area (oopao_bin_area)
+        self.fractional = bool(cam) and N % cam != 0 and bool(L.oopao_pwfs_supported(n, N, 4))
+        if not (self.fast_geometry or self.fractional) and not L.oopao_pwfs_generic_supported(n, N, self.bin):
             raise OopaoError('PWFS geometry n=%d N=%d binning=%d is not supported by the HIP path '
-                             '(needs n <= N <= 2048 and a detector size dividing N)' % (n, N, self.bin))
-        self.mode = 'fast' if self.fast_geometry else 'generic'
+                             '(needs n <= N <= 2048 and a detector size dividing N, or N = 4 n on a compiled '
+                             'length for a non-integer binning)' % (n, N, self.bin))
+        self.mode = 'fast' if (self.fast_geometry or self.fractional) else 'generic'
         _lib.prepare_tables(N, self.device)
         self.nchunk_user = nchunk
         self.pts_per_task_user = pts_per_task
@@ -212,7 +216,8 @@ class PWFSEngine:
         geometry selects the two-pass kernels; anything else the general path with the full mask."""
         m = np.asarray(m, np.float64)
         self._fullT = None
-        f = separable_mask_factors(m) if (self.fast_geometry and factor is None and not general) else None
+        f = separable_mask_factors(m) if ((self.fast_geometry or self.fractional) and factor is None
+                                          and not general) else None
         if f is not None:
             my, mx = mask_vectors(f[0], f[1], self.N, self.n)
             self.my = torch.from_numpy(my.astype(np.complex64)).to(self.device)
@@ -220,6 +225,9 @@ class PWFSEngine:
             self.mode = 'fast'
             self.maskT = None
             return
+        if self.fractional:
+            raise OopaoError('a non-integer detector sampling needs the separable pyramid mask (no rooftop, '
+                             'rotation, per-quadrant shift, spatial filter or old mask)')
         full = np.complex64(np.exp(1j * m)) if mask is None else np.asarray(mask)
         full = full.astype(np.complex128)
         if factor is not None:
@@ -260,6 +268,10 @@ class PWFSEngine:
     def bytes_per_screen(self):
         """Workspace bytes per phase screen of the current mode (for batch sizing)."""
         nb = ctypes.c_size_t(0)
+        if self.mode == 'fast' and self.fractional:
+            nchunk, _ = self._knobs(1)
+            _lib.call('oopao_pwfs_raw_bytes', self.n, self.N, self.T, 1, nchunk, ctypes.byref(nb))
+            return nb.value + 4 * self.N * self.N
         if self.mode == 'fast':
             nchunk, _ = self._knobs(1)
             _lib.call('oopao_pwfs_workspace_bytes', self.n, self.N, self.T, 1, nchunk, ctypes.byref(nb))
@@ -270,14 +282,19 @@ class PWFSEngine:
 
     def frames(self, opd, scale, amp, out=None, raw=None, weighted=False, bands=None):
         """Detector frames [B, cam, cam] float32 for OPD/phase maps opd [B, n, n] (phase = opd*scale).
-        ``raw`` (a float32 [B, N, N] tensor) also receives the unbinned frames (general path);
-        ``weighted`` uses the per-point weights of set_weights. ``bands`` (fast path): detector-column
-        intervals [(start, stop), ...] to compute, the other columns are returned as zero (the
-        slopesMaps signal needs only the quadrant columns); ignored on the general path."""
+        ``raw`` (a float32 [B, N, N] tensor) also receives the unbinned frames (the binned frames then
+        come from them); ``weighted`` uses the per-point weights of set_weights. ``bands`` (separable path
+        with 4 x 4 binning): detector-column intervals [(start, stop), ...] to compute, the other columns
+        are returned as zero (the slopesMaps signal needs only the quadrant columns); ignored otherwise.
+        With a non-integer binning (``fractional``) the unbinned frames are integrated by area."""
         opd = self._check(opd, amp)
         B = opd.shape[0]
         if out is None:
             out = torch.empty((B, self.cam, self.cam), dtype=torch.float32, device=self.device)
+        if self.mode == 'fast' and (self.fractional or raw is not None):
+            r = self._raw_fast(opd, scale, amp, weighted, raw)
+            _lib.call('oopao_bin_area', _ptr(r), B, self.N, self.cam, _ptr(out), _stream_ptr(self.device))
+            return out
         if self.mode == 'generic' or raw is not None:
             return self._frames_generic(opd, scale, amp, out, raw, self._tx(weighted))
         ty = self._ty_sorted(weighted)
@@ -322,9 +339,27 @@ class PWFSEngine:
                   _stream_ptr(self.device))
         return out
 
+    def _raw_fast(self, opd, scale, amp, weighted, raw=None):
+        """Unbinned frames [B, N, N] on the separable path (oopao_pwfs_raw)."""
+        B = opd.shape[0]
+        if raw is None:
+            raw = torch.empty((B, self.N, self.N), dtype=torch.float32, device=self.device)
+        elif raw.dtype != torch.float32 or tuple(raw.shape) != (B, self.N, self.N) or not raw.is_contiguous():
+            raise OopaoError('raw must be a contiguous float32 [B, N, N] tensor')
+        nchunk, ppt = self._knobs(B)
+        nb = ctypes.c_size_t(0)
+        _lib.call('oopao_pwfs_raw_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
+        ws = self._scratch(nb.value)
+        _lib.call('oopao_pwfs_raw', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
+                  _ptr(self.pstart), _ptr(self._ty_sorted(weighted)), _ptr(self.mx), _ptr(self.my), self.n, self.N,
+                  self.T, B, nchunk, ppt, _ptr(ws), ctypes.c_size_t(nb.value), _ptr(raw), _stream_ptr(self.device))
+        return raw
+
     def raw(self, opd, scale, amp, weighted=False):
         """Unbinned modulated frames [B, N, N] (the reference's raw_data, Pyramid.py:751-757)."""
         opd = self._check(opd, amp)
+        if self.mode == 'fast':
+            return self._raw_fast(opd, scale, amp, weighted)
         B = opd.shape[0]
         raw = torch.empty((B, self.N, self.N), dtype=torch.float32, device=self.device)
         out = torch.empty((B, self.cam, self.cam), dtype=torch.float32, device=self.device)
@@ -341,7 +376,7 @@ class PWFSEngine:
         B = opd.shape[0]
         out = torch.empty((B, crop, crop), dtype=torch.float32, device=self.device)
         nb = ctypes.c_size_t(0)
-        if self.fast_geometry:
+        if self.fast_geometry or self.fractional:
             nchunk, ppt = self._knobs(B)
             _lib.call('oopao_pwfs_focal_bytes', self.n, self.N, self.T, B, nchunk, crop, ctypes.byref(nb))
             ws = self._scratch(nb.value)
@@ -360,8 +395,8 @@ class PWFSEngine:
     def adjoint(self, opd, scale, amp, gframe, weighted=False):
         """dL/dopd [B, n, n] for the cotangent gframe = dL/dframes [B, cam, cam] of
         frames(opd, scale, amp, weighted=weighted) (fast path; the modulation points are recomputed)."""
-        if self.mode != 'fast':
-            raise OopaoError('the PWFS adjoint is implemented for the separable fast path only')
+        if self.mode != 'fast' or self.fractional:
+            raise OopaoError('the PWFS adjoint is implemented for the separable fast path with 4 x 4 binning only')
         opd = self._check(opd, amp)
         B = opd.shape[0]
         gframe = gframe.reshape(B, self.cam, self.cam)

@@ -84,14 +84,16 @@ class Geometry:
     """Pyramid geometry (Pyramid.py:216-217, 260-289, 292-296)."""
 
     def __init__(self, n, nSubap, n_pix_separation=2.0, n_pix_edge=None, calibModulation=50.0,
-                 binning=1):
-        if (n / nSubap) % 2 != 0 and n / nSubap != 1:
+                 binning=1, fractional=False):
+        # fractional: a non-integer n / nSubap (the reference raises; the detector then integrates by area)
+        if (n / nSubap) % 2 != 0 and n / nSubap != 1 and not fractional:
             raise ValueError('The resolution should be an even number and be a multiple of 2**i where i>=2')
         self.n = n
         self.nSubap = nSubap
         self.sep = n_pix_separation
         self.edge = n_pix_separation // 2 if n_pix_edge is None else n_pix_edge
-        self.N = int((nSubap * 2 + self.sep + self.edge * 2) * n / nSubap)
+        full = (nSubap * 2 + self.sep + self.edge * 2) * n / nSubap
+        self.N = int(round(full)) if fractional else int(full)
         self.zeroPaddingFactor = self.N / n
         self.center = self.N // 2
         self.cam = round(nSubap * self.zeroPaddingFactor)
@@ -239,6 +241,21 @@ def set_binning(array, binning_factor):
     return array.reshape(shape).sum(-1).sum(1)
 
 
+def bin_area(array, cam):
+    """Detector integration by area onto cam x cam pixels: detector pixel R covers [R s, (R + 1) s) with
+    s = N / cam and takes each raw pixel in proportion to the overlap; the sum binning of set_binning
+    when cam divides N (tools/tools.py:464-483), flux-conserving for any s."""
+    a = np.asarray(array, dtype=np.float64)
+    N = a.shape[0]
+    s = N / cam
+    W = np.zeros((cam, N))
+    for R in range(cam):
+        lo, hi = R * s, (R + 1) * s
+        for Y in range(int(np.floor(lo)), min(N, int(np.ceil(hi)))):
+            W[R, Y] = min(Y + 1, hi) - max(Y, lo)
+    return W @ a @ W.T
+
+
 def grab_quadrant(frame, k, geom):
     """Quadrant k of the detector frame (Pyramid.py:946-961)."""
     b = geom.binning
@@ -284,8 +301,8 @@ class PyramidOracle:
                  nTheta_user_defined=None, psfCentering=True, precision='fp64', n_jobs=1,
                  rooftop=0.0, theta_rotation=0.0, delta_theta=0.0, user_modulation_path=None,
                  validI4Q=None, sx=(0, 0, 0, 0), sy=(0, 0, 0, 0), spatial_filter=None, weight_vector=None,
-                 old_mask=False):
-        self.geom = Geometry(n, nSubap, n_pix_separation, n_pix_edge, calibModulation)
+                 old_mask=False, fractional=False):
+        self.geom = Geometry(n, nSubap, n_pix_separation, n_pix_edge, calibModulation, fractional=fractional)
         self.precision = precision
         self.psfCentering = psfCentering
         self.pupil = pupil
@@ -392,8 +409,11 @@ class PyramidOracle:
                      N // 2 - resolution // 2:N // 2 + resolution // 2]
 
     def frame(self, phase):
-        """Detector frame: sum-binning, QE = gain = 1, float64 (Pyramid.py:1193-1203, Detector.py:273-354)."""
+        """Detector frame: sum-binning (by area for a non-integer binning), QE = gain = 1, float64
+        (Pyramid.py:1193-1203, Detector.py:273-354)."""
         raw = self.raw(phase)
+        if self.geom.N % self.geom.cam:
+            return bin_area(raw, self.geom.cam)
         return set_binning(raw, self.geom.N / self.geom.cam).astype(np.float64)
 
     def measure(self, phase):

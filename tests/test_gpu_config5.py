@@ -108,3 +108,58 @@ def test_elt_bf16_contractions_vs_float64():
         ratio, e = _row_bound_check(c[rrows].double().cpu().numpy(), exact, S, nsig, bits, float(s.abs().max()))
         print('reconstructor MVM %s: rel-L2 %.2e vs float64, worst error / bound %.3f' % (prec, e, ratio))
         assert ratio <= 1.0
+
+
+def test_elt_pwfs_4p8_px_per_subaperture_vs_oracle():
+    # the configuration as specified: 100 x 100 subapertures on the 480 px pupil (4.8 px each), N = 1920,
+    # 96 points; the reference refuses this sampling (Pyramid.py:216-217), so the drop-in takes it with
+    # fractional_sampling=True: the separable transforms give the unbinned frame, integrated onto the
+    # 400 x 400 detector by area; the oracle is the float64 restatement with the same area integration
+    # (oracle/pwfs.bin_area, pinned by exact sub-pixel refinement)
+    from oopao_amd import Pyramid
+    from oracle import pwfs as opw
+    from tests.helpers import random_phase
+    print('ELT PWFS at 4.8 px per subaperture: n=480, N=1920, cam=400, 96 points', flush=True)
+    tel, ngs = _elt()
+    wfs = Pyramid(100, tel, 3, 0.5, n_pix_separation=100, n_pix_edge=50, nTheta_user_defined=96,
+                  fractional_sampling=True)
+    assert wfs.engine.N == 1920 and wfs.cam.resolution == 400 and wfs.engine.fractional
+    assert wfs.slope_bands() is None
+    flux = opw.flux_map(tel.pupil, ngs.nPhoton, tel.samplingTime, tel.D)
+    o = opw.PyramidOracle(480, 100, 3, 0.5, tel.pupil, flux, ngs.wavelength, n_pix_separation=100, n_pix_edge=50,
+                          nTheta_user_defined=96, n_jobs=16, fractional=True)
+    try:
+        assert np.array_equal(wfs.validI4Q, o.validI4Q) and wfs.nSignal == o.nSignal
+        opd = random_phase(480, 22, 1.0, tel.pupil) * 60e-9
+        od = torch.from_numpy((opd * tel.pupil).astype(np.float32)).to(wfs.device)[None]
+        frame = wfs.frames_dev(od)[0].double().cpu().numpy()
+        assert abs(frame.sum() / flux.sum() - 1) < 1e-5
+        raw_ref = o.raw(opd * tel.pupil * 2 * np.pi / o.wavelength)
+        ref = opw.bin_area(raw_ref, 400)
+        e_frame = rel_l2(frame, ref)
+        raw = wfs.engine.raw(od, 2 * np.pi / ngs.wavelength, wfs._amp())[0].double().cpu().numpy()
+        e_raw = rel_l2(raw, raw_ref)
+        s2d, s = wfs.slopes_engine.slopes(torch.from_numpy(frame.astype(np.float32)).to(wfs.device)[None])
+        e_s = rel_l2(s[0].double().cpu().numpy(), o.measure_opd(opd)[1])
+        print('4.8 px: raw rel-L2 %.2e, frame %.2e, slopes %.2e vs float64 oracle' % (e_raw, e_frame, e_s))
+        assert e_raw <= 1e-5 and e_frame <= 1e-5 and e_s <= 1e-5
+    finally:
+        if o._pool is not None:
+            o._pool.shutdown()
+
+
+@pytest.mark.parametrize('N,cam', [(1920, 400), (1280, 320), (320, 80), (264, 44)])
+def test_bin_area_kernel(N, cam):
+    import ctypes
+    from oopao_amd import _lib
+    from oracle import pwfs as opw
+    dev = torch.device('cuda', 0)
+    rs = np.random.RandomState(N)
+    raw = rs.rand(2, N, N).astype(np.float32)
+    out = torch.empty((2, cam, cam), dtype=torch.float32, device=dev)
+    r = torch.from_numpy(raw).to(dev)
+    _lib.call('oopao_bin_area', ctypes.c_void_p(r.data_ptr()), 2, N, cam, ctypes.c_void_p(out.data_ptr()),
+              ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
+    got = out.double().cpu().numpy()
+    for b in range(2):
+        assert rel_l2(got[b], opw.bin_area(raw[b].astype(np.float64), cam)) <= 1e-7

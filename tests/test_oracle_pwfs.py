@@ -247,3 +247,26 @@ def test_old_mask_closed_form():
     step = 2 * np.pi / h
     assert np.allclose(np.diff(q1, axis=1), (L + sx[0]) * norma * step)
     assert np.allclose(np.diff(q1, axis=0), (L - sy[0]) * norma * step)
+
+
+def test_bin_area_pins():
+    # integer factors: the reference's sum binning exactly (tools/tools.py:464-483)
+    rs = np.random.RandomState(3)
+    a = rs.rand(24, 24)
+    assert np.allclose(opw.bin_area(a, 6), opw.set_binning(a, 4), rtol=0, atol=1e-13)
+    # rational factor s = 24/5 = 4.8 (the ELT sensor's 4.8 px per subaperture): refine each raw pixel into
+    # 5 x 5 equal sub-pixels, then a detector pixel is exactly 24 x 24 sub-pixels -- an independent route
+    fine = np.kron(a, np.ones((5, 5))) / 25.0
+    assert np.allclose(opw.bin_area(a, 5), opw.set_binning(fine, 24), rtol=0, atol=1e-13)
+    # flux is conserved for any factor
+    for cam in (7, 9, 11, 13):
+        assert abs(opw.bin_area(a, cam).sum() - a.sum()) < 1e-12 * a.sum()
+
+
+def test_fractional_geometry_config5():
+    # 100 x 100 subapertures on a 480 px pupil (4.8 px each), separation 100, edge 50: N = 1920, cam = 400
+    g = opw.Geometry(480, 100, 100, 50, fractional=True)
+    assert (g.N, g.cam) == (1920, 400)
+    assert g.N / g.cam == 4.8
+    with pytest.raises(ValueError):
+        opw.Geometry(480, 100, 100, 50)          # the reference's rule without the extension
