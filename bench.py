@@ -276,6 +276,18 @@ def extended_leg(dev):
     out['elt_pwfs_128x128'] = {'ms_per_frame': round(_wall_ms(elt2_frame, reps=10), 3), 'N': int(wfs_elt2.nRes),
                                'nTheta': int(wfs_elt2.nTheta), 'nSignal': int(wfs_elt2.nSignal),
                                'path': wfs_elt2.engine.mode}
+    # the config-5 sensor as specified: 100 x 100 subapertures at 4.8 px on the 480 px pupil (unbinned
+    # frame on the separable path, detector integration by area)
+    wfs_elt3 = Pyramid(100, elt, 3, 0.5, n_pix_separation=100, n_pix_edge=50, nTheta_user_defined=96,
+                       fractional_sampling=True)
+
+    def elt3_frame():
+        elt.OPD_no_pupil = opd_elt
+        ngs3 * elt * wfs_elt3
+    out['elt_pwfs_100x100_4p8px'] = {'ms_per_frame': round(_wall_ms(elt3_frame, reps=5), 3),
+                                     'N': int(wfs_elt3.nRes), 'cam': int(wfs_elt3.cam.resolution),
+                                     'nTheta': int(wfs_elt3.nTheta), 'nSignal': int(wfs_elt3.nSignal),
+                                     'path': 'separable, area-integrated detector'}
     return out
 
 
