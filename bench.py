@@ -163,6 +163,32 @@ def closed_loop_leg(tel, ngs, wfs, dev, n_iter=1000, n_modes=1000):
             'bf16_reconstructor': bf16, 'split_bf16_reconstructor': forms['bf16x3']}
 
 
+def atmosphere_leg(tel, n_steps=200):
+    """Config-2 atmosphere updates (2 layers, 10 and 20 m/s at 1 kHz: 0.4 / 0.8 px per step, screens
+    with 326^2 supports): frozen flow and boiling layers (t_boiling 5 and 10 ms), with the reference's
+    host RandomState draws and with the device counter-based draws; wall clock per update incl. the
+    summed OPD (steps/s)."""
+    import torch
+    from oopao_amd import Atmosphere
+    out = {}
+    for name, t_b, rng in (('frozen_host_rng', None, 'host'), ('frozen_device_rng', None, 'device'),
+                           ('boiling_host_rng', [0.005, 0.01], 'host'), ('boiling_device_rng', [0.005, 0.01], 'device')):
+        atm = Atmosphere(tel, 0.15, 25, windSpeed=[10, 20], fractionalR0=[0.7, 0.3], windDirection=[0, 90],
+                         altitude=[0, 10000], t_boiling=t_b, rng=rng)
+        atm.initializeAtmosphere(tel)
+        atm.generateNewPhaseScreen(10)
+        for _ in range(5):
+            atm.update()
+        torch.cuda.synchronize()
+        steps = n_steps if rng == 'device' or t_b is None else n_steps // 4
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            atm.update()
+        torch.cuda.synchronize()
+        out[name] = {'steps_per_s': round(steps / (time.perf_counter() - t0), 1), 'steps': steps}
+    return out
+
+
 def _wall_ms(fn, reps=10):
     import torch
     fn()
@@ -573,7 +599,8 @@ def main():
     ap.add_argument('--batch', type=int, default=8, help='screens per call for the batched form')
     ap.add_argument('--no-extra', action='store_true',
                     help='skip the closed-loop, interaction-matrix, KL-basis, adjoint and contraction legs')
-    ap.add_argument('--legs', default='closed_loop,interaction_matrix,im_batched,kl_basis,adjoint,contractions,extended',
+    ap.add_argument('--legs', default='closed_loop,interaction_matrix,im_batched,kl_basis,adjoint,contractions,extended,'
+                                      'atmosphere',
                     help='comma-separated extra legs to run')
     ap.add_argument('--calib-copy', action='store_true',
                     help='also run one elementwise copy of 256 MiB (counter calibration for rocprof --pmc)')
@@ -743,6 +770,8 @@ def main():
             extra['contractions'] = contraction_leg(tel, dev)
         if 'extended' in legs:
             extra['extended'] = extended_leg(dev)
+        if 'atmosphere' in legs:
+            extra['atmosphere'] = atmosphere_leg(tel)
 
     out = {
         'metric': 'PWFS frames/s at 80x80 subaps, 72 mod pts',

@@ -330,6 +330,18 @@ int oopao_atm_minmax(const double* map, int count, double* mm, void* stream);
  *   sub [nsub][4] (Re c, Im c, fx, fy) of the subharmonic terms, xc [M] their grid coordinates.
  * Workspace of oopao_atm_boil_bytes(M) bytes. */
 int oopao_atm_boil_bytes(int M, size_t* bytes);
+/* The subharmonic rows of oopao_atm_boil from draws on the device (device RNG mode):
+ * sub[t] = (g[re_t] a_t, g[im_t] a_t, fx_t, fy_t) for table[t] = (re_t, im_t, a_t, fx_t, fy_t),
+ * the indices re_t, im_t < 54 (the first draws of the screen's stream, phaseStats.py:225-263). */
+int oopao_atm_boil_subharmonics(const double* g, long long ng, const double* table, int nsub, double* sub,
+                                void* stream);
+
+/* Device RNG mode of the atmosphere (no counterpart in the reference, which draws on the host with
+ * numpy's RandomState): out[j] = normal j of draw `draw` on stream `stream` under the key
+ * (key0, key1), Philox4x32-10 (Salmon et al., SC 2011) with counter (j / 2, draw, stream) and a
+ * Box-Muller pair of 53-bit uniforms. */
+int oopao_normal_philox(double* out, long long count, unsigned key0, unsigned key1, unsigned stream,
+                        unsigned long long draw, void* hip_stream);
 int oopao_atm_boil(double* map, int M, const double* g, const double* amp, const void* W, const double* sub,
                    int nsub, const double* xc, double alpha, double beta, void* ws, size_t ws_bytes, void* stream);
 

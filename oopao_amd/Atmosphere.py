@@ -33,7 +33,8 @@ from numpy.random import RandomState
 from . import _lib
 from ._device import ptr, stream_ptr, to_host
 from .phaseStats import (ft_sh_phase_screen, ft_phase_screen, makeCovarianceMatrix, extrusion_rings,
-                         extrusion_operators, centred_dft_matrix, ft_amplitude, subharmonic_terms)
+                         extrusion_operators, centred_dft_matrix, ft_amplitude, subharmonic_terms,
+                         subharmonic_table)
 from .tools import OopaoError, warning
 
 
@@ -46,9 +47,15 @@ class Atmosphere:
                  altitude: list, src=None, param=None, elevation: float = 90.0, mode: float = 2,
                  angular_spectrum_propagation: bool = False, geometric_phase_backup: bool = False,
                  unwrap_diffractive_phase=False, rytov_var: float = None, rytov_wvl: float = 500e-9,
-                 t_boiling=None):
+                 t_boiling=None, rng: str = 'host'):
         self.tag = 'atmosphere'
         self.device = telescope.device
+        # where the per-step draws come from: 'host' = each layer's numpy RandomState in the reference's
+        # order (a seeded run reproduces the reference's screens); 'device' = counter-based Philox4x32-10
+        # draws on the GPU keyed by the same seeds (same statistics, other numbers, no host work per step)
+        if rng not in ('host', 'device'):
+            raise OopaoError("rng must be 'host' or 'device', got %r" % (rng,))
+        self.rng = rng
         self.hasNotBeenInitialized = True
         self.angular_spectrum_propagation = bool(angular_spectrum_propagation)
         self.geometric_phase_backup = geometric_phase_backup
@@ -133,6 +140,7 @@ class Atmosphere:
         lay.n_extra_pixel = self.n_extra_pixel
         lay.t_boiling = self.t_boiling[i_layer]
         lay.boiling_seed = 100000 + i_layer * 100000      # Atmosphere.py:329-333
+        lay.rng_key, lay.rng_draw = (42 + i_layer * 1000) & 0xFFFFFFFF, 0
         res, M = lay.resolution, lay.resolution + self.n_extra_pixel
         lay.OPD_host = ft_sh_phase_screen(self.r0, self.L0, res, lay.D / res, seed=i_layer)
         lay.seed = i_layer
@@ -180,7 +188,7 @@ class Atmosphere:
             raise OopaoError('The Atmosphere object needs to be initialised using the initialiseAtmosphere()')
         out = {'nLayer': np.int64(self.nLayer), 'r0': np.float64(self.r0), 'L0': np.float64(self.L0),
                'resolution': np.int64(self.telescope.resolution),
-               'samplingTime': np.float64(self.telescope.samplingTime)}
+               'samplingTime': np.float64(self.telescope.samplingTime), 'rng': np.str_(self.rng)}
         for i in range(self.nLayer):
             lay = getattr(self, 'layer_' + str(i + 1))
             key, pos, has_gauss, cached = lay.randomState.get_state()[1:]
@@ -198,6 +206,8 @@ class Atmosphere:
             for name, val in self._layer_params(i).items():
                 out[p + name] = np.float64(val)
             out[p + 'boiling_seed'] = np.int64(lay.boiling_seed)
+            out[p + 'rng_key'] = np.int64(lay.rng_key)
+            out[p + 'rng_draw'] = np.int64(lay.rng_draw)
         np.savez(path, **out)
 
     def _layer_params(self, i):
@@ -219,6 +229,9 @@ class Atmosphere:
                     or float(f['r0']) != float(self.r0) or float(f['L0']) != float(self.L0):
                 raise OopaoError('the saved atmosphere state does not match this atmosphere (nLayer, resolution, '
                                  'r0, L0)')
+            if 'rng' in f and str(f['rng']) != self.rng:
+                raise OopaoError("the saved atmosphere state used rng=%r, this atmosphere rng=%r"
+                                 % (str(f['rng']), self.rng))
             if 'samplingTime' in f and float(f['samplingTime']) != float(self.telescope.samplingTime):
                 raise OopaoError('the saved atmosphere state was taken at another sampling time (%g s, here %g s)'
                                  % (float(f['samplingTime']), self.telescope.samplingTime))
@@ -240,6 +253,8 @@ class Atmosphere:
                     raise OopaoError('the saved atmosphere state does not match layer %d (M)' % (i + 1))
                 if p + 'boiling_seed' in f:
                     lay.boiling_seed = int(f[p + 'boiling_seed'])
+                if p + 'rng_key' in f:
+                    lay.rng_key, lay.rng_draw = int(f[p + 'rng_key']), int(f[p + 'rng_draw'])
                 lay.map.copy_(torch.from_numpy(f[p + 'map']))
                 lay.mm[:2].copy_(torch.from_numpy(f[p + 'mm']))
                 lay.buff = f[p + 'buff'].copy()
@@ -278,6 +293,7 @@ class Atmosphere:
             lay.buff = np.zeros(2)
             lay.shift = (0.0, 0.0)
             lay.boiling_seed = 100000 + seed + i * 100000      # Atmosphere.py:797-799
+            lay.rng_key, lay.rng_draw = (seed + i * 1000) & 0xFFFFFFFF, 0
         self._compute_opd()
         if self.telescope.isPaired:
             self.telescope.src ** self * self.telescope
@@ -286,7 +302,13 @@ class Atmosphere:
     def _add_row(self, lay, step):
         """Atmosphere.add_row (Atmosphere.py:410-421) on the device."""
         sx, sy = int(step[0]), int(step[1])
-        self._stage_noise(lay, lay.randomState.normal(size=lay.outer_idx.numel()))
+        if self.rng == 'device':
+            # draw lay.rng_draw of the layer's extrusion stream (key (seed + 1000 i, 0))
+            _lib.call('oopao_normal_philox', ptr(lay.xi), lay.outer_idx.numel(), lay.rng_key, 0, 0, lay.rng_draw,
+                      stream_ptr(self.device))
+            lay.rng_draw += 1
+        else:
+            self._stage_noise(lay, lay.randomState.normal(size=lay.outer_idx.numel()))
         _lib.call('oopao_atm_add_row', ptr(lay.map), ptr(lay.tmp), lay.M, sx, sy, ptr(lay.inner_idx),
                   lay.inner_idx.numel(), ptr(lay.outer_idx), lay.outer_idx.numel(), ptr(lay.A_dev), ptr(lay.B_dev),
                   ptr(lay.xi), ptr(lay.z), ptr(lay.X), ptr(lay.mm), stream_ptr(self.device))
@@ -357,7 +379,8 @@ class Atmosphere:
                      W=torch.from_numpy(centred_dft_matrix(M)).to(dev),
                      xc=torch.from_numpy(np.arange(-M / 2, M / 2) * pixel).to(dev),
                      ws=torch.empty(nb.value, dtype=torch.uint8, device=dev),
-                     g=torch.empty(2 * M * M + 48, dtype=torch.float64, device=dev))
+                     g=torch.empty(2 * M * M + 48, dtype=torch.float64, device=dev),
+                     table=torch.from_numpy(subharmonic_table(self.r0, self.L0, M, pixel).reshape(-1)).to(dev))
             cache[key] = t
         return t
 
@@ -378,15 +401,22 @@ class Atmosphere:
         # one draw of the FFT-grid coefficients: real parts then imaginary parts (phaseStats.py:192); the
         # subharmonics restart the same seed (a second RandomState, phaseStats.py:225), i.e. reuse the
         # first 54 of these numbers
-        g = RandomState(lay.boiling_seed).normal(size=2 * M * M)
-        nsub = 0
-        staged = g
-        if self.mode == 2:
-            sub, _ = subharmonic_terms(self.r0, self.L0, M, lay.D / lay.resolution, g[:54])
-            nsub = sub.shape[0]
-            staged = np.concatenate((g, sub.reshape(-1)))
-        dst = tb['g'][:staged.size]
-        self._stage(lay, 'boil', staged, dst)
+        nsub = 12 if self.mode == 2 else 0
+        dst = tb['g']
+        if self.rng == 'device':
+            # the screen's draws from the Philox key (boiling_seed, 1); the subharmonics from its first 54
+            _lib.call('oopao_normal_philox', ptr(dst), 2 * M * M, lay.boiling_seed & 0xFFFFFFFF, 1, 0, 0,
+                      stream_ptr(self.device))
+            if nsub:
+                _lib.call('oopao_atm_boil_subharmonics', ptr(dst), 2 * M * M, ptr(tb['table']), nsub,
+                          ctypes.c_void_p(dst.data_ptr() + 8 * 2 * M * M), stream_ptr(self.device))
+        else:
+            g = RandomState(lay.boiling_seed).normal(size=2 * M * M)
+            staged = g
+            if nsub:
+                sub, _ = subharmonic_terms(self.r0, self.L0, M, lay.D / lay.resolution, g[:54])
+                staged = np.concatenate((g, sub.reshape(-1)))
+            self._stage(lay, 'boil', staged, dst[:staged.size])
         _lib.call('oopao_atm_boil', ptr(lay.map), M, ptr(dst), ptr(tb['amp']), ptr(tb['W']),
                   ctypes.c_void_p(dst.data_ptr() + 8 * 2 * M * M) if nsub else None, nsub, ptr(tb['xc']),
                   ctypes.c_double(alpha), ctypes.c_double(beta), ptr(tb['ws']), ctypes.c_size_t(tb['ws'].numel()),

@@ -66,6 +66,9 @@ SIGNATURES = {
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'oopao_atm_minmax': (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     'oopao_atm_boil_bytes': (c_int, [c_int, ctypes.POINTER(c_size_t)]),
+    'oopao_atm_boil_subharmonics': (c_int, [c_void_p, ctypes.c_longlong, c_void_p, c_int, c_void_p, c_void_p]),
+    'oopao_normal_philox': (c_int, [c_void_p, ctypes.c_longlong, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                    ctypes.c_ulonglong, c_void_p]),
     'oopao_atm_boil': (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_double,
                                c_double, c_void_p, c_size_t, c_void_p]),
     'oopao_atm_layer_opd': (c_int, [c_void_p, c_int, c_void_p, c_double, c_double, c_int, c_int, c_double, c_void_p,

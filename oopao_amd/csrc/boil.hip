@@ -113,6 +113,20 @@ __global__ void __launch_bounds__(256) k_boil_mix(double* __restrict__ map, cons
   map[i] = fma(alpha, map[i], beta * phi);
 }
 
+// sub[t] = (g[re_t] a_t, g[im_t] a_t, fx_t, fy_t) from table[t] = (re_t, im_t, a_t, fx_t, fy_t): the subharmonic
+// coefficients from draws already on the device (the device RNG mode)
+__global__ void k_boil_sub(const double* __restrict__ g, const double* __restrict__ table, int nsub,
+                           double* __restrict__ sub) {
+  const int t = threadIdx.x;
+  if (t >= nsub) return;
+  const double* e = table + 5 * t;
+  const double a = e[2];
+  sub[4 * t] = g[(int)e[0]] * a;
+  sub[4 * t + 1] = g[(int)e[1]] * a;
+  sub[4 * t + 2] = e[3];
+  sub[4 * t + 3] = e[4];
+}
+
 static size_t al256b(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace oopao
@@ -163,6 +177,16 @@ int oopao_atm_boil(double* map, int M, const double* g, const double* amp, const
     OOPAO_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_boil_mix, dim3(nblk), dim3(256), 0, st, map, hi, lo, part, nblk, M2, alpha, beta);
+  OOPAO_LAUNCH_CHECK();
+  return OK;
+}
+
+int oopao_atm_boil_subharmonics(const double* g, long long ng, const double* table, int nsub, double* sub,
+                                void* stream) {
+  OOPAO_CHECK_ARG(g && table && sub && nsub > 0 && nsub <= 256 && ng >= 54, "invalid arguments nsub=%d ng=%lld",
+                  nsub, ng);
+  hipLaunchKernelGGL(k_boil_sub, dim3(1), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g, table, nsub,
+                     sub);
   OOPAO_LAUNCH_CHECK();
   return OK;
 }

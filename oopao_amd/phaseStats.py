@@ -110,6 +110,23 @@ def subharmonic_terms(r0, L0, resolution, pixel_size, draws, l0=1e-10):
     return np.asarray(rows, np.float64), coords
 
 
+def subharmonic_table(r0, L0, resolution, pixel_size, l0=1e-10):
+    """The 12 subharmonic terms of ft_sh_phase_screen as rows (re index, im index, a, fx, fy): the term's
+    coefficient is (g[re] + i g[im]) a for the screen's normal draws g, whose first 54 are the
+    subharmonics' (per level 9 real then 9 imaginary parts, row-major 3 x 3; phaseStats.py:236-266)."""
+    unit = np.zeros(54)
+    rows = []
+    for p in range(3):
+        for i in range(2):
+            for j in range(2):
+                re, im = 18 * p + 3 * i + j, 18 * p + 9 + 3 * i + j
+                unit[:] = 0
+                unit[re] = 1.0
+                t = subharmonic_terms(r0, L0, resolution, pixel_size, unit)[0][4 * p + 2 * i + j]
+                rows.append((re, im, t[0], t[2], t[3]))
+    return np.asarray(rows, np.float64)
+
+
 def ift2(G):
     """fftshift(fft2(fftshift(G))) (phaseStats.py:135-150)."""
     return np.fft.fftshift(np.fft.fft2(np.fft.fftshift(G)))

@@ -109,6 +109,41 @@ def ft_sh_phase_screen(r0, L0, resolution, pixel_size, seed, l0=1e-10):
     return (lo.real - lo.real.mean()) + hi
 
 
+def screen_from_draws(r0, L0, M, pixel_size, g, mode=2, l0=1e-10):
+    """The phase screen ft_sh_phase_screen (mode 2) or ft_phase_screen builds from its normal draws g
+    (2 M^2 values: real parts, then imaginary parts of the FFT-grid coefficients; the subharmonics take
+    the first 54), phaseStats.py:153-271; with g = RandomState(seed).normal(size=2 M^2) it is the seeded
+    screen."""
+    g = np.asarray(g, np.float64)
+    del_f = 1. / (M * pixel_size)
+    fx = np.arange(-M / 2., M / 2.) * del_f
+    fx, fy = np.meshgrid(fx, fx)
+    f = np.sqrt(fx ** 2 + fy ** 2)
+    fm = 5.92 / l0 / (2 * np.pi)
+    psd = 0.023 * r0 ** (-5. / 3.) * np.exp(-1 * ((f / fm) ** 2)) / (((f ** 2) + (1. / L0) ** 2) ** (11. / 6))
+    psd[int(M / 2), int(M / 2)] = 0
+    cn = (g[:M * M] + 1j * g[M * M:2 * M * M]).reshape(M, M) * np.sqrt(psd) * del_f
+    hi = np.fft.fftshift(np.fft.fft2(np.fft.fftshift(cn))).real
+    if mode != 2:
+        return hi
+    D = M * pixel_size
+    coords = np.arange(-M / 2, M / 2) * pixel_size
+    x, y = np.meshgrid(coords, coords)
+    lo = np.zeros((M, M), complex)
+    sub = g[:54].reshape(3, 2, 3, 3)
+    for p in range(1, 4):
+        d = 1 / (3 ** p * D)
+        fxs, fys = np.meshgrid(np.arange(-1, 2) * d, np.arange(-1, 2) * d)
+        fs = np.sqrt(fxs ** 2 + fys ** 2)
+        ps = 0.023 * r0 ** (-5. / 3) * np.exp(-1 * (fs / fm) ** 2) / ((fs ** 2 + (1. / L0) ** 2) ** (11. / 6))
+        ps[1, 1] = 0
+        c = (sub[p - 1, 0] + 1j * sub[p - 1, 1]) * np.sqrt(ps) * d
+        for i in range(2):
+            for j in range(2):
+                lo += c[i, j] * np.exp(1j * 2 * np.pi * (fxs[i, j] * x + fys[i, j] * y))
+    return (lo.real - lo.real.mean()) + hi
+
+
 class LayerOracle:
     """One frozen-flow layer: mapShift [(res+2)^2] float64, A, B, RandomState (Atmosphere.py:281-374, 410-516)."""
 
@@ -125,9 +160,10 @@ class LayerOracle:
         self.OPD = self.mapShift[1:-1, 1:-1].copy()
         self.boiling = None
 
-    def set_boiling(self, t_boiling, sampling_time, r0, L0, pixel_size, boiling_seed, mode=2):
-        """Boiling state of the layer (Atmosphere.py:329-333, 387-408)."""
-        self.boiling = dict(t=t_boiling, dt=sampling_time, r0=r0, L0=L0, pixel=pixel_size, mode=mode)
+    def set_boiling(self, t_boiling, sampling_time, r0, L0, pixel_size, boiling_seed, mode=2, draws=None):
+        """Boiling state of the layer (Atmosphere.py:329-333, 387-408). ``draws(seed, count)`` gives a
+        screen's normal draws (default: RandomState(seed), the reference's)."""
+        self.boiling = dict(t=t_boiling, dt=sampling_time, r0=r0, L0=L0, pixel=pixel_size, mode=mode, draws=draws)
         self.boiling_seed = boiling_seed
 
     def apply_boiling(self):
@@ -141,8 +177,12 @@ class LayerOracle:
         beta = float(np.sqrt(max(0.0, 1.0 - alpha ** 2)))
         self.boiling_seed += 1
         M = self.mapShift.shape[0]
-        gen = ft_sh_phase_screen if b['mode'] == 2 else ft_phase_screen
-        phi = gen(b['r0'], b['L0'], M, b['pixel'], seed=self.boiling_seed)
+        if b['draws'] is None:
+            gen = ft_sh_phase_screen if b['mode'] == 2 else ft_phase_screen
+            phi = gen(b['r0'], b['L0'], M, b['pixel'], seed=self.boiling_seed)
+        else:
+            g = b['draws'](self.boiling_seed, 2 * M * M)
+            phi = screen_from_draws(b['r0'], b['L0'], M, b['pixel'], g, b['mode'])
         self.mapShift = alpha * self.mapShift + beta * phi
         return True
 
