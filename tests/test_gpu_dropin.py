@@ -249,3 +249,28 @@ def test_workspace_failure_halves_the_batch(monkeypatch):
     from oopao_amd.tools import WorkspaceAllocationError
     with pytest.raises(WorkspaceAllocationError):
         _config1_objects()
+
+
+def test_config1_exact_input():
+    # config 1 as specified: the seed-1 von Karman screen of a one-layer atmosphere (r0 0.15 m, L0 25 m)
+    # plus the OPD of the 21 x 21 DM driven by RandomState(2).normal(0, 20e-9, 357), through
+    # ngs**atm*tel*dm*wfs. The screen is ft_sh_phase_screen on the layer's 84^2 support (8.4 m, 0.1 m
+    # pixels, seed 1) cropped to the pupil footprint [2:82, 2:82] and scaled by 500e-9 / 2 pi
+    # (Atmosphere.py:779-810, 589-622; oracle/atmosphere restates the generator)
+    from oopao_amd import Atmosphere
+    from oracle import atmosphere as oat
+    from oracle.dm import DMOracle
+    tel, ngs, dm, wfs = _config1_objects()
+    atm = Atmosphere(tel, 0.15, 25, windSpeed=[10], fractionalR0=[1], windDirection=[0], altitude=[0])
+    atm.initializeAtmosphere(tel)
+    atm.generateNewPhaseScreen(1)
+    screen = oat.ft_sh_phase_screen(0.15, 25, 84, 8.4 / 84, seed=1)[2:82, 2:82] * 500e-9 / (2 * np.pi)
+    assert rel_l2(atm.OPD, screen) < 1e-6
+    coefs = np.random.RandomState(2).normal(0, 20e-9, 357)
+    dm.coefs = coefs
+    ngs ** atm * tel * dm * wfs
+    od = DMOracle(80, 8, 20, 0.35)
+    s_ref = config1_system().measure_opd(screen + od.opd(od.modes(), coefs))[1]
+    err = rel_l2(wfs.signal, s_ref)
+    print('config-1 exact input: slopes rel-L2 %.2e vs float64 oracle (nSignal %d)' % (err, wfs.nSignal))
+    assert err <= 1e-5
