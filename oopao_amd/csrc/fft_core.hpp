@@ -162,6 +162,20 @@ __host__ __device__ constexpr int plan_stages(int rem, int E) {
   return rem == 1 ? 0 : 1 + plan_stages(rem / pick_radix(rem, E), E);
 }
 
+// Largest twiddle index read by the stages from span NS on (REM = N / NS still to transform): a stage
+// of radix R and span NS > 1 reads tw[r k N / (NS R)] for r < R, k < NS.
+__host__ __device__ constexpr int plan_tw_max(int N, int E, int NS, int REM) {
+  int mx = 0;
+  while (REM > 1) {
+    const int R = pick_radix(REM, E);
+    if (R == 0) return N - 1;
+    if (NS > 1 && (R - 1) * (NS - 1) * (N / (NS * R)) > mx) mx = (R - 1) * (NS - 1) * (N / (NS * R));
+    NS *= R;
+    REM /= R;
+  }
+  return mx;
+}
+
 // Exchange buffers are unpadded: on gfx950 every LDS address then folds into the instruction's
 // immediate offset, which saved registers (occupancy 2 -> 4 waves/SIMD for the 1280-point columns
 // pass) and ran 10-30 % faster than one float2 of padding every 32.

@@ -147,8 +147,24 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
 //  * narrower transforms: one wavefront per (b, chunk, g), its 64/TPT lane
 //    groups taking the columns.
 // Only the detector columns in `bd` are computed (one task per such column).
+// With 64-lane transforms at the sizes cols_tw_lds() names, the workgroup first
+// copies the twiddle entries the plans read into LDS: every stage's twiddles
+// are then LDS reads (lgkm queue, ~100-cycle latency) instead of global loads
+// queued behind the column loads (the table loads cost ~9 % of the kernel at
+// N = 1280). The column sums reuse the exchange buffers' LDS, so the workgroup
+// needs 48 KB and three still fit per CU.
 // Output: part[b][chunk][g][r] (detector column-major partial sums).
 // ---------------------------------------------------------------------------
+// Length of the twiddle-table prefix the columns pass reads: the pruned forward (from span 4) and the
+// full inverse transform.
+template <int N, int TPT>
+__host__ __device__ constexpr int cols_tw_len() {
+  return (plan_tw_max(N, N / TPT, 4, N / 4) > plan_tw_max(N, N / TPT, 1, N) ? plan_tw_max(N, N / TPT, 4, N / 4)
+                                                                               : plan_tw_max(N, N / TPT, 1, N)) +
+         1;
+}
+__host__ __device__ constexpr bool cols_tw_lds(int N) { return N == 1024 || N == 1280; }
+
 template <int N, int TPT>
 __global__ void __launch_bounds__(64 * kWaves, fast_cols_min_waves(N))
 k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
@@ -158,8 +174,12 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   constexpr int G = 64 / TPT;
   constexpr int NG = N / 4;
   constexpr bool kWaveCols = (TPT == 64);
+  constexpr bool kTwLds = kWaveCols && cols_tw_lds(N);
+  constexpr int kTw = kTwLds ? cols_tw_len<N, TPT>() : 1;
+  static_assert(!kTwLds || kTw <= N, "twiddle prefix within the table");
+  static_assert(kWaves * NG <= 2 * kWaves * G * lds_len(N), "column sums fit in the exchange buffers");
   __shared__ float2 lds[kWaves * G][lds_len(N)];
-  __shared__ float red[kWaveCols ? kWaves * NG : 1];
+  __shared__ float2 twl[kTw];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int h = lane / TPT;
@@ -174,6 +194,11 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   const int i0 = (int)(((long)chunk * T) / nchunk);
   const int i1 = (int)(((long)(chunk + 1) * T) / nchunk);
   float2* xl = lds[wave * G + h];
+  if constexpr (kTwLds) {
+    for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
+    __syncthreads();
+  }
+  const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;
 
   float acc[E];
 #pragma unroll
@@ -188,10 +213,10 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
       for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + c];
       // XOR-swizzled LDS exchange (conflict-free small-span stages), as in the
       // rows pass.
-      fft_fwd_pruned<N, TPT, 4, true>(v, xl, tw, t);
+      fft_fwd_pruned<N, TPT, 4, true>(v, xl, twp, t);
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = cmul(v[m], launder(my)[t + TPT * m]);
-      fft_full<N, TPT, +1, true>(v, xl, tw, t);
+      fft_full<N, TPT, +1, true>(v, xl, twp, t);
 #pragma unroll
       for (int m = 0; m < E; ++m) acc[m] += cabs2(v[m]);
     }
@@ -199,6 +224,9 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   // rows t..t+3 (a lane quad) form one detector row
   float* dst = part + (((size_t)b * nchunk + chunk) * NG + g) * (size_t)NG;
   if constexpr (kWaveCols) {
+    // the column sums go to the exchange buffers once every wavefront's last exchange is done
+    float* red = reinterpret_cast<float*>(&lds[0][0]);
+    __syncthreads();
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       float s = acc[m];
