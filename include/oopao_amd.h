@@ -47,6 +47,15 @@ int oopao_prepare_tables(int N);
  * 4 x 4 detector binning, N in {64,128,256,320,384,512,640,768,1024,1280,1536,1920,2048}. */
 int oopao_pwfs_supported(int n, int N, int bin);
 
+/* FFT plans compiled for padded length N, as lanes per transform (each lane
+ * count fixes the radix plan, the values held per lane and the occupancy):
+ * rows_lanes[0] / cols_lanes[0] are the default plans of the rows and columns
+ * passes, rows_lanes[1] / cols_lanes[1] a second plan (0 when none is compiled;
+ * rows: N = 320; columns: N = 320, 640, 1024, 1280). Both arrays hold 2 ints.
+ * No counterpart in the reference: the plan is a tuning choice of this
+ * library's transforms, picked per geometry by the calibration autotuner. */
+int oopao_pwfs_plans(int N, int* rows_lanes, int* cols_lanes);
+
 /* Workspace needed by oopao_pwfs_frames for B screens of T modulation points
  * reduced in `nchunk` point chunks. */
 int oopao_pwfs_workspace_bytes(int n, int N, int T, int B, int nchunk, size_t* bytes);
@@ -74,11 +83,15 @@ int oopao_pwfs_workspace_bytes(int n, int N, int T, int B, int nchunk, size_t* b
  * [b0, b1) are computed (bit for bit as in the whole frame), the others are
  * written as zero. The slopesMaps signal reads only the quadrant columns
  * [c-e-w, c-e) and [c+e, c+e+w) (Pyramid.py:838-853, 946-961), so a
- * slopes-only measurement passes those and does half the work at config 2. */
+ * slopes-only measurement passes those and does half the work at config 2.
+ * lanes_rows / lanes_cols: lanes per transform of the rows / columns pass, one
+ * of the plans oopao_pwfs_plans lists, or 0 for the default (ERR_UNSUPPORTED
+ * for a plan not compiled at this N); every plan computes the same frame up to
+ * float rounding. */
 int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
                       const void* ty, const void* mx, const void* my, int n, int N, int T, int B, int bin,
-                      int nchunk, int pts_per_task, const int* bands, void* workspace, size_t ws_bytes,
-                      float* frames, void* stream);
+                      int nchunk, int pts_per_task, int lanes_rows, int lanes_cols, const int* bands,
+                      void* workspace, size_t ws_bytes, float* frames, void* stream);
 
 /* The two passes of oopao_pwfs_frames as separate calls (same arguments), for
  * callers that own the intermediate buffers or time the passes:
@@ -91,14 +104,15 @@ int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const voi
  *               chunks into frames[B][cam][cam].
  * inter_bytes / part_bytes: sizes of the caller's buffers, checked against
  * oopao_pwfs_intermediate_bytes (ERR_WORKSPACE when smaller); bands as in
- * oopao_pwfs_frames (the same bands for both passes). */
+ * oopao_pwfs_frames (the same bands for both passes); lanes as lanes_rows /
+ * lanes_cols of oopao_pwfs_frames. */
 int oopao_pwfs_intermediate_bytes(int n, int N, int T, int B, int nchunk, size_t* inter_bytes,
                                   size_t* part_bytes);
 int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
-                    const void* ty, const void* mx, int n, int N, int T, int B, int pts_per_task, const int* bands,
-                    void* inter, size_t inter_bytes, void* stream);
+                    const void* ty, const void* mx, int n, int N, int T, int B, int pts_per_task, int lanes,
+                    const int* bands, void* inter, size_t inter_bytes, void* stream);
 int oopao_pwfs_cols(const void* inter, size_t inter_bytes, const void* my, int n, int N, int T, int B, int nchunk,
-                    const int* bands, float* part, size_t part_bytes, float* frames, void* stream);
+                    int lanes, const int* bands, float* part, size_t part_bytes, float* frames, void* stream);
 
 /* Unbinned modulated frames on the separable path (the reference's raw_data,
  * Pyramid.py:751-757): raw[B][N][N] float32 = sum over points of

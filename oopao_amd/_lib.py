@@ -24,10 +24,11 @@ SIGNATURES = {
     'oopao_last_error': (ctypes.c_char_p, []),
     'oopao_prepare_tables': (c_int, [c_int]),
     'oopao_pwfs_supported': (c_int, [c_int, c_int, c_int]),
+    'oopao_pwfs_plans': (c_int, [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
     'oopao_pwfs_workspace_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_frames': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
-                                  c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
-                                  c_size_t, c_void_p, c_void_p]),
+                                  c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                  c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
     'oopao_pwfs_slopes_workspace_bytes': (c_int, [c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_slopes': (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_float,
                                   c_void_p, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p, c_void_p]),
@@ -36,8 +37,8 @@ SIGNATURES = {
     'oopao_pwfs_intermediate_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t),
                                               ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_rows': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,
-                                c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
-    'oopao_pwfs_cols': (c_int, [c_void_p, c_size_t, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p,
+                                c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_size_t, c_void_p]),
+    'oopao_pwfs_cols': (c_int, [c_void_p, c_size_t, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p,
                                 c_void_p, c_size_t, c_void_p, c_void_p]),
     'oopao_pwfs_raw_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_raw': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -12,10 +12,11 @@ the kernels:
         times the pupil, PWFS row pass, column pass, slopesMaps of the 2 B frames,
         D[batch] = 0.5 (s_push - s_pull) / stroke.
 
-``PWFSAutotuner`` picks, per PWFS geometry, the column-pass chunk (modulation points per
-chunk), the row-pass task size (tip tables per task) and the batch size by timing candidate
-settings on the device; choices are cached per geometry for the process and can be saved to /
-loaded from a JSON file.
+``PWFSAutotuner`` picks, per PWFS geometry, the FFT plan of each pass (lanes per transform, which
+fixes the radix plan, the values per lane and the occupancy; ``PWFSEngine.plans()`` lists the compiled
+ones), the column-pass chunk (modulation points per chunk), the row-pass task size (tip tables per
+task) and the batch size by timing candidate settings on the device; choices are cached per geometry
+for the process and can be saved to / loaded from a JSON file.
 """
 import json
 import math
@@ -59,6 +60,17 @@ class PWFSAutotuner:
         with open(path) as f:
             self.cache.update(json.load(f))
 
+    def _time_pass(self, wfs, opd, amp, scale, B, which, reps=3):
+        """ms of one pass ('rows' or 'cols') for B screens with the engine's current settings."""
+        e = wfs.engine
+        buf = e.buffers(B)
+        frames = torch.empty((B, e.cam, e.cam), dtype=torch.float32, device=wfs.device)
+        bands = wfs.slope_bands()
+        e.rows(opd[:B], scale, amp, buf, bands)
+        if which == 'rows':
+            return _event_ms(lambda: e.rows(opd[:B], scale, amp, buf, bands), reps)
+        return _event_ms(lambda: e.cols(buf, frames, bands), reps)
+
     def _time(self, wfs, opd, amp, scale, B, reps=3):
         e, se = wfs.engine, wfs.slopes_engine
         buf = e.buffers(B)
@@ -83,6 +95,7 @@ class PWFSAutotuner:
         if k in self.cache:
             best = self.cache[k]
             e.nchunk_user, e.pts_per_task_user = best['nchunk'], best['pts_per_task']
+            e.set_plans(best.get('lanes_rows', 0), best.get('lanes_cols', 0))
             return best
         free = torch.cuda.mem_get_info(wfs.device)[0]
         per = max(1, e.bytes_per_screen())
@@ -92,6 +105,16 @@ class PWFSAutotuner:
         batches = [b for b in self.batch_candidates if b <= bmax] or [1]
         B0 = min(8, batches[-1])
         opd = opd_sample.expand(max(batches), -1, -1).contiguous()
+        # FFT plan of each pass at a fixed batch and the default tiles (each plan changes only its pass)
+        rows_plans, cols_plans = e.plans()
+        e.nchunk_user = e.pts_per_task_user = None
+        plan_ms = {}
+        for which, plans in (('rows', rows_plans), ('cols', cols_plans)):
+            for lanes in plans:
+                e.set_plans(lanes if which == 'rows' else e.lanes_rows, lanes if which == 'cols' else e.lanes_cols)
+                plan_ms[(which, lanes)] = self._time_pass(wfs, opd, amp, scale, B0, which)
+            pick = min(plans, key=lambda v: plan_ms[(which, v)])
+            e.set_plans(pick if which == 'rows' else e.lanes_rows, pick if which == 'cols' else e.lanes_cols)
         # tile sizes at a fixed batch: modulation points per column-pass chunk, tip tables per row task
         chunks = sorted({max(1, math.ceil(e.T / c)) for c in (4, 6, 8, 12)})
         tasks = sorted({max(1, min(e.P, p)) for p in (4, 6, 8)})
@@ -106,7 +129,9 @@ class PWFSAutotuner:
         bt = [(self._time(wfs, opd, amp, scale, b), b) for b in batches]
         tb, b = min(bt)
         best = {'nchunk': nc, 'pts_per_task': pt, 'batch': b, 'ms_per_frame': round(tb, 5),
-                'trials': len(trials) + len(bt)}
+                'lanes_rows': e.lanes_rows, 'lanes_cols': e.lanes_cols,
+                'plan_ms': {'%s_%d' % key: round(v, 5) for key, v in plan_ms.items()},
+                'trials': len(trials) + len(bt) + len(plan_ms)}
         self.cache[k] = best
         return best
 

@@ -513,11 +513,10 @@ static size_t ws_part_bytes(int N, int B, int nchunk) {
   return align256((size_t)B * nchunk * (size_t)(N / 4) * (N / 4) * sizeof(float));
 }
 
-template <int N, bool FWD_ONLY = false>
-static int launch_rows(const float* opd, float scale, const float* amp, const float2* tx, int P, const int* pstart,
-                       const float2* ty, const float2* mx, int T, int B, int pts_per_task, Bands bd, float2* inter,
-                       hipStream_t st) {
-  constexpr int TPT = fast_tpt_rows(N);
+template <int N, int TPT, bool FWD_ONLY>
+static int launch_rows_plan(const float* opd, float scale, const float* amp, const float2* tx, int P,
+                            const int* pstart, const float2* ty, const float2* mx, int T, int B, int pts_per_task,
+                            Bands bd, float2* inter, hipStream_t st) {
   constexpr int n = N / 4;
   constexpr int G = 64 / TPT;
   const float2* tw = nullptr;
@@ -545,10 +544,27 @@ static int launch_rows(const float* opd, float scale, const float* amp, const fl
   return OK;
 }
 
-template <int N>
-static int launch_cols(const float2* inter, const float2* my, int T, int B, int nchunk, Bands bd, float* part,
-                       float* frames, hipStream_t st) {
-  constexpr int TPT = fast_tpt(N);
+// lanes = lanes per transform of a compiled plan (0 = the default one)
+template <int N, bool FWD_ONLY = false>
+static int launch_rows(const float* opd, float scale, const float* amp, const float2* tx, int P, const int* pstart,
+                       const float2* ty, const float2* mx, int T, int B, int pts_per_task, Bands bd, float2* inter,
+                       int lanes, hipStream_t st) {
+  constexpr int D = fast_tpt_rows(N);
+  if (lanes == 0 || lanes == D)
+    return launch_rows_plan<N, D, FWD_ONLY>(opd, scale, amp, tx, P, pstart, ty, mx, T, B, pts_per_task, bd, inter,
+                                            st);
+  if constexpr (!FWD_ONLY && alt_tpt_rows(N) > 0) {
+    if (lanes == alt_tpt_rows(N))
+      return launch_rows_plan<N, alt_tpt_rows(N), false>(opd, scale, amp, tx, P, pstart, ty, mx, T, B, pts_per_task,
+                                                         bd, inter, st);
+  }
+  set_error("no %d-lane rows plan compiled for N=%d", lanes, N);
+  return ERR_UNSUPPORTED;
+}
+
+template <int N, int TPT>
+static int launch_cols_plan(const float2* inter, const float2* my, int T, int B, int nchunk, Bands bd, float* part,
+                            float* frames, hipStream_t st) {
   const float2* tw = nullptr;
   int rc = twiddles(N, &tw);
   if (rc) return rc;
@@ -567,6 +583,19 @@ static int launch_cols(const float2* inter, const float2* my, int T, int B, int 
     OOPAO_LAUNCH_CHECK();
   }
   return OK;
+}
+
+template <int N>
+static int launch_cols(const float2* inter, const float2* my, int T, int B, int nchunk, Bands bd, float* part,
+                       float* frames, int lanes, hipStream_t st) {
+  constexpr int D = fast_tpt(N);
+  if (lanes == 0 || lanes == D) return launch_cols_plan<N, D>(inter, my, T, B, nchunk, bd, part, frames, st);
+  if constexpr (alt_tpt_cols(N) > 0) {
+    if (lanes == alt_tpt_cols(N))
+      return launch_cols_plan<N, alt_tpt_cols(N)>(inter, my, T, B, nchunk, bd, part, frames, st);
+  }
+  set_error("no %d-lane columns plan compiled for N=%d", lanes, N);
+  return ERR_UNSUPPORTED;
 }
 
 template <int N>
@@ -698,6 +727,26 @@ int oopao_pwfs_workspace_bytes(int n, int N, int T, int B, int nchunk, size_t* b
 
 int oopao_pwfs_supported(int n, int N, int bin) { return fast_supported(n, N, bin) ? 1 : 0; }
 
+int oopao_pwfs_plans(int N, int* rows_lanes, int* cols_lanes) {
+  OOPAO_CHECK_ARG(rows_lanes && cols_lanes, "null pointer argument");
+  rows_lanes[0] = rows_lanes[1] = cols_lanes[0] = cols_lanes[1] = 0;
+  switch (N) {
+#define OOPAO_CASE(NN)                                                                                \
+  case NN:                                                                                            \
+    rows_lanes[0] = fast_tpt_rows(NN);                                                                \
+    rows_lanes[1] = alt_tpt_rows(NN);                                                                 \
+    cols_lanes[0] = fast_tpt(NN);                                                                     \
+    cols_lanes[1] = alt_tpt_cols(NN);                                                                 \
+    return OK;
+    OOPAO_FAST_SIZES(OOPAO_CASE)
+#undef OOPAO_CASE
+    default:
+      break;
+  }
+  set_error("unsupported padded length N=%d", N);
+  return ERR_UNSUPPORTED;
+}
+
 static int check_geometry(int n, int N, int bin) {
   if (!fast_supported(n, N, bin)) {
     set_error("unsupported PWFS geometry n=%d N=%d bin=%d (need N = 4 n, 4x4 binning, N in the compiled set)",
@@ -718,8 +767,8 @@ int oopao_pwfs_intermediate_bytes(int n, int N, int T, int B, int nchunk, size_t
 }
 
 int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
-                    const void* ty, const void* mx, int n, int N, int T, int B, int pts_per_task, const int* bands,
-                    void* inter, size_t inter_bytes, void* stream) {
+                    const void* ty, const void* mx, int n, int N, int T, int B, int pts_per_task, int lanes,
+                    const int* bands, void* inter, size_t inter_bytes, void* stream) {
   OOPAO_CHECK_ARG(opd && amp && tx && pstart && ty && mx && inter, "null pointer argument");
   OOPAO_CHECK_ARG(P > 0 && P <= T && B > 0 && pts_per_task > 0, "invalid P=%d T=%d B=%d pts_per_task=%d", P, T, B,
                   pts_per_task);
@@ -740,7 +789,7 @@ int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void*
   switch (N) {
 #define OOPAO_CASE(NN) \
   case NN:             \
-    return launch_rows<NN>(opd, scale, amp, txp, P, pstart, typ, mxp, T, B, pts_per_task, bd, ip, st);
+    return launch_rows<NN>(opd, scale, amp, txp, P, pstart, typ, mxp, T, B, pts_per_task, bd, ip, lanes, st);
     OOPAO_FAST_SIZES(OOPAO_CASE)
 #undef OOPAO_CASE
     default:
@@ -751,7 +800,7 @@ int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void*
 }
 
 int oopao_pwfs_cols(const void* inter, size_t inter_bytes, const void* my, int n, int N, int T, int B, int nchunk,
-                    const int* bands, float* part, size_t part_bytes, float* frames, void* stream) {
+                    int lanes, const int* bands, float* part, size_t part_bytes, float* frames, void* stream) {
   OOPAO_CHECK_ARG(inter && my && part && frames, "null pointer argument");
   OOPAO_CHECK_ARG(T > 0 && B > 0 && nchunk > 0 && nchunk <= T, "invalid T=%d B=%d nchunk=%d", T, B, nchunk);
   int rc = check_geometry(n, N, 4);
@@ -770,7 +819,7 @@ int oopao_pwfs_cols(const void* inter, size_t inter_bytes, const void* my, int n
   switch (N) {
 #define OOPAO_CASE(NN) \
   case NN:             \
-    return launch_cols<NN>(ip, myp, T, B, nchunk, bd, part, frames, st);
+    return launch_cols<NN>(ip, myp, T, B, nchunk, bd, part, frames, lanes, st);
     OOPAO_FAST_SIZES(OOPAO_CASE)
 #undef OOPAO_CASE
     default:
@@ -782,8 +831,8 @@ int oopao_pwfs_cols(const void* inter, size_t inter_bytes, const void* my, int n
 
 int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
                       const void* ty, const void* mx, const void* my, int n, int N, int T, int B, int bin, int nchunk,
-                      int pts_per_task, const int* bands, void* workspace, size_t ws_bytes, float* frames,
-                      void* stream) {
+                      int pts_per_task, int lanes_rows, int lanes_cols, const int* bands, void* workspace,
+                      size_t ws_bytes, float* frames, void* stream) {
   OOPAO_CHECK_ARG(opd && amp && tx && pstart && ty && mx && my && workspace && frames, "null pointer argument");
   OOPAO_CHECK_ARG(P > 0 && P <= T && B > 0 && nchunk > 0 && nchunk <= T && pts_per_task > 0,
                   "invalid P=%d T=%d B=%d nchunk=%d pts_per_task=%d", P, T, B, nchunk, pts_per_task);
@@ -796,9 +845,10 @@ int oopao_pwfs_frames(const float* opd, float scale, const float* amp, const voi
   }
   char* w = reinterpret_cast<char*>(workspace);
   float* part = reinterpret_cast<float*>(w + ib);
-  rc = oopao_pwfs_rows(opd, scale, amp, tx, P, pstart, ty, mx, n, N, T, B, pts_per_task, bands, w, ib, stream);
+  rc = oopao_pwfs_rows(opd, scale, amp, tx, P, pstart, ty, mx, n, N, T, B, pts_per_task, lanes_rows, bands, w, ib,
+                       stream);
   if (rc) return rc;
-  return oopao_pwfs_cols(w, ib, my, n, N, T, B, nchunk, bands, part, pb, frames, stream);
+  return oopao_pwfs_cols(w, ib, my, n, N, T, B, nchunk, lanes_cols, bands, part, pb, frames, stream);
 }
 
 int oopao_pwfs_focal_bytes(int n, int N, int T, int B, int nchunk, int crop, size_t* bytes) {
@@ -833,7 +883,7 @@ int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void
 #define OOPAO_CASE(NN)                                                                                     \
   case NN:                                                                                                 \
     rc = launch_rows<NN, true>(opd, scale, amp, txp, P, pstart, typ, txp, T, B, pts_per_task,              \
-                               Bands{0, NN / 4, NN / 4, NN / 4}, inter, st);                              \
+                               Bands{0, NN / 4, NN / 4, NN / 4}, inter, 0, st);                           \
     if (rc) return rc;                                                                                     \
     return launch_focal<NN>(inter, T, B, nchunk, crop, part, out, st);
     OOPAO_FAST_SIZES(OOPAO_CASE)
@@ -867,7 +917,7 @@ int oopao_pwfs_raw(const float* opd, float scale, const float* amp, const void* 
     return ERR_WORKSPACE;
   }
   char* w = reinterpret_cast<char*>(workspace);
-  rc = oopao_pwfs_rows(opd, scale, amp, tx, P, pstart, ty, mx, n, N, T, B, pts_per_task, nullptr, w, ib, stream);
+  rc = oopao_pwfs_rows(opd, scale, amp, tx, P, pstart, ty, mx, n, N, T, B, pts_per_task, 0, nullptr, w, ib, stream);
   if (rc) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const float2* ip = reinterpret_cast<const float2*>(w);

@@ -21,8 +21,8 @@
 
 extern "C" int oopao_pwfs_rows(const float* opd, float scale, const float* amp, const void* tx, int P,
                                const int* pstart, const void* ty, const void* mx, int n, int N, int T, int B,
-                               int pts_per_task, const int* bands, void* inter, size_t inter_bytes,
-                               void* stream);
+                               int pts_per_task, int lanes, const int* bands, void* inter,
+                               size_t inter_bytes, void* stream);
 
 namespace oopao {
 
@@ -218,7 +218,7 @@ int oopao_pwfs_adjoint(const float* opd, float scale, const float* amp, const vo
   float* gpart = reinterpret_cast<float*>(w + adj_inter_bytes(n, N, T, B) +
                                           al256a((size_t)B * T * (size_t)N * n * sizeof(float2)));
   // recompute the per-point row transforms (and the pupil field) exactly as the forward
-  rc = oopao_pwfs_rows(opd, scale, amp, tx, P, pstart, ty, mx, n, N, T, B, pts_per_task, nullptr, inter,
+  rc = oopao_pwfs_rows(opd, scale, amp, tx, P, pstart, ty, mx, n, N, T, B, pts_per_task, 0, nullptr, inter,
                        adj_inter_bytes(n, N, T, B), stream);
   if (rc) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);

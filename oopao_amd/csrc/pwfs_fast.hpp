@@ -25,6 +25,19 @@ __host__ __device__ constexpr int fast_tpt_rows(int N) { return N == 640 ? 32 : 
 // spills when asked for two waves, which runs it 25 % faster.
 __host__ __device__ constexpr int fast_cols_min_waves(int N) { return N == 2048 ? 2 : 1; }
 
+// Second FFT plan compiled for each pass at the sensor sizes a batched calibration mixes (n = 80, 160,
+// 256, 320): a different lane count per transform, hence a different radix plan, registers per lane and
+// occupancy. N = 320: 32 lanes (10 values, radices 5 / 2 instead of 4 / 5 on 20; rows pass 167 VGPRs,
+// three waves per SIMD instead of two); N = 640 columns: 32 lanes (20 values instead of 40; 146 VGPRs,
+// three waves instead of two); N = 1024 / 1280 columns: 32 lanes (32 / 40 values, radix 8 first; two
+// waves instead of three). The 32-lane rows passes at N = 1024 / 1280 need 256 VGPRs plus 116-206
+// AGPRs of spill at one wave per SIMD and are not compiled. 0 where only the default is compiled.
+// The calibration autotuner times both plans per geometry.
+__host__ __device__ constexpr int alt_tpt_rows(int N) { return N == 320 ? 32 : 0; }
+__host__ __device__ constexpr int alt_tpt_cols(int N) {
+  return N == 320 ? 32 : N == 640 ? 32 : N == 1024 ? 32 : N == 1280 ? 32 : 0;
+}
+
 #define OOPAO_FAST_SIZES(X) X(64) X(128) X(256) X(320) X(384) X(512) X(640) X(768) X(1024) X(1280) X(1536) X(1920) X(2048)
 
 constexpr int kWaves = 4;  // wavefronts per workgroup

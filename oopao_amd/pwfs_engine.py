@@ -148,6 +148,9 @@ class PWFSEngine:
         _lib.prepare_tables(N, self.device)
         self.nchunk_user = nchunk
         self.pts_per_task_user = pts_per_task
+        # lanes per transform of the rows / columns pass (0 = the default plan; see plans())
+        self.lanes_rows = 0
+        self.lanes_cols = 0
         self.workspace = Workspace(self.device) if workspace is None else workspace
         self.T = 0
         self.maskT = None
@@ -238,6 +241,22 @@ class PWFSEngine:
         if self.bin == 0:
             raise OopaoError('detector size %d does not divide N=%d' % (self.cam, self.N))
 
+    def plans(self):
+        """FFT plans compiled for this N on the separable path, as lanes per transform:
+        ([rows plans], [columns plans]), the default plan first."""
+        r, c = (ctypes.c_int * 2)(), (ctypes.c_int * 2)()
+        _lib.call('oopao_pwfs_plans', self.N, r, c)
+        return [v for v in r if v], [v for v in c if v]
+
+    def set_plans(self, lanes_rows=0, lanes_cols=0):
+        """Select the rows / columns plans (lanes per transform, 0 = default); raises for a plan that is
+        not compiled at this N."""
+        rows, cols = self.plans()
+        if (lanes_rows and lanes_rows not in rows) or (lanes_cols and lanes_cols not in cols):
+            raise OopaoError('no compiled plan with %d / %d lanes at N=%d (rows %s, columns %s)'
+                             % (lanes_rows, lanes_cols, self.N, rows, cols))
+        self.lanes_rows, self.lanes_cols = int(lanes_rows), int(lanes_cols)
+
     # -------------------------------------------------------------- execution
     def _knobs(self, B):
         """Chunk sizes: 6 points per columns-pass chunk and 6 tip tables per rows-pass task (measured
@@ -304,8 +323,8 @@ class PWFSEngine:
         ws = self._scratch(nb.value)
         _lib.call('oopao_pwfs_frames', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
                   _ptr(self.pstart), _ptr(ty), _ptr(self.mx), _ptr(self.my), self.n, self.N, self.T, B, self.bin,
-                  nchunk, ppt, column_bands(bands, self.cam), _ptr(ws), ctypes.c_size_t(nb.value), _ptr(out),
-                  _stream_ptr(self.device))
+                  nchunk, ppt, self.lanes_rows, self.lanes_cols, column_bands(bands, self.cam), _ptr(ws),
+                  ctypes.c_size_t(nb.value), _ptr(out), _stream_ptr(self.device))
         return out
 
     def _full_maskT(self):
@@ -432,14 +451,14 @@ class PWFSEngine:
         self._check_buffers(buf)
         _lib.call('oopao_pwfs_rows', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
                   _ptr(self.pstart), _ptr(self.ty_s), _ptr(self.mx), self.n, self.N, self.T, buf['B'], buf['ppt'],
-                  column_bands(bands, self.cam), _ptr(buf['inter']), ctypes.c_size_t(buf['inter'].numel()),
-                  _stream_ptr(self.device))
+                  self.lanes_rows, column_bands(bands, self.cam), _ptr(buf['inter']),
+                  ctypes.c_size_t(buf['inter'].numel()), _stream_ptr(self.device))
 
     def cols(self, buf, out, bands=None):
         self._check_buffers(buf)
         _lib.call('oopao_pwfs_cols', _ptr(buf['inter']), ctypes.c_size_t(buf['inter'].numel()), _ptr(self.my),
-                  self.n, self.N, self.T, buf['B'], buf['nchunk'], column_bands(bands, self.cam), _ptr(buf['part']),
-                  ctypes.c_size_t(4 * buf['part'].numel()), _ptr(out), _stream_ptr(self.device))
+                  self.n, self.N, self.T, buf['B'], buf['nchunk'], self.lanes_cols, column_bands(bands, self.cam),
+                  _ptr(buf['part']), ctypes.c_size_t(4 * buf['part'].numel()), _ptr(out), _stream_ptr(self.device))
 
 
 class SlopesEngine:

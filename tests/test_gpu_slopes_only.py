@@ -116,9 +116,9 @@ def test_band_and_buffer_argument_checks():
     L = _lib.lib()
     dummy = ctypes.c_void_p(256)
     bad = (ctypes.c_int * 4)(30, 10, 50, 70)
-    rc = L.oopao_pwfs_cols(dummy, 1 << 30, dummy, 80, 320, 48, 1, 8, bad, dummy, 1 << 30, dummy, None)
+    rc = L.oopao_pwfs_cols(dummy, 1 << 30, dummy, 80, 320, 48, 1, 8, 0, bad, dummy, 1 << 30, dummy, None)
     assert rc == 1 and b'bands' in L.oopao_last_error()
-    rc = L.oopao_pwfs_cols(dummy, 100, dummy, 80, 320, 48, 1, 8, None, dummy, 1 << 30, dummy, None)
+    rc = L.oopao_pwfs_cols(dummy, 100, dummy, 80, 320, 48, 1, 8, 0, None, dummy, 1 << 30, dummy, None)
     assert rc == 4
     # a buffers dict sized for another number of modulation points is refused
     buf = eng.buffers(1)
