@@ -177,6 +177,18 @@ int oopao_pwfs_generic_focal_bytes(int n, int N, int T, int B, int nchunk, int c
 int oopao_pwfs_generic_focal(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
                              int n, int N, int T, int B, int nchunk, int crop, void* workspace, size_t ws_bytes,
                              float* out, void* stream);
+/* Adjoint of oopao_pwfs_generic_frames (no counterpart in the reference, which
+ * has no gradients): grad [B][n][n] float32 = dL/dopd for the cotangent
+ * gframe [B][N/bin][N/bin] = dL/dframes, same field inputs and mask, every
+ * modulation point recomputed (nothing is kept from the forward). For the
+ * forward E_i = ifft2(fft2(U_i) mask) of Pyramid.pyramid_transform
+ * (Pyramid.py:612-646) it computes sum_i 2 scale Im(W_i conj(U_i)) on the pupil,
+ * W_i = ifft2(conj(mask) fft2(g E_i)), g = gframe spread over each bin x bin
+ * block. Workspace of oopao_pwfs_generic_adjoint_bytes bytes. */
+int oopao_pwfs_generic_adjoint_bytes(int n, int N, int T, int B, int nchunk, size_t* bytes);
+int oopao_pwfs_generic_adjoint(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
+                               const void* maskT, const float* gframe, int n, int N, int T, int B, int bin,
+                               int nchunk, void* workspace, size_t ws_bytes, float* grad, void* stream);
 
 /* slopesMaps post-processing (Pyramid.py:835-854, grabQuadrant 946-961):
  *   frames [B][cam][cam] float32, valid [w][w] uint8 (validI4Q),

@@ -60,6 +60,10 @@ SIGNATURES = {
                                                ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_generic_focal': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                          c_int, c_int, c_int, c_void_p, c_size_t, c_void_p, c_void_p]),
+    'oopao_pwfs_generic_adjoint_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
+    'oopao_pwfs_generic_adjoint': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_size_t, c_void_p,
+                                           c_void_p]),
     'oopao_loop_residual_workspace_bytes': (c_int, [ctypes.POINTER(c_size_t)]),
     'oopao_loop_residual': (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_void_p, c_void_p,
                                     c_void_p, c_void_p]),

@@ -119,6 +119,121 @@ k_gen_rows_inv(const float2* __restrict__ inter2, const float2* __restrict__ tw,
   for (int X = lane; X < N; X += 64) dst[X] = acc[X];
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint (vector-Jacobian product) of the general frame. With U_i the support
+// field of point i (u_i = amp exp(i phi) tx_i ty_i, the PSF-centring phasor
+// folded into tx, ty) and E_i = F+ (M' . F U_i) the forward above (2-D; F the
+// forward DFT, F+ its conjugate transpose, the unnormalised inverse; M' holds
+// both 1/N), and g = dL/dI (the detector gradient spread over each bin x bin
+// block):
+//   W_i = F+ (conj(M') . F (g . E_i)),  dL/dphi = sum_i 2 Im(W_i conj(U_i))
+// on the pupil support. The modulation points are recomputed by passes 1-2,
+// then
+//   k_gadj_mid  : per (point, row Y): E row = IFFT(G row), times g, FFT, back
+//                 into G's row
+//   k_gadj_cols : per (point, column kx): IFFT(conj(M') . FFT(column)), the
+//                 pupil rows kept -> Z [B*T][n][N] (over pass 1's buffer)
+//   k_gadj_rows : per (screen, point chunk, pupil row y): IFFT of Z's row,
+//                 2 scale Im(w conj(u)) on the pupil columns, summed over the
+//                 chunk's points
+// and the fixed-order sum over chunks.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64)
+k_gadj_mid(float2* __restrict__ inter2, const float* __restrict__ gframe, const float2* __restrict__ tw, GenPlan p,
+           int T, int bin) {
+  const int N = p.N;
+  const int cam = N / bin;
+  const int lane = threadIdx.x;
+  const size_t task = blockIdx.x;  // (screen * T + point) * N + Y
+  const int Y = (int)(task % N);
+  const size_t b = task / N / T;
+  float2* a = gen_lds;
+  float2* s = gen_lds + N;
+  float2* row = inter2 + task * N;
+  for (int X = lane; X < N; X += 64) a[X] = row[X];
+  wave_lds_sync();
+  float2* e = gen_fft<+1>(a, s, tw, p, lane);
+  float2* o = (e == a) ? s : a;
+  const float* grow = gframe + (b * cam + Y / bin) * (size_t)cam;
+  for (int X = lane; X < N; X += 64) e[X] = cscale(e[X], grow[X / bin]);
+  wave_lds_sync();
+  const float2* r = gen_fft<-1>(e, o, tw, p, lane);
+  for (int X = lane; X < N; X += 64) row[X] = r[X];
+}
+
+// as k_gen_cols (adjacent columns per workgroup, consecutive groups on one XCD), with the conjugate mask,
+// all N rows in and the n pupil rows out
+__global__ void __launch_bounds__(256)
+k_gadj_cols(const float2* __restrict__ inter2, const float2* __restrict__ maskT, const float2* __restrict__ tw,
+            GenPlan p, int n, int c0, float2* __restrict__ Z) {
+  const int N = p.N;
+  const int wpg = (int)(blockDim.x >> 6);
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int NG = (N + wpg - 1) / wpg;
+  const size_t task = (size_t)xcd_task((int)blockIdx.x, (int)gridDim.x);
+  const int kx = (int)(task % NG) * wpg + wave;
+  const size_t bi = task / NG;
+  if (kx >= N) return;  // whole wavefront; the waves of a workgroup share no data
+  float2* a = gen_lds + (size_t)wave * 2 * N;
+  float2* s = a + N;
+  const float2* src = inter2 + bi * N * N + kx;
+  for (int Y = lane; Y < N; Y += 64) a[Y] = src[(size_t)Y * N];
+  wave_lds_sync();
+  float2* r = gen_fft<-1>(a, s, tw, p, lane);
+  float2* o = (r == a) ? s : a;
+  const float2* mcol = maskT + (size_t)kx * N;
+  for (int ky = lane; ky < N; ky += 64) r[ky] = cmulc(r[ky], mcol[ky]);
+  wave_lds_sync();
+  const float2* e = gen_fft<+1>(r, o, tw, p, lane);
+  float2* dst = Z + bi * (size_t)n * N + kx;
+  for (int y = lane; y < n; y += 64) dst[(size_t)y * N] = e[c0 + y];
+}
+
+// gpart[b][chunk][y][x] = 2 scale sum_{i in chunk} Im(w_i(y, x) conj(u_i(y, x)))
+__global__ void __launch_bounds__(64)
+k_gadj_rows(const float2* __restrict__ Z, const float* __restrict__ opd, float scale, const float* __restrict__ amp,
+            const float2* __restrict__ tx, const float2* __restrict__ ty, const float2* __restrict__ tw, GenPlan p,
+            int n, int T, int nchunk, int c0, float* __restrict__ gpart) {
+  const int N = p.N;
+  const int lane = threadIdx.x;
+  const size_t task = blockIdx.x;  // (screen * nchunk + chunk) * n + y
+  const int y = (int)(task % n);
+  const size_t bc = task / n;
+  const int c = (int)(bc % nchunk);
+  const size_t b = bc / nchunk;
+  const int i0 = (int)(((long)c * T) / nchunk);
+  const int i1 = (int)(((long)(c + 1) * T) / nchunk);
+  float2* a = gen_lds;
+  float2* s = gen_lds + N;
+  float2* fld = gen_lds + 2 * N;                          // amp exp(i scale opd) on the pupil row
+  float* acc = reinterpret_cast<float*>(gen_lds + 2 * N + n);
+  const float* orow = opd + (b * n + y) * (size_t)n;
+  const float* arow = amp + (size_t)y * n;
+  for (int x = lane; x < n; x += 64) {
+    float sn, cs;
+    sincosf(orow[x] * scale, &sn, &cs);
+    fld[x] = make_float2(arow[x] * cs, arow[x] * sn);
+    acc[x] = 0.f;
+  }
+  for (int i = i0; i < i1; ++i) {
+    const float2* src = Z + ((b * T + i) * n + y) * (size_t)N;
+    for (int X = lane; X < N; X += 64) a[X] = src[X];
+    wave_lds_sync();
+    const float2* w = gen_fft<+1>(a, s, tw, p, lane);
+    const float2 tyv = ty[(size_t)i * n + y];
+    const float2* txi = tx + (size_t)i * n;
+    for (int x = lane; x < n; x += 64) {
+      const float2 u = cmul(cmul(fld[x], txi[x]), tyv);
+      const float2 wv = w[c0 + x];
+      acc[x] += wv.y * u.x - wv.x * u.y;  // Im(w conj(u))
+    }
+    wave_lds_sync();
+  }
+  float* dst = gpart + task * (size_t)n;
+  for (int x = lane; x < n; x += 64) dst[x] = 2.f * scale * acc[x];
+}
+
 // Focal camera: one wavefront per (screen, point chunk, window column);
 // sum over points of |FFT_N(column)|^2 / N^2 on the window rows.
 __global__ void __launch_bounds__(64)
@@ -348,6 +463,78 @@ int oopao_pwfs_generic_focal(const float* opd, float scale, const float* amp, co
   OOPAO_LAUNCH_CHECK();
   const size_t cnt = (size_t)crop * crop;
   hipLaunchKernelGGL(k_gen_sum, dim3((unsigned)((B * cnt + 255) / 256)), dim3(256), 0, st, part, out, B, nchunk, cnt);
+  OOPAO_LAUNCH_CHECK();
+  return OK;
+}
+
+int oopao_pwfs_generic_adjoint_bytes(int n, int N, int T, int B, int nchunk, size_t* bytes) {
+  OOPAO_CHECK_ARG(bytes != nullptr, "bytes is null");
+  OOPAO_CHECK_ARG(n > 0 && n <= N && T > 0 && B > 0 && gen_nchunk_check(T, nchunk),
+                  "invalid sizes n=%d N=%d T=%d B=%d nchunk=%d", n, N, T, B, nchunk);
+  *bytes = al256((size_t)B * T * n * (size_t)N * sizeof(float2)) +
+           al256((size_t)B * T * (size_t)N * N * sizeof(float2)) +
+           al256((size_t)B * nchunk * (size_t)n * n * sizeof(float));
+  return OK;
+}
+
+int oopao_pwfs_generic_adjoint(const float* opd, float scale, const float* amp, const void* tx, const void* ty,
+                               const void* maskT, const float* gframe, int n, int N, int T, int B, int bin,
+                               int nchunk, void* workspace, size_t ws_bytes, float* grad, void* stream) {
+  OOPAO_CHECK_ARG(opd && amp && tx && ty && maskT && gframe && workspace && grad, "null pointer argument");
+  size_t need = 0;
+  int rc = oopao_pwfs_generic_adjoint_bytes(n, N, T, B, nchunk, &need);
+  if (rc) return rc;
+  OOPAO_CHECK_ARG(bin >= 1 && N % bin == 0, "binning %d does not divide N=%d", bin, N);
+  GenPlan p;
+  if (!gen_plan(N, &p)) {
+    set_error("padded length N=%d outside 2..%d", N, kGenMaxN);
+    return ERR_UNSUPPORTED;
+  }
+  if (ws_bytes < need) {
+    set_error("workspace too small: %zu < %zu bytes", ws_bytes, need);
+    return ERR_WORKSPACE;
+  }
+  const size_t rows_tasks = (size_t)B * T * N;
+  const int wpg = (4 * gen_lds_bytes(N, 0) <= 65536) ? 4 : 2;
+  const size_t col_groups = (size_t)B * T * ((N + wpg - 1) / wpg);
+  if (rows_tasks > 0x7FFFFFFF || col_groups > 0x7FFFFFFF) {
+    set_error("too many transform tasks (B*T*N = %zu)", rows_tasks);
+    return ERR_UNSUPPORTED;
+  }
+  const float2* tw = nullptr;
+  rc = twiddles(N, &tw);
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  char* w = reinterpret_cast<char*>(workspace);
+  float2* inter1 = reinterpret_cast<float2*>(w);
+  w += al256((size_t)B * T * n * (size_t)N * sizeof(float2));
+  float2* inter2 = reinterpret_cast<float2*>(w);
+  w += al256((size_t)B * T * (size_t)N * N * sizeof(float2));
+  float* gpart = reinterpret_cast<float*>(w);
+  const int c0 = N / 2 - n / 2;
+  const float2* txp = reinterpret_cast<const float2*>(tx);
+  const float2* typ = reinterpret_cast<const float2*>(ty);
+  const float2* mT = reinterpret_cast<const float2*>(maskT);
+  const size_t l2 = gen_lds_bytes(N, 0);
+  // the forward's first two passes recompute every point's column-transformed field
+  hipLaunchKernelGGL(k_gen_rows, dim3((unsigned)((size_t)B * T * n)), dim3(64), l2, st, opd, scale, amp, txp, typ,
+                     tw, p, n, T, c0, inter1);
+  OOPAO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_gen_cols, dim3((unsigned)col_groups), dim3(64 * wpg), wpg * l2, st, inter1, mT, tw, p, n, c0,
+                     inter2);
+  OOPAO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_gadj_mid, dim3((unsigned)rows_tasks), dim3(64), l2, st, inter2, gframe, tw, p, T, bin);
+  OOPAO_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_gadj_cols, dim3((unsigned)col_groups), dim3(64 * wpg), wpg * l2, st, inter2, mT, tw, p, n, c0,
+                     inter1);
+  OOPAO_LAUNCH_CHECK();
+  const size_t l3 = l2 + (size_t)n * (sizeof(float2) + sizeof(float));
+  hipLaunchKernelGGL(k_gadj_rows, dim3((unsigned)((size_t)B * nchunk * n)), dim3(64), l3, st, inter1, opd, scale,
+                     amp, txp, typ, tw, p, n, T, nchunk, c0, gpart);
+  OOPAO_LAUNCH_CHECK();
+  const size_t cnt = (size_t)n * n;
+  hipLaunchKernelGGL(k_gen_sum, dim3((unsigned)((B * cnt + 255) / 256)), dim3(256), 0, st, gpart, grad, B, nchunk,
+                     cnt);
   OOPAO_LAUNCH_CHECK();
   return OK;
 }

@@ -1,8 +1,9 @@
 """Differentiable modulated PWFS: torch.autograd.Function over the HIP forward and its adjoint.
 
-The forward is the fast-path frame kernel (oopao_pwfs_frames); the backward is the hand-written
-adjoint (oopao_pwfs_adjoint): w_i = M^H (g . E_i) with the conjugated separable mask, and
-dL/dphi = sum_i 2 Im(w_i conj(u_i)), recomputing every modulation point instead of storing it.
+The forward is the engine's frame kernel (oopao_pwfs_frames on the separable path,
+oopao_pwfs_generic_frames for any other N, mask or binning); the backward is the matching hand-written
+adjoint (oopao_pwfs_adjoint / oopao_pwfs_generic_adjoint): w_i = M^H (g . E_i) with the conjugated mask,
+and dL/dphi = sum_i 2 Im(w_i conj(u_i)), recomputing every modulation point instead of storing it.
 The slopesMaps post-processing (Pyramid.py:835-854, grabQuadrant 946-961) is written as torch
 operations so that autograd carries the gradient from the slopes to the detector frame.
 
@@ -53,15 +54,31 @@ def slopes_maps(frames, validI4Q, quad, w, ref2d=0.0, units=1.0):
     return s2d, s2d[:, vs]
 
 
+def sensor_frames(wfs, phase):
+    """Differentiable detector frames of a drop-in Pyramid or Bi-O Edge for phase maps [B, n, n] in
+    radians (before the detector's extra binning): the Pyramid's frame, or the Bi-O Edge's four mask
+    images tiled [[A, B], [C, D]] (BioEdge.py:510-544)."""
+    amp = wfs._amp()
+    engines = getattr(wfs, 'engines', None)
+    if engines is None:
+        return pwfs_frames(wfs.engine, phase, amp, 1.0)
+    t = [pwfs_frames(e, phase, amp, 1.0) for e in engines]
+    return torch.cat((torch.cat((t[0], t[1]), dim=2), torch.cat((t[2], t[3]), dim=2)), dim=1)
+
+
 def pwfs_signal(wfs, phase):
-    """Differentiable slopes [B, nSignal] of a drop-in Pyramid (slopesMaps post-processing) for phase
-    maps [B, n, n] in radians at the WFS wavelength; gradients flow to ``phase``."""
+    """Differentiable slopes [B, nSignal] of a drop-in Pyramid or Bi-O Edge (slopesMaps post-processing)
+    for phase maps [B, n, n] in radians at the WFS wavelength; gradients flow to ``phase``. Any sensor
+    geometry of the separable or the general path (any N, mask, integer binning) is covered."""
     if wfs.postProcessing != 'slopesMaps':
         raise OopaoError('pwfs_signal supports the slopesMaps post-processing')
-    if wfs.engine.mode != 'fast' or wfs.binning != 1:
-        raise OopaoError('the PWFS adjoint is implemented for the separable fast path without extra binning')
+    if wfs.engine.fractional:
+        raise OopaoError('the PWFS adjoint does not cover non-integer detector sampling')
     phase = phase.reshape(-1, wfs.engine.n, wfs.engine.n)
-    frames = pwfs_frames(wfs.engine, phase, wfs._amp(), 1.0)
+    frames = sensor_frames(wfs, phase)
+    if wfs.binning != 1:                       # the detector's extra binning (Pyramid.py:800-808)
+        b, c = wfs.binning, frames.shape[-1]
+        frames = frames.reshape(-1, c // b, b, c // b, b).sum(dim=(2, 4))
     quad, w = wfs._quadrant_origins()
     valid = torch.from_numpy(np.asarray(wfs.validI4Q, bool)).to(frames.device)
     ref = wfs._referenceSignal_2D if wfs._referenceSignal_2D is not None else 0.0

@@ -413,14 +413,18 @@ class PWFSEngine:
 
     def adjoint(self, opd, scale, amp, gframe, weighted=False):
         """dL/dopd [B, n, n] for the cotangent gframe = dL/dframes [B, cam, cam] of
-        frames(opd, scale, amp, weighted=weighted) (fast path; the modulation points are recomputed)."""
-        if self.mode != 'fast' or self.fractional:
-            raise OopaoError('the PWFS adjoint is implemented for the separable fast path with 4 x 4 binning only')
+        frames(opd, scale, amp, weighted=weighted): the separable path's adjoint (oopao_pwfs_adjoint) or the
+        general path's (oopao_pwfs_generic_adjoint: any N, mask and integer binning); the modulation points
+        are recomputed. Non-integer detector sampling (area-integrated pixels) has no adjoint."""
+        if self.fractional:
+            raise OopaoError('the PWFS adjoint does not cover non-integer detector sampling')
         opd = self._check(opd, amp)
         B = opd.shape[0]
         gframe = gframe.reshape(B, self.cam, self.cam)
         if gframe.dtype != torch.float32 or not gframe.is_contiguous() or gframe.device != self.device:
             gframe = gframe.to(self.device, torch.float32).contiguous()
+        if self.mode == 'generic':
+            return self._adjoint_generic(opd, scale, amp, gframe, weighted)
         nchunk, ppt = self._knobs(B)
         nb = ctypes.c_size_t(0)
         _lib.call('oopao_pwfs_adjoint_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
@@ -430,6 +434,18 @@ class PWFSEngine:
                   _ptr(self.pstart), _ptr(self._ty_sorted(weighted)), _ptr(self.tx_s), _ptr(self.mx), _ptr(self.my), _ptr(gframe),
                   self.n, self.N, self.T, B, nchunk, ppt, _ptr(ws), ctypes.c_size_t(nb.value), _ptr(grad),
                   _stream_ptr(self.device))
+        return grad
+
+    def _adjoint_generic(self, opd, scale, amp, gframe, weighted):
+        B = opd.shape[0]
+        nchunk = self._generic_nchunk(B)
+        nb = ctypes.c_size_t(0)
+        _lib.call('oopao_pwfs_generic_adjoint_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
+        ws = self._scratch(nb.value)
+        grad = torch.empty((B, self.n, self.n), dtype=torch.float32, device=self.device)
+        _lib.call('oopao_pwfs_generic_adjoint', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self._tx(weighted)),
+                  _ptr(self.ty), _ptr(self._full_maskT()), _ptr(gframe), self.n, self.N, self.T, B, self.bin, nchunk,
+                  _ptr(ws), ctypes.c_size_t(nb.value), _ptr(grad), _stream_ptr(self.device))
         return grad
 
     # split passes (same result as frames(); used to time the passes separately)
