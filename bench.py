@@ -130,23 +130,27 @@ def closed_loop_leg(tel, ngs, wfs, dev, n_iter=1000, n_modes=1000):
     out = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=n_iter, strehl_every=10)
     dt = time.perf_counter() - t0
     # the same loop (same screens, same start) with the reconstructor held in BF16 (plain, and split
-    # BF16 = hi + lo) on the matrix-core GEMV path; each residual trace is compared with the float32 loop's
+    # BF16 = hi + lo) on the matrix-core GEMV path, and with the reconstructor in its factored form
+    # (M2C, M): M2C @ (M @ s), 61 MB per iteration instead of 210 MB; each residual trace is compared with
+    # the float32 loop's
     half = n_iter // 2
     r32 = out['wfe_residual']
     forms = {}
-    for prec in ('bf16', 'bf16x3'):
+    factors = (M2C, calib.M_dev)
+    for key, prec, rec in (('bf16', 'bf16', R), ('bf16x3', 'bf16x3', R), ('factored', 'fp32', factors),
+                           ('factored_bf16x3', 'bf16x3', factors)):
         atm.generateNewPhaseScreen(10)
         dm.coefs = np.zeros(dm.nValidAct)
-        run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=5, reconstructor_precision=prec)
+        run_closed_loop(tel, ngs, atm, dm, wfs, rec, gain=0.4, n_iter=5, reconstructor_precision=prec)
         atm.generateNewPhaseScreen(10)
         dm.coefs = np.zeros(dm.nValidAct)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        o = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=0.4, n_iter=n_iter, strehl_every=10,
+        o = run_closed_loop(tel, ngs, atm, dm, wfs, rec, gain=0.4, n_iter=n_iter, strehl_every=10,
                             reconstructor_precision=prec)
         dtp = time.perf_counter() - t0
         r = o['wfe_residual']
-        forms[prec] = {'value': round(n_iter / dtp, 2), 'unit': 'iterations/s',
+        forms[key] = {'value': round(n_iter / dtp, 2), 'unit': 'iterations/s',
                        'wfe_residual_nm': round(float(np.mean(r[half:])), 1),
                        'strehl_pct_mean_second_half': round(float(np.mean(o['strehl'][half // 10:])), 2),
                        'residual_trace_max_rel_diff_vs_fp32': float(np.max(np.abs(r - r32) / r32)),
@@ -160,7 +164,9 @@ def closed_loop_leg(tel, ngs, wfs, dev, n_iter=1000, n_modes=1000):
             'wfe_residual_nm': round(float(np.mean(out['wfe_residual'][half:])), 1),
             'strehl_pct_mean_second_half': round(float(np.mean(out['strehl'][half // 10:])), 2),
             'note': 'wall clock of the loop incl. atmosphere update (host RNG draws) and Strehl every 10 it.',
-            'bf16_reconstructor': bf16, 'split_bf16_reconstructor': forms['bf16x3']}
+            'bf16_reconstructor': bf16, 'split_bf16_reconstructor': forms['bf16x3'],
+            'factored_reconstructor': dict(forms['factored'], form='M2C @ (M @ s), float32'),
+            'factored_split_bf16_reconstructor': dict(forms['factored_bf16x3'], form='M2C @ (M @ s), split BF16')}
 
 
 def atmosphere_leg(tel, n_steps=200):

@@ -7,7 +7,12 @@ closed_loop/run_cl.py:176-217): per iteration
     dm.coefs = dm.coefs - gain * reconstructor @ s
     frame_delay == 2: s = wfs.signal
 with the OPD sum, PWFS frame, slopes, reconstructor GEMV and integrator update
-kept on the device (no host round trip per iteration); with
+kept on the device (no host round trip per iteration). The reconstructor is the
+matrix R [nAct, nSignal] the tutorial forms (reconstructor = M2C_CL @ calib_CL.M),
+or the pair (M2C [nAct, nModes], M [nModes, nSignal]) of its factors: then
+R @ s = M2C @ (M @ s), two GEMVs moving (nAct + nSignal) nModes values per
+iteration instead of nAct nSignal (61 MB instead of 210 MB in float32 at 1000
+modes of the 80 x 80 sensor), the same update up to float rounding. With
 ``reconstructor_precision='bf16'`` the reconstructor is held in BF16 (half the
 bytes per iteration, FP32 accumulation), with ``'bf16x3'`` as split BF16 (hi + lo,
 the reconstructor to 2^-16, the float32 loop's accuracy). Records the
@@ -28,13 +33,25 @@ from ._device import ptr, stream_ptr
 def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100, frame_delay=2,
                     record_slopes=False, strehl_every=0, psf_sampling=2, reconstructor_precision='fp32'):
     dev = wfs.device
-    R = reconstructor if isinstance(reconstructor, torch.Tensor) else torch.from_numpy(
-        np.ascontiguousarray(reconstructor, dtype=np.float32))
-    R = R.to(dev, torch.float32).contiguous()
-    nAct, nSig = R.shape
+
+    def as_dev(a):
+        a = a if isinstance(a, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32))
+        return a.to(dev, torch.float32).contiguous()
+    factored = isinstance(reconstructor, (tuple, list))
+    if factored:
+        if len(reconstructor) != 2:
+            raise ValueError('a factored reconstructor is the pair (M2C, M), got %d factors' % len(reconstructor))
+        F1, F2 = as_dev(reconstructor[0]), as_dev(reconstructor[1])
+        if F1.dim() != 2 or F2.dim() != 2 or F1.shape[1] != F2.shape[0]:
+            raise ValueError('factors %s and %s do not chain' % (tuple(F1.shape), tuple(F2.shape)))
+        nAct, nSig, nModes = F1.shape[0], F2.shape[1], F1.shape[1]
+        mode_vec = torch.empty(nModes, dtype=torch.float32, device=dev)
+    else:
+        R = as_dev(reconstructor)
+        nAct, nSig = R.shape
     if nAct != dm.nValidAct or nSig != wfs.nSignal:
         raise ValueError('reconstructor shape %s does not match (nAct=%d, nSignal=%d)'
-                         % (tuple(R.shape), dm.nValidAct, wfs.nSignal))
+                         % ((nAct, nSig), dm.nValidAct, wfs.nSignal))
     if frame_delay not in (1, 2):
         raise ValueError('frame_delay must be 1 or 2, got %r' % (frame_delay,))
     if reconstructor_precision not in ('fp32', 'bf16', 'bf16x3'):
@@ -43,7 +60,11 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
     recon_bf16 = None
     if reconstructor_precision != 'fp32':
         from .contract import Contraction, PackedBF16
-        recon_bf16 = Contraction(PackedBF16(R, split=reconstructor_precision == 'bf16x3'))
+        split = reconstructor_precision == 'bf16x3'
+        if factored:
+            recon_bf16 = [Contraction(PackedBF16(F1, split=split)), Contraction(PackedBF16(F2, split=split))]
+        else:
+            recon_bf16 = Contraction(PackedBF16(R, split=split))
     pupil = tel._pupil_dev
     pupil_f = (pupil > 0).float().contiguous()
     npix = int((pupil > 0).sum())
@@ -93,8 +114,17 @@ def run_closed_loop(tel, ngs, atm, dm, wfs, reconstructor, gain=0.4, n_iter=100,
         if record_slopes:
             slopes[i] = s_cur[0]
         sig = s_cur[0] if frame_delay == 1 else s_prev[0]
-        # coefs <- coefs - gain * R @ sig   (R [nAct, nSig] row-major GEMV, float32 or BF16 matrix)
-        if recon_bf16 is not None:
+        # coefs <- coefs - gain * R @ sig   (R [nAct, nSig] row-major GEMV, float32 or BF16 matrix; factored:
+        # the modal vector M @ sig first)
+        if factored and recon_bf16 is not None:
+            recon_bf16[1].gemv(sig, out=mode_vec, precision=reconstructor_precision)
+            recon_bf16[0].gemv(mode_vec, y0=coefs, alpha=-gain, beta=1.0, out=new_coefs,
+                               precision=reconstructor_precision)
+        elif factored:
+            _lib.call('oopao_gemv', ptr(F2), ptr(sig), ptr(mode_vec), None, nModes, nSig, 1.0, 0.0, stream_ptr(dev))
+            _lib.call('oopao_gemv', ptr(F1), ptr(mode_vec), ptr(new_coefs), ptr(coefs), nAct, nModes, float(-gain),
+                      1.0, stream_ptr(dev))
+        elif recon_bf16 is not None:
             recon_bf16.gemv(sig, y0=coefs, alpha=-gain, beta=1.0, out=new_coefs, precision=reconstructor_precision)
         else:
             _lib.call('oopao_gemv', ptr(R), ptr(sig), ptr(new_coefs), ptr(coefs), nAct, nSig, float(-gain), 1.0,

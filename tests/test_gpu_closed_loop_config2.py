@@ -9,7 +9,9 @@ oracle/science; the reconstructor (a push-pull calibration of the first 1000 KL 
 specifies) is shared. Tolerances: residual wfe trace 1e-4 relative, per-iteration slopes rel-L2 1e-4, Strehl
 1e-3 %. The same loop with the reconstructor in split BF16 (hi + lo on the matrix cores' format, FP32
 accumulation) must stay within 1e-4 of the float32 loop (wfe trace and slopes); plain BF16 within 5e-2 (its
-stated tolerance: the reconstructor to 2^-8). Every form's error against the float64 oracle is printed.
+stated tolerance: the reconstructor to 2^-8). The factored reconstructor (M2C, M), applied as M2C @ (M @ s)
+in float32 and in split BF16, stays within 1e-4 of the float32 loop. Every form's error against the float64
+oracle is printed.
 """
 import os
 
@@ -57,11 +59,12 @@ def test_config2_loop_traces_vs_oracle():
     # 30 iterations by default (the float64 oracle frame takes ~1 s); longer runs via the environment
     n_iter, gain = int(os.environ.get('OOPAO_CONFIG2_LOOP_ITERS', '30')), 0.4
     forms = {}
-    for prec in ('fp32', 'bf16x3', 'bf16'):
+    for key, prec, rec in (('fp32', 'fp32', R), ('bf16x3', 'bf16x3', R), ('bf16', 'bf16', R),
+                           ('fp32_factored', 'fp32', (M2C, calib.M)), ('bf16x3_factored', 'bf16x3', (M2C, calib.M))):
         atm.generateNewPhaseScreen(seed)            # the same initial screens and RandomState streams
         dm.coefs = np.zeros(dm.nValidAct)
-        forms[prec] = run_closed_loop(tel, ngs, atm, dm, wfs, R, gain=gain, n_iter=n_iter, record_slopes=True,
-                                      strehl_every=1, reconstructor_precision=prec)
+        forms[key] = run_closed_loop(tel, ngs, atm, dm, wfs, rec, gain=gain, n_iter=n_iter, record_slopes=True,
+                                     strehl_every=1, reconstructor_precision=prec)
     torch.cuda.synchronize()
     out = forms['fp32']
     flux = opw.flux_map(tel.pupil, ngs.nPhoton, tel.samplingTime, tel.D)
@@ -91,7 +94,7 @@ def test_config2_loop_traces_vs_oracle():
     print('Strehl device', np.round(out['strehl'], 4), 'oracle', np.round(sr, 4))
     print('max rel wfe %.2e  worst slopes rel-L2 %.2e  max Strehl diff %.2e %%' % (e_wfe, e_s, e_sr))
     assert e_wfe < 1e-4 and e_s < 1e-4 and e_sr < 1e-3
-    for prec, tol in (('bf16x3', 1e-4), ('bf16', 5e-2)):
+    for prec, tol in (('bf16x3', 1e-4), ('bf16', 5e-2), ('fp32_factored', 1e-4), ('bf16x3_factored', 1e-4)):
         f = forms[prec]
         d_wfe = np.max(np.abs(f['wfe_residual'] - out['wfe_residual']) / out['wfe_residual'])
         d_s = max(rel_l2(f['slopes'][k], out['slopes'][k]) for k in range(n_iter))
