@@ -48,6 +48,16 @@ __device__ __forceinline__ int band_col(int gi, Bands bd) {
   return gi < na ? bd.a0 + gi : bd.b0 + (gi - na);
 }
 
+// Length of the twiddle-table prefix both passes read: the pruned forward (from span 4) and the full
+// inverse transform. At the sizes tw_lds() names, the 64-lane kernels copy it into LDS once per workgroup.
+template <int N, int TPT>
+__host__ __device__ constexpr int sep_tw_len() {
+  return (plan_tw_max(N, N / TPT, 4, N / 4) > plan_tw_max(N, N / TPT, 1, N) ? plan_tw_max(N, N / TPT, 4, N / 4)
+                                                                               : plan_tw_max(N, N / TPT, 1, N)) +
+         1;
+}
+__host__ __device__ constexpr bool tw_lds(int N) { return N == 1024 || N == 1280; }
+
 // Pupil-plane field u[b][y][x] = amp[y][x] exp(i scale opd[b][y][x]), once per
 // screen (the reference re-forms it per modulation point, Pyramid.py:619-620).
 __global__ void k_field(const float* __restrict__ opd, float scale, const float* __restrict__ amp,
@@ -69,7 +79,9 @@ __global__ void k_field(const float* __restrict__ opd, float scale, const float*
 // phasor. With BANDS only the pupil-plane columns of the detector columns in
 // `bd` are written (a separate instantiation: the whole-frame kernel carries no
 // column tests). FWD_ONLY stops after the forward transform (focal-plane field
-// rows, no mask).
+// rows, no mask). As in the columns pass, 64-lane transforms at the tw_lds()
+// sizes read their twiddles from an LDS copy (48 KB per workgroup; at N = 1280
+// 84 -> 79 us for the slopes-only rows, 86 -> 77 us for the whole frame).
 // ---------------------------------------------------------------------------
 template <int N, int TPT, bool FWD_ONLY = false, bool BANDS = false>
 __global__ void __launch_bounds__(64 * kWaves)
@@ -96,6 +108,14 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
   const bool active = y_raw < n;
   const int y = active ? y_raw : n - 1;
   float2* xl = lds[wave * G + h];
+  constexpr bool kTwLds = (TPT == 64) && tw_lds(N);
+  constexpr int kTw = kTwLds ? sep_tw_len<N, TPT>() : 1;
+  __shared__ float2 twl[kTw];
+  if constexpr (kTwLds) {
+    for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
+    __syncthreads();
+  }
+  const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;
 
   float2 u[E];
   const float2* fp = field + ((size_t)b * n + y) * n;
@@ -108,12 +128,12 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(u[m], txi[(t + TPT * m) >> 2]);
-    fft_fwd_pruned<N, TPT, 4, true>(v, xl, tw, t);
+    fft_fwd_pruned<N, TPT, 4, true>(v, xl, twp, t);
     if constexpr (!FWD_ONLY) {
       const float2* mxp = launder(mx);
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = cmul(v[m], mxp[t + TPT * m]);
-      fft_full<N, TPT, +1, true>(v, xl, tw, t);
+      fft_full<N, TPT, +1, true>(v, xl, twp, t);
     }
     if (active) {
       const int k0 = min(max(pstart[i], 0), T);
@@ -147,7 +167,7 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
 //  * narrower transforms: one wavefront per (b, chunk, g), its 64/TPT lane
 //    groups taking the columns.
 // Only the detector columns in `bd` are computed (one task per such column).
-// With 64-lane transforms at the sizes cols_tw_lds() names, the workgroup first
+// With 64-lane transforms at the sizes tw_lds() names, the workgroup first
 // copies the twiddle entries the plans read into LDS: every stage's twiddles
 // are then LDS reads (lgkm queue, ~100-cycle latency) instead of global loads
 // queued behind the column loads (the table loads cost ~9 % of the kernel at
@@ -155,16 +175,6 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
 // needs 48 KB and three still fit per CU.
 // Output: part[b][chunk][g][r] (detector column-major partial sums).
 // ---------------------------------------------------------------------------
-// Length of the twiddle-table prefix the columns pass reads: the pruned forward (from span 4) and the
-// full inverse transform.
-template <int N, int TPT>
-__host__ __device__ constexpr int cols_tw_len() {
-  return (plan_tw_max(N, N / TPT, 4, N / 4) > plan_tw_max(N, N / TPT, 1, N) ? plan_tw_max(N, N / TPT, 4, N / 4)
-                                                                               : plan_tw_max(N, N / TPT, 1, N)) +
-         1;
-}
-__host__ __device__ constexpr bool cols_tw_lds(int N) { return N == 1024 || N == 1280; }
-
 template <int N, int TPT>
 __global__ void __launch_bounds__(64 * kWaves, fast_cols_min_waves(N))
 k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
@@ -174,8 +184,8 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   constexpr int G = 64 / TPT;
   constexpr int NG = N / 4;
   constexpr bool kWaveCols = (TPT == 64);
-  constexpr bool kTwLds = kWaveCols && cols_tw_lds(N);
-  constexpr int kTw = kTwLds ? cols_tw_len<N, TPT>() : 1;
+  constexpr bool kTwLds = kWaveCols && tw_lds(N);
+  constexpr int kTw = kTwLds ? sep_tw_len<N, TPT>() : 1;
   static_assert(!kTwLds || kTw <= N, "twiddle prefix within the table");
   static_assert(kWaves * NG <= 2 * kWaves * G * lds_len(N), "column sums fit in the exchange buffers");
   __shared__ float2 lds[kWaves * G][lds_len(N)];
