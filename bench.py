@@ -335,7 +335,9 @@ def im_leg(dev, n_modes=1000):
     wfs = Pyramid(20, tel, 3, 0.5, n_pix_separation=20, n_pix_edge=10, nTheta_user_defined=48)
     M2C = np.random.RandomState(1).randn(dm.nValidAct, n_modes)
     M2C /= np.std(M2C, axis=0)
-    InteractionMatrix(ngs, tel, dm, wfs, M2C[:, :50], 10e-9, nMeasurements=50, single_pass=False)
+    # warm-up on the timed call's routes: the same batch (workspace size) and >= 256 modes, so that the
+    # float64 pseudo-inverse takes the Gram-eigensolver route of the timed 632 x 1000 matrix
+    InteractionMatrix(ngs, tel, dm, wfs, M2C[:, :300], 10e-9, nMeasurements=250, single_pass=False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     calib = InteractionMatrix(ngs, tel, dm, wfs, M2C, 10e-9, nMeasurements=250, single_pass=False)

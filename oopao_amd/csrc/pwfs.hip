@@ -188,7 +188,7 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   constexpr int kTw = kTwLds ? sep_tw_len<N, TPT>() : 1;
   static_assert(!kTwLds || kTw <= N, "twiddle prefix within the table");
   static_assert(kWaves * NG <= 2 * kWaves * G * lds_len(N), "column sums fit in the exchange buffers");
-  __shared__ float2 lds[kWaves * G][lds_len(N)];
+  __shared__ float2 lds[kWaves * G][lds_len(N) + cols_group_pad(N, TPT)];
   __shared__ float2 twl[kTw];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;

@@ -38,6 +38,13 @@ __host__ __device__ constexpr int alt_tpt_cols(int N) {
   return N == 320 ? 32 : N == 640 ? 32 : N == 1024 ? 32 : N == 1280 ? 32 : 0;
 }
 
+// Padding (complex values) after each lane group's exchange buffer in the columns pass. At N = 320 the
+// four 16-lane transforms of a wavefront otherwise start their buffers on the same LDS bank; four values
+// of padding take the pass from 7.9 to 7.3 us per frame (config 1, 64 screens per call), although the
+// extra 0.5 KB per workgroup lowers its occupancy from four waves per SIMD to three. Other narrow sizes
+// would lose whole workgroups per CU (N = 640 already uses 80 KB per workgroup) and keep 0.
+__host__ __device__ constexpr int cols_group_pad(int N, int TPT) { return (N == 320 && TPT == 16) ? 4 : 0; }
+
 #define OOPAO_FAST_SIZES(X) X(64) X(128) X(256) X(320) X(384) X(512) X(640) X(768) X(1024) X(1280) X(1536) X(1920) X(2048)
 
 constexpr int kWaves = 4;  // wavefronts per workgroup
