@@ -25,12 +25,20 @@ def measure_modes(ngs, tel, dm, wfs, coefs):
 def measure_opd(tel, wfs, opd):
     """Slopes [nSignal, B] (device float32) of the OPD maps opd [B, n, n] (device, metres) through tel*wfs."""
     pupil = tel._pupil_dev
-    frames = wfs.frames_dev((opd * pupil).contiguous())
+    x = (opd * pupil).contiguous()
+    # the calibration keeps only the slopes: with an identity camera readout the slopesMaps signal reads
+    # the quadrant columns only (as in Pyramid.wfs_measure), and the camera's whole frame is computed if read
+    bands = None
+    if wfs.isInitialized and wfs.isCalibrated and wfs.cam.identity_readout():
+        bands = wfs.slope_bands()
+    frames = wfs.frames_dev(x, bands=bands)
     if wfs.binning != 1:
         b = wfs.binning
         c = frames.shape[-1]
         frames = frames.reshape(-1, c // b, b, c // b, b).sum(dim=(2, 4))
     wfs.cam.integrate(frames)
+    if bands is not None:
+        wfs.cam._defer_full(lambda: wfs.frames_dev(x))
     s2d, s = wfs._signal_processing_dev()
     return s.t()
 

@@ -104,6 +104,12 @@ def test_interaction_matrix_matches_oracle():
     Dpp = np.stack([0.5 * (o.measure_opd(od.opd(modes, M2C[:, k] * stroke))[1]
                            - o.measure_opd(od.opd(modes, -M2C[:, k] * stroke))[1]) / stroke for k in range(24)], axis=1)
     assert rel_l2(calib_pp.D, Dpp) < 1e-4
+    # the calibration measures slopes-only frames; read afterwards, the camera frame is the whole frame of
+    # the last measurement (the pulls of the last batch)
+    last = np.asarray(wfs.cam.frame)            # a batch reads as [res, res, B], as OOPAO's cubes
+    assert last.shape == (wfs.cam.resolution, wfs.cam.resolution, 24)
+    ph = od.opd(modes, -M2C[:, 23] * stroke) * tel.pupil * 2 * np.pi / ngs.wavelength
+    assert rel_l2(last[:, :, 23], o.frame(ph)) < 1e-5
 
 
 @pytest.mark.gpu
