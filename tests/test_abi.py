@@ -67,3 +67,47 @@ def test_tip_groups_default_and_user_paths():
     sq = [[x, y] for x in (-1.0, 0.0, 1.0) for y in (-1.0, 0.0, 1.0) if (x, y) != (0.0, 0.0)]
     _, pidx, reps = tip_groups(np.asarray(sq))
     assert reps.size == 3
+
+
+def test_fft_plan_query_without_gpu_calls():
+    # oopao_pwfs_plans lists the compiled lane counts per pass, the default plan first (0 = none)
+    from oopao_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip('library not built')
+    L = _lib.lib()
+
+    def plans(N):
+        r, c = (ctypes.c_int * 2)(), (ctypes.c_int * 2)()
+        rc = L.oopao_pwfs_plans(N, r, c)
+        return rc, list(r), list(c)
+    assert plans(1280) == (0, [64, 0], [64, 32])
+    assert plans(320) == (0, [16, 32], [16, 32])
+    assert plans(640) == (0, [32, 0], [16, 32])
+    assert plans(1024) == (0, [64, 0], [64, 32])
+    assert plans(2048) == (0, [64, 0], [64, 0])
+    rc, _, _ = plans(1000)
+    assert rc == 2 and b'N=1000' in L.oopao_last_error()
+    assert L.oopao_pwfs_plans(1280, None, None) == 1
+
+
+def test_generic_adjoint_sizes_and_argument_checks_without_gpu_calls():
+    # workspace = pass-1 rows [B*T][n][N] + pass-2 field [B*T][N][N] (complex64) + chunk partial gradients
+    # [B][nchunk][n][n] (float32), each rounded up to 256 bytes; invalid arguments fail before device work
+    from oopao_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip('library not built')
+    L = _lib.lib()
+    al = lambda x: (x + 255) // 256 * 256
+    nb = ctypes.c_size_t(0)
+    n, N, T, B, nch = 120, 264, 20, 2, 5
+    assert L.oopao_pwfs_generic_adjoint_bytes(n, N, T, B, nch, ctypes.byref(nb)) == 0
+    assert nb.value == al(B * T * n * N * 8) + al(B * T * N * N * 8) + al(B * nch * n * n * 4)
+    assert L.oopao_pwfs_generic_adjoint_bytes(n, N, T, B, T + 1, ctypes.byref(nb)) == 1
+    assert L.oopao_pwfs_generic_adjoint_bytes(N + 1, N, T, B, nch, ctypes.byref(nb)) == 1
+    d = ctypes.c_void_p(256)
+    assert L.oopao_pwfs_generic_adjoint(None, 1.0, d, d, d, d, d, n, N, T, B, 6, nch, d, 1 << 40, d, None) == 1
+    assert b'null' in L.oopao_last_error()
+    # a binning that does not divide N, then a workspace smaller than required
+    assert L.oopao_pwfs_generic_adjoint(d, 1.0, d, d, d, d, d, n, N, T, B, 7, nch, d, 1 << 40, d, None) == 1
+    assert b'binning' in L.oopao_last_error()
+    assert L.oopao_pwfs_generic_adjoint(d, 1.0, d, d, d, d, d, n, N, T, B, 6, nch, d, 100, d, None) == 4
