@@ -98,12 +98,34 @@ def column_bands(bands, cam):
     return (ctypes.c_int * 4)(merged[0][0], merged[0][1], merged[1][0], merged[1][1])
 
 
-def mask_vectors(a, b, N, n):
-    """my[N], mx[N] complex128 from the separable mask phase (see module docstring)."""
+def factor_vectors(p, q, N, n):
+    """my[N], mx[N] complex128 for a mask that factorises as p[k] q[l] (k: row, l: column), with the
+    zero-padding offset shift and the 1/N of each inverse transform folded in (see module docstring)."""
     c0 = N // 2 - n // 2
     k = np.arange(N)
     shift = np.exp(-2j * np.pi * k * c0 / N) / N
-    return np.exp(1j * a) * shift, np.exp(1j * b) * shift
+    return np.asarray(p, np.complex128) * shift, np.asarray(q, np.complex128) * shift
+
+
+def mask_vectors(a, b, N, n):
+    """my[N], mx[N] complex128 from the separable mask phase (see module docstring)."""
+    return factor_vectors(np.exp(1j * np.asarray(a)), np.exp(1j * np.asarray(b)), N, n)
+
+
+def rank1_mask_factors(M, tol=1e-5):
+    """(p, q) with M[k, l] = p[k] q[l] to ``tol`` of the largest entry when the complex mask M is of rank
+    one, else None. A Bi-O Edge mask (a knife-edge amplitude along one axis times a tip / tilt phase,
+    BioEdge.py:331-390) is; the factors are a column and a row of M through its largest entry."""
+    M = np.asarray(M, np.complex128)
+    k0, l0 = np.unravel_index(int(np.argmax(np.abs(M))), M.shape)
+    piv = M[k0, l0]
+    if piv == 0:
+        return None
+    p = M[:, l0] / piv
+    q = M[k0, :]
+    if np.max(np.abs(M - np.outer(p, q))) > tol * abs(piv):
+        return None
+    return p, q
 
 
 class Workspace:
@@ -215,14 +237,21 @@ class PWFSEngine:
     def set_mask(self, m, mask=None, factor=None, general=False):
         """Pyramid mask from its phase m [N, N] (and the complex64 mask exp(i m) the reference
         stores, Pyramid.py:388). ``factor`` multiplies the mask in the focal plane (a spatial filter);
-        ``general`` forces the full mask (when ``mask`` is not exp(i m)). A separable phase on a fast-path
-        geometry selects the two-pass kernels; anything else the general path with the full mask."""
+        ``general``: ``mask`` is not exp(i m) (an amplitude mask such as a Bi-O Edge knife edge) and is
+        used as given. On a fast-path geometry a separable phase, or a given mask of rank one, selects the
+        two-pass kernels; anything else the general path with the full mask."""
         m = np.asarray(m, np.float64)
         self._fullT = None
-        f = separable_mask_factors(m) if ((self.fast_geometry or self.fractional) and factor is None
-                                          and not general) else None
+        f = None
+        if (self.fast_geometry or self.fractional) and factor is None:
+            if not general:
+                s = separable_mask_factors(m)
+                f = None if s is None else mask_vectors(s[0], s[1], self.N, self.n)
+            elif mask is not None:
+                r = rank1_mask_factors(mask)
+                f = None if r is None else factor_vectors(r[0], r[1], self.N, self.n)
         if f is not None:
-            my, mx = mask_vectors(f[0], f[1], self.N, self.n)
+            my, mx = f
             self.my = torch.from_numpy(my.astype(np.complex64)).to(self.device)
             self.mx = torch.from_numpy(mx.astype(np.complex64)).to(self.device)
             self.mode = 'fast'

@@ -109,3 +109,44 @@ def test_bioedge_workspace_failure_halves_the_batch(monkeypatch):
     monkeypatch.delenv('OOPAO_AMD_MAX_WORKSPACE_BYTES')
     assert wfs2.max_batch_bytes <= 2 * per
     assert rel_l2(got.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
+
+
+def test_bioedge_rank_one_masks_take_the_separable_kernels():
+    # N = 4 n with 4 x 4 binning (n = 80, 20 subapertures, separation 20): each knife-edge mask (an
+    # amplitude along one axis times a tip / tilt phase) factorises as p[k] q[l] and runs on the two-pass
+    # kernels; frames, signals, the raw frame and the adjoint against the float64 oracle. The grey-length
+    # masks (quadrant blocks) are not of rank one and stay on the general path.
+    import torch
+    from oopao_amd.BioEdge import BioEdge
+    from oopao_amd.differentiable import pwfs_signal
+    print('bioedge rank-one masks', flush=True)
+    tel, ngs = _setup(80, 20)
+    kw = dict(modulation=3, grey_width=1, lightRatio=0.1, n_pix_separation=20, nTheta_user_defined=16,
+              postProcessing='slopesMaps')
+    wfs = BioEdge(nSubap=20, telescope=tel, **kw)
+    assert [e.mode for e in wfs.engines] == ['fast'] * 4
+    o = _oracle(tel, ngs, wfs, postProcessing='slopesMaps', n_pix_separation=20, grey_width=1,
+                nTheta_user_defined=16)
+    assert np.array_equal(np.asarray(wfs.validSignal), o.validSignal)
+    opd = _opd(tel, 5)
+    tel.OPD_no_pupil = opd
+    ngs * tel * wfs
+    phase = opd * tel.pupil * 2 * np.pi / ngs.wavelength
+    ef, es = rel_l2(wfs.cam.frame, o.frame(phase)), rel_l2(wfs.signal, o.measure(phase)[1])
+    er = rel_l2(wfs.raw_data, o.raw(phase))
+    print('fast-path Bi-O Edge: frame %.2e signal %.2e raw %.2e' % (ef, es, er), flush=True)
+    assert ef < 1e-5 and es < 1e-5 and er < 1e-5
+    # the separable adjoint: directional derivatives of <c, s(phi)> against float64 oracle differences
+    rs = np.random.RandomState(6)
+    c = rs.randn(int(wfs.nSignal))
+    ph = torch.from_numpy(phase[None].astype(np.float32)).cuda().requires_grad_(True)
+    (pwfs_signal(wfs, ph).double()[0] * torch.from_numpy(c).cuda()).sum().backward()
+    grad = ph.grad.cpu().numpy()[0]
+    for k in range(2):
+        d = rs.randn(80, 80) * tel.pupil
+        fd = (c @ o.measure(phase + 1e-4 * d)[1] - c @ o.measure(phase - 1e-4 * d)[1]) / 2e-4
+        an = float(np.sum(grad * d))
+        print('  oracle FD %.6e adjoint %.6e' % (fd, an), flush=True)
+        assert abs(fd - an) <= 1e-3 * abs(fd)
+    wfs_gl = BioEdge(nSubap=20, telescope=tel, grey_length=2, **kw)
+    assert [e.mode for e in wfs_gl.engines] == ['generic'] * 4
