@@ -75,9 +75,12 @@ int oopao_pwfs_workspace_bytes(int n, int N, int T, int B, int nchunk, size_t* b
  *                    folded in (Pyramid.py:388, 453-480)
  *   frames [B][N/bin][N/bin] float32: sum over points of |ifft2(fft2(E) mask)|^2,
  *          sum-binned (tools/tools.py:464-483)
- * pts_per_task: tip tables per rows-pass task; nchunk: point chunks of the
- * columns pass (both are tuning knobs, any valid value gives the same result
- * up to float summation order within a chunk).
+ * pts_per_task: the most tip tables one rows-pass workgroup transforms per
+ * pupil row (the pass launches the resident number of workgroups, times the
+ * rounds this cap needs, with equal ranges of work); nchunk: point chunks of
+ * the columns pass. Both are tuning knobs: the rows pass writes the same
+ * intermediate for any value, nchunk changes only the float summation order
+ * over points.
  * bands: null for the whole frame, or a host array {a0, a1, b0, b1} of detector
  * columns (0 <= a0 <= a1 <= b0 <= b1 <= N/4): only the columns in [a0, a1) and
  * [b0, b1) are computed (bit for bit as in the whole frame), the others are

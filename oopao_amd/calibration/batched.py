@@ -115,9 +115,10 @@ class PWFSAutotuner:
                 plan_ms[(which, lanes)] = self._time_pass(wfs, opd, amp, scale, B0, which)
             pick = min(plans, key=lambda v: plan_ms[(which, v)])
             e.set_plans(pick if which == 'rows' else e.lanes_rows, pick if which == 'cols' else e.lanes_cols)
-        # tile sizes at a fixed batch: modulation points per column-pass chunk, tip tables per row task
-        chunks = sorted({max(1, math.ceil(e.T / c)) for c in (4, 6, 8, 12)})
-        tasks = sorted({max(1, min(e.P, p)) for p in (4, 6, 8)})
+        # tile sizes at a fixed batch: modulation points per column-pass chunk, the cap of tip tables per
+        # rows-pass workgroup
+        chunks = sorted({max(1, math.ceil(e.T / c)) for c in (4, 6, 9, 12)})
+        tasks = sorted({max(1, min(e.P, p)) for p in (2, 6, 12)})
         trials = []
         for nc in chunks:
             for pt in tasks:

@@ -44,6 +44,27 @@ int twiddles(int N, const float2** out) {
   return OK;
 }
 
+int resident_workgroups(const void* kernel, int block, int* out) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, int> cache;
+  int dev = 0;
+  OOPAO_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto key = std::make_pair(dev, kernel);
+  auto it = cache.find(key);
+  if (it != cache.end()) {
+    *out = it->second;
+    return OK;
+  }
+  int cus = 0, per_cu = 0;
+  OOPAO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  OOPAO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
+  const int v = cus * per_cu > 0 ? cus * per_cu : 1;
+  cache[key] = v;
+  *out = v;
+  return OK;
+}
+
 }  // namespace oopao
 
 extern "C" const char* oopao_last_error(void) { return oopao::get_error(); }

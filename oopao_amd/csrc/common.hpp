@@ -68,4 +68,9 @@ const char* get_error();
 // it ahead, e.g. before a graph capture).
 int twiddles(int N, const float2** out);
 
+// Workgroups of `block` threads that the current device keeps resident at once for `kernel`: compute
+// units x workgroups per compute unit at the kernel's register / LDS occupancy. Cached per (device,
+// kernel); queries device attributes only (nothing is enqueued, so it is safe under stream capture).
+int resident_workgroups(const void* kernel, int block, int* out);
+
 }  // namespace oopao

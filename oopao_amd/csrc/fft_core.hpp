@@ -221,6 +221,19 @@ __device__ __forceinline__ float2 quad_bcast(float2 x, int c) {
   return make_float2(dpp_quad_bcast(x.x, c), dpp_quad_bcast(x.y, c));
 }
 
+// Raw buffer access (gfx9 family descriptor: word 3 = 0x00020000, untyped dword data). A buffer store
+// takes a per-lane byte offset in a VGPR and a wave-uniform offset in an SGPR, so a sequence of stores
+// at constant strides from one lane base needs one address register instead of a 64-bit pointer per
+// store. Accesses at or beyond `bytes` are dropped by the hardware.
+typedef __amdgpu_buffer_rsrc_t BufferRsrc;
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ BufferRsrc buffer_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ void buffer_store_c64(float2 v, BufferRsrc r, unsigned lane_off, int uniform_off) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)lane_off, uniform_off, 0);
+}
+
 // One Stockham stage (radix R, already-sorted span NS) on the strided layout.
 // When NS divides TPT the twiddle index k = j mod NS equals t mod NS for all of
 // the lane's butterflies, so R-1 twiddles are loaded once per stage.

@@ -26,6 +26,8 @@
 // Layout of the n x N intermediate (per point, per screen): tiled as
 // [N/4][n][4] complex64, so that one detector column (4 pupil-plane columns)
 // is one contiguous n*32-byte slab for the columns pass.
+#include <algorithm>
+
 #include "common.hpp"
 #include "fft_core.hpp"
 #include "pwfs_fast.hpp"
@@ -73,21 +75,30 @@ __global__ void k_field(const float* __restrict__ opd, float scale, const float*
 }
 
 // ---------------------------------------------------------------------------
-// Rows pass. One lane group (TPT lanes) per pupil row y; the row's field is
-// loaded once and reused for a chunk of tip tables tx_p; each transformed row
+// Rows pass. One lane group (TPT lanes) per pupil row y; each transformed row
 // is written once per point of the table's group, times that point's tilt
-// phasor. With BANDS only the pupil-plane columns of the detector columns in
-// `bd` are written (a separate instantiation: the whole-frame kernel carries no
-// column tests). FWD_ONLY stops after the forward transform (focal-plane field
-// rows, no mask). As in the columns pass, 64-lane transforms at the tw_lds()
-// sizes read their twiddles from an LDS copy (48 KB per workgroup; at N = 1280
-// 84 -> 79 us for the slopes-only rows, 86 -> 77 us for the whole frame).
+// phasor. A workgroup covers RPB adjacent rows (so that its stores fill whole
+// 128-byte lines of the tiled intermediate). The items (screen b, tip table p,
+// row block), row block fastest, are dealt out with a stride of the grid size:
+// workgroup w takes items w, w + nwg, w + 2 nwg, ... The host launches the
+// resident number of workgroups times the rounds that keep each workgroup
+// within pts_per_task items, so the pass runs in rounds of equal work
+// (config 2 slopes-only: 2960 items over 512 workgroups, 5 or 6 each, where
+// fixed tasks of 6 tables left a second, nearly empty round), and at any
+// moment the running workgroups write the slabs of a few consecutive tables,
+// every row of them (a contiguous range of items per workgroup instead spread
+// the stores over all points at once and was slower).
+// With BANDS only the pupil-plane columns of the detector columns in `bd` are
+// written (a separate instantiation: the whole-frame kernel carries no column
+// tests). FWD_ONLY stops after the forward transform (focal-plane field rows,
+// no mask). As in the columns pass, 64-lane transforms at the tw_lds() sizes
+// read their twiddles from an LDS copy (48 KB per workgroup).
 // ---------------------------------------------------------------------------
 template <int N, int TPT, bool FWD_ONLY = false, bool BANDS = false>
 __global__ void __launch_bounds__(64 * kWaves)
 k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, const int* __restrict__ pstart,
            const float2* __restrict__ ty, const float2* __restrict__ mx, const float2* __restrict__ tw,
-           float2* __restrict__ ws, int B, int P, int T, int pts_per_task, Bands bd) {
+           float2* __restrict__ ws, int B, int P, int T, long items, long nwg, Bands bd) {
   constexpr int n = N / 4;
   constexpr int E = N / TPT;
   constexpr int G = 64 / TPT;
@@ -98,15 +109,6 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
   const int h = lane / TPT;
   const int t = lane % TPT;
   constexpr int nrb = (n + RPB - 1) / RPB;
-  const int npc = (P + pts_per_task - 1) / pts_per_task;
-  int bid = blockIdx.x;
-  const int rb = bid % nrb;
-  bid /= nrb;
-  const int pc = bid % npc;
-  const int b = bid / npc;
-  const int y_raw = rb * RPB + wave * G + h;
-  const bool active = y_raw < n;
-  const int y = active ? y_raw : n - 1;
   float2* xl = lds[wave * G + h];
   constexpr bool kTwLds = (TPT == 64) && tw_lds(N);
   constexpr int kTw = kTwLds ? sep_tw_len<N, TPT>() : 1;
@@ -117,30 +119,61 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
   }
   const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;
 
-  float2 u[E];
-  const float2* fp = field + ((size_t)b * n + y) * n;
-#pragma unroll
-  for (int m = 0; m < E; ++m) u[m] = fp[(t + TPT * m) >> 2];
-  const int i0 = pc * pts_per_task;
-  const int i1 = min(P, i0 + pts_per_task);
-  for (int i = i0; i < i1; ++i) {
+  for (long it = blockIdx.x; it < items; it += nwg) {
+    // item = (b * P + i) * nrb + rb (uniform over the workgroup)
+    const long bi = it / nrb;
+    const int rb = (int)(it - bi * nrb);
+    const int b = (int)(bi / P);
+    const int i = (int)(bi - (long)b * P);
+    const int y_raw = rb * RPB + wave * G + h;
+    const bool active = y_raw < n;
+    const int y = active ? y_raw : n - 1;
+    // the row's field is re-read per table (an L2 hit after the first) rather than held in registers
+    const float2* fp = launder(field) + ((size_t)b * n + y) * n;
     const float2* txi = launder(tx) + (size_t)i * n;
+    // All of an item's global loads are issued before the first of them is used (the scheduling
+    // barriers keep the products below the loads): left to itself the compiler issued one load pair at
+    // a time, each followed by a wait for it (and, the counter being shared, for every store in
+    // flight), 40 dependent round trips per item.
     float2 v[E];
+    {
+      float2 a[E], c[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cmul(u[m], txi[(t + TPT * m) >> 2]);
+      for (int m = 0; m < E; ++m) a[m] = fp[(t + TPT * m) >> 2];
+#pragma unroll
+      for (int m = 0; m < E; ++m) c[m] = txi[(t + TPT * m) >> 2];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cmul(a[m], c[m]);
+    }
     fft_fwd_pruned<N, TPT, 4, true>(v, xl, twp, t);
     if constexpr (!FWD_ONLY) {
       const float2* mxp = launder(mx);
+      float2 c[E];
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = cmul(v[m], mxp[t + TPT * m]);
+      for (int m = 0; m < E; ++m) c[m] = mxp[t + TPT * m];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cmul(v[m], c[m]);
       fft_full<N, TPT, +1, true>(v, xl, twp, t);
     }
     if (active) {
       const int k0 = min(max(pstart[i], 0), T);
       const int k1 = min(max(pstart[i + 1], k0), T);
+      // element (X, y) of a point's [N/4][n][4] slab, X = t + TPT m: byte offset lane_off + m * kStride
+      // (TPT is a multiple of 4, so X & 3 = t & 3 and X >> 2 = t / 4 + m TPT / 4). Buffer stores take
+      // the lane part in one VGPR and the per-m part as a uniform offset, instead of a 64-bit address
+      // per element.
+      const unsigned lane_off = (unsigned)((((t >> 2) * n + y) * 4 + (t & 3)) * (int)sizeof(float2));
+      constexpr int kStride = (TPT / 4) * n * 4 * (int)sizeof(float2);
+      constexpr unsigned kSlab = (unsigned)N * n * sizeof(float2);
       for (int k = k0; k < k1; ++k) {
         const float2 tyv = ty[(size_t)k * n + y];
-        float2* dst = ws + ((size_t)b * T + k) * (size_t)(N / 4) * n * 4;
+        // wait for the tilt value here, once: with the band tests each store sits in its own branch, and
+        // a first use of tyv inside those branches put a vmcnt(0) wait (which on CDNA also waits for every
+        // store in flight) at the head of each one, serialising the stores
+        asm volatile("" ::"v"(tyv.x), "v"(tyv.y));
+        const BufferRsrc dst = buffer_rsrc(ws + ((size_t)b * T + k) * (size_t)(N / 4) * n * 4, kSlab);
         // band limits through an opaque zero: the column tests stay here instead of being hoisted out
         // of both loops as E lane masks (which cost a wave per SIMD of occupancy)
         int z = 0;
@@ -149,7 +182,7 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
 #pragma unroll
         for (int m = 0; m < E; ++m) {
           const int X = t + TPT * m;
-          if (!BANDS || in_bands(X >> 2, bk)) dst[((size_t)(X >> 2) * n + y) * 4 + (X & 3)] = cmul(v[m], tyv);
+          if (!BANDS || in_bands(X >> 2, bk)) buffer_store_c64(cmul(v[m], tyv), dst, lane_off, m * kStride);
         }
       }
     }
@@ -534,22 +567,30 @@ static int launch_rows_plan(const float* opd, float scale, const float* amp, con
   if (rc) return rc;
   constexpr int RPB = kWaves * G;
   const int nrb = (n + RPB - 1) / RPB;
-  const int npc = (P + pts_per_task - 1) / pts_per_task;
   float2* field = inter + (size_t)B * T * (size_t)N * n;
+  const bool all = bd.a0 == 0 && bd.a1 == n && bd.b0 == n && bd.b1 == n;
+  const void* kern = (FWD_ONLY || all) ? reinterpret_cast<const void*>(&k_rows_sep<N, TPT, FWD_ONLY, false>)
+                                       : reinterpret_cast<const void*>(&k_rows_sep<N, TPT, false, true>);
+  // workgroups: the resident count, times the rounds that keep every range within pts_per_task tables
+  int slots = 0;
+  rc = resident_workgroups(kern, 64 * kWaves, &slots);
+  if (rc) return rc;
+  const long items = (long)B * nrb * P;
+  const long per_round = (long)slots * pts_per_task;
+  const long nwg = std::min(items, (long)slots * ((items + per_round - 1) / per_round));
   {
     const size_t total = (size_t)B * n * n;
     hipLaunchKernelGGL(k_field, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, opd, scale, amp, field,
                        n, B);
     OOPAO_LAUNCH_CHECK();
   }
-  dim3 grid((unsigned)(nrb * npc * B));
-  const bool all = bd.a0 == 0 && bd.a1 == n && bd.b0 == n && bd.b1 == n;
+  dim3 grid((unsigned)nwg);
   if (FWD_ONLY || all)
     hipLaunchKernelGGL((k_rows_sep<N, TPT, FWD_ONLY, false>), grid, dim3(64 * kWaves), 0, st, field, tx, pstart, ty,
-                       mx, tw, inter, B, P, T, pts_per_task, bd);
+                       mx, tw, inter, B, P, T, items, nwg, bd);
   else
     hipLaunchKernelGGL((k_rows_sep<N, TPT, false, true>), grid, dim3(64 * kWaves), 0, st, field, tx, pstart, ty, mx,
-                       tw, inter, B, P, T, pts_per_task, bd);
+                       tw, inter, B, P, T, items, nwg, bd);
   OOPAO_LAUNCH_CHECK();
   return OK;
 }

@@ -287,11 +287,15 @@ class PWFSEngine:
         self.lanes_rows, self.lanes_cols = int(lanes_rows), int(lanes_cols)
 
     # -------------------------------------------------------------- execution
-    def _knobs(self, B):
-        """Chunk sizes: 6 points per columns-pass chunk and 6 tip tables per rows-pass task (measured
-        best at n=320, N=1280, T=72; any value gives the same frames up to float summation order)."""
-        nchunk = self.nchunk_user or max(1, -(-self.T // 6))
-        ppt = self.pts_per_task_user or max(1, min(self.P, 6))
+    def _knobs(self, B, whole=False):
+        """Chunk sizes: about 9 points per columns-pass chunk (config 2: 8 chunks, 186 us slopes-only
+        against 195 us for 12; config 1 at 64 screens: 6 chunks, 466 against 471 us), and a cap of 12 tip
+        tables per rows-pass workgroup, i.e. one round of the resident workgroups (config 2 slopes-only:
+        74 us, 504 us for 8 screens). Whole frames of several screens write twice the intermediate and run
+        best with short tasks (a cap of 2: 720 against 1118 us for 8 config-2 screens). Any value gives the
+        same frames up to the float summation order over points."""
+        nchunk = self.nchunk_user or max(1, -(-self.T // 9))
+        ppt = self.pts_per_task_user or max(1, min(self.P, 2 if (whole and B > 1) else 12))
         return min(nchunk, self.T), ppt
 
     def _check(self, opd, amp):
@@ -346,7 +350,7 @@ class PWFSEngine:
         if self.mode == 'generic' or raw is not None:
             return self._frames_generic(opd, scale, amp, out, raw, self._tx(weighted))
         ty = self._ty_sorted(weighted)
-        nchunk, ppt = self._knobs(B)
+        nchunk, ppt = self._knobs(B, whole=bands is None)
         nb = ctypes.c_size_t(0)
         _lib.call('oopao_pwfs_workspace_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
         ws = self._scratch(nb.value)
@@ -394,7 +398,7 @@ class PWFSEngine:
             raw = torch.empty((B, self.N, self.N), dtype=torch.float32, device=self.device)
         elif raw.dtype != torch.float32 or tuple(raw.shape) != (B, self.N, self.N) or not raw.is_contiguous():
             raise OopaoError('raw must be a contiguous float32 [B, N, N] tensor')
-        nchunk, ppt = self._knobs(B)
+        nchunk, ppt = self._knobs(B, whole=True)
         nb = ctypes.c_size_t(0)
         _lib.call('oopao_pwfs_raw_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
         ws = self._scratch(nb.value)
@@ -425,7 +429,7 @@ class PWFSEngine:
         out = torch.empty((B, crop, crop), dtype=torch.float32, device=self.device)
         nb = ctypes.c_size_t(0)
         if self.fast_geometry or self.fractional:
-            nchunk, ppt = self._knobs(B)
+            nchunk, ppt = self._knobs(B, whole=True)
             _lib.call('oopao_pwfs_focal_bytes', self.n, self.N, self.T, B, nchunk, crop, ctypes.byref(nb))
             ws = self._scratch(nb.value)
             _lib.call('oopao_pwfs_focal', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
@@ -454,7 +458,9 @@ class PWFSEngine:
             gframe = gframe.to(self.device, torch.float32).contiguous()
         if self.mode == 'generic':
             return self._adjoint_generic(opd, scale, amp, gframe, weighted)
-        nchunk, ppt = self._knobs(B)
+        # the backward kernels were tuned with 6 points per chunk
+        nchunk = min(self.T, self.nchunk_user or max(1, -(-self.T // 6)))
+        _, ppt = self._knobs(B, whole=True)
         nb = ctypes.c_size_t(0)
         _lib.call('oopao_pwfs_adjoint_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
         ws = self._scratch(nb.value)
@@ -494,8 +500,9 @@ class PWFSEngine:
 
     def rows(self, opd, scale, amp, buf, bands=None):
         self._check_buffers(buf)
+        ppt = buf['ppt'] if bands is not None else self._knobs(buf['B'], whole=True)[1]
         _lib.call('oopao_pwfs_rows', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
-                  _ptr(self.pstart), _ptr(self.ty_s), _ptr(self.mx), self.n, self.N, self.T, buf['B'], buf['ppt'],
+                  _ptr(self.pstart), _ptr(self.ty_s), _ptr(self.mx), self.n, self.N, self.T, buf['B'], ppt,
                   self.lanes_rows, column_bands(bands, self.cam), _ptr(buf['inter']),
                   ctypes.c_size_t(buf['inter'].numel()), _stream_ptr(self.device))
 
