@@ -237,8 +237,12 @@ __device__ __forceinline__ void buffer_store_c64(float2 v, BufferRsrc r, unsigne
 // One Stockham stage (radix R, already-sorted span NS) on the strided layout.
 // When NS divides TPT the twiddle index k = j mod NS equals t mod NS for all of
 // the lane's butterflies, so R-1 twiddles are loaded once per stage.
+// The last stage (NS R = N) needs no exchange: butterfly j < N/R writes its
+// outputs to j + r N/R, the positions its inputs came from, and in the strided
+// layout those are the same lane's registers v[b + r E/R].
 template <int N, int TPT, int R, int NS, int DIR, bool SWZ = false>
 __device__ __forceinline__ void stockham_stage(float2* v, float2* lds, const float2* __restrict__ tw, int t) {
+  constexpr bool kLast = (NS * R == N);
   constexpr int E = N / TPT;
   constexpr int NB = E / R;
   constexpr bool kUniform = (TPT % NS) == 0;
@@ -285,7 +289,10 @@ __device__ __forceinline__ void stockham_stage(float2* v, float2* lds, const flo
       }
     }
     Dft<R, DIR>::run(x);
-    if constexpr (kSwz) {
+    if constexpr (kLast) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b + r * NB] = x[r];
+    } else if constexpr (kSwz) {
       // swizzled slot of butterfly b = slot of butterfly 0 + a multiple of 64
       const int off = ((64 * b) / NS) * (NS * R) + (64 * b) % NS;   // folds to a constant
 #pragma unroll
@@ -296,6 +303,7 @@ __device__ __forceinline__ void stockham_stage(float2* v, float2* lds, const flo
       for (int r = 0; r < R; ++r) lds[base + r * NS] = x[r];
     }
   }
+  if constexpr (kLast) return;
   wave_lds_sync();
   if constexpr (kSwz) {
     const int q = lds_swz(t);
