@@ -139,14 +139,43 @@ struct Dft<8, DIR> {
   }
 };
 
+// 20-point DFT as a prime-factor (Good-Thomas) 4 x 5 transform: input
+// n = (5 n1 + 4 n2) mod 20, output k = (5 k1 + 16 k2) mod 20 (k1 = k mod 4,
+// k2 = k mod 5), so X = DFT_4 over n1 of DFT_5 over n2 with no twiddles
+// between the two (every index is a compile-time register name).
+template <int DIR>
+struct Dft<20, DIR> {
+  __device__ __forceinline__ static void run(float2* a) {
+    float2 y[4][5];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) {
+#pragma unroll
+      for (int n2 = 0; n2 < 5; ++n2) y[n1][n2] = a[(5 * n1 + 4 * n2) % 20];
+      Dft<5, DIR>::run(y[n1]);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 5; ++k2) {
+      float2 z[4] = {y[0][k2], y[1][k2], y[2][k2], y[3][k2]};
+      Dft<4, DIR>::run(z);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) a[(5 * k1 + 16 * k2) % 20] = z[k1];
+    }
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Compile-time radix plans. Every radix must divide E so that each lane owns
-// whole butterflies. Larger radices first: fewer LDS exchanges.
+// whole butterflies. Larger radices first: fewer LDS exchanges. When exactly 20
+// points remain and a lane holds a multiple of 20 values, the last stage is one
+// radix-20 stage (4 x 5 without internal twiddles): as the last stage it needs
+// no exchange, so the transform does one exchange fewer than with a radix-4
+// and a radix-5 stage.
 // ---------------------------------------------------------------------------
 // (Measured on gfx950 at N = 1280: plans with the odd radices first run 2x slower, because spans 5,
 // 20, 80, 320 do not divide the 64-lane count and every twiddle becomes a per-butterfly load.)
 __host__ __device__ constexpr int pick_radix(int rem, int E) {
-  return (rem % 8 == 0 && E % 8 == 0) ? 8
+  return (rem == 20 && E % 20 == 0) ? 20
+       : (rem % 8 == 0 && E % 8 == 0) ? 8
        : (rem % 4 == 0 && E % 4 == 0) ? 4
        : (rem % 5 == 0 && E % 5 == 0) ? 5
        : (rem % 3 == 0 && E % 3 == 0) ? 3
