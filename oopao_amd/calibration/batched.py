@@ -118,7 +118,7 @@ class PWFSAutotuner:
         # tile sizes at a fixed batch: modulation points per column-pass chunk, the cap of tip tables per
         # rows-pass workgroup
         chunks = sorted({max(1, math.ceil(e.T / c)) for c in (4, 6, 9, 12)})
-        tasks = sorted({max(1, min(e.P, p)) for p in (2, 6, 12)})
+        tasks = sorted({max(1, min(e.P, p)) for p in (2, 4, 8)})
         trials = []
         for nc in chunks:
             for pt in tasks:

@@ -83,7 +83,7 @@ __global__ void k_field(const float* __restrict__ opd, float scale, const float*
 // workgroup w takes items w, w + nwg, w + 2 nwg, ... The host launches the
 // resident number of workgroups times the rounds that keep each workgroup
 // within pts_per_task items, so the pass runs in rounds of equal work
-// (config 2 slopes-only: 2960 items over 512 workgroups, 5 or 6 each, where
+// (config 2 slopes-only: 2960 items over 768 workgroups, 3 or 4 each, where
 // fixed tasks of 6 tables left a second, nearly empty round), and at any
 // moment the running workgroups write the slabs of a few consecutive tables,
 // every row of them (a contiguous range of items per workgroup instead spread
@@ -164,7 +164,8 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
       // (TPT is a multiple of 4, so X & 3 = t & 3 and X >> 2 = t / 4 + m TPT / 4). Buffer stores take
       // the lane part in one VGPR and the per-m part as a uniform offset, instead of a 64-bit address
       // per element.
-      const unsigned lane_off = (unsigned)((((t >> 2) * n + y) * 4 + (t & 3)) * (int)sizeof(float2));
+      const int tq = t >> 2;
+      const unsigned lane_off = (unsigned)(((tq * n + y) * 4 + (t & 3)) * (int)sizeof(float2));
       constexpr int kStride = (TPT / 4) * n * 4 * (int)sizeof(float2);
       constexpr unsigned kSlab = (unsigned)N * n * sizeof(float2);
       for (int k = k0; k < k1; ++k) {
@@ -181,8 +182,12 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
         const Bands bk{bd.a0 + z, bd.a1 + z, bd.b0 + z, bd.b1 + z};
 #pragma unroll
         for (int m = 0; m < E; ++m) {
-          const int X = t + TPT * m;
-          if (!BANDS || in_bands(X >> 2, bk)) buffer_store_c64(cmul(v[m], tyv), dst, lane_off, m * kStride);
+          // detector column X >> 2 = t / 4 + m TPT / 4: the lane part t / 4 (one register) is compared with
+          // the band limits shifted by the uniform part (scalar arithmetic), rather than keeping E column
+          // indices in registers
+          const int cm = (TPT / 4) * m;
+          if (!BANDS || (tq >= bk.a0 - cm && tq < bk.a1 - cm) || (tq >= bk.b0 - cm && tq < bk.b1 - cm))
+            buffer_store_c64(cmul(v[m], tyv), dst, lane_off, m * kStride);
         }
       }
     }

@@ -288,14 +288,15 @@ class PWFSEngine:
 
     # -------------------------------------------------------------- execution
     def _knobs(self, B, whole=False):
-        """Chunk sizes: about 9 points per columns-pass chunk (config 2: 8 chunks, 186 us slopes-only
-        against 195 us for 12; config 1 at 64 screens: 6 chunks, 466 against 471 us), and a cap of 12 tip
-        tables per rows-pass workgroup, i.e. one round of the resident workgroups (config 2 slopes-only:
-        74 us, 504 us for 8 screens). Whole frames of several screens write twice the intermediate and run
-        best with short tasks (a cap of 2: 720 against 1118 us for 8 config-2 screens). Any value gives the
-        same frames up to the float summation order over points."""
+        """Chunk sizes: about 9 points per columns-pass chunk (config 2: 8 chunks, 134 us slopes-only
+        against 141 us for 12; config 1 at 64 screens: 6 chunks), and a cap of 4 tip tables per rows-pass
+        workgroup for slopes-only frames (config 2: one round of 768 workgroups for one screen, 57 us; 8
+        screens 403 us against 468 us with a cap of 12, whose longer-lived workgroups drift apart and
+        scatter their stores), 2 for whole frames, which write twice the intermediate (8 screens: 0.86 ms
+        against 1.0 ms with 4). Any value gives the same frames up to the float summation order over
+        points."""
         nchunk = self.nchunk_user or max(1, -(-self.T // 9))
-        ppt = self.pts_per_task_user or max(1, min(self.P, 2 if (whole and B > 1) else 12))
+        ppt = self.pts_per_task_user or max(1, min(self.P, 2 if whole else 4))
         return min(nchunk, self.T), ppt
 
     def _check(self, opd, amp):
