@@ -460,20 +460,28 @@ __global__ void k_sum_chunks(const float* __restrict__ part, float* __restrict__
   out[idx] = s;
 }
 
-// frame[b][r][g] = sum_k part[b][k][g][r]  (fixed summation order)
-// (columns outside the bands `bd` are set to zero)
-__global__ void k_reduce_chunks(const float* __restrict__ part, float* __restrict__ frame, int B,
-                                int nchunk, int cam, Bands bd) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t total = (size_t)B * cam * cam;
-  if (idx >= total) return;
-  const int r = (int)(idx % cam);
-  const int g = (int)((idx / cam) % cam);
-  const int b = (int)(idx / ((size_t)cam * cam));
-  float s = 0.f;
-  if (in_bands(g, bd))
-    for (int k = 0; k < nchunk; ++k) s += part[(((size_t)b * nchunk + k) * cam + g) * cam + r];
-  frame[((size_t)b * cam + r) * cam + g] = s;
+// frame[b][r][g] = sum_k part[b][k][g][r]  (fixed summation order), through a 32 x 32 LDS tile so that the
+// reads (along r) and the writes (along g) are both coalesced; columns outside the bands `bd` are set to zero
+__global__ void __launch_bounds__(1024) k_reduce_chunks(const float* __restrict__ part, float* __restrict__ frame,
+                                                        int B, int nchunk, int cam, Bands bd) {
+  __shared__ float tile[32][33];
+  const int nt = (cam + 31) / 32;
+  int bid = blockIdx.x;
+  const int r0 = (bid % nt) * 32;
+  bid /= nt;
+  const int g0 = (bid % nt) * 32;
+  const int b = bid / nt;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // one element per thread
+  {
+    const int g = g0 + ty, r = r0 + tx;
+    float s = 0.f;
+    if (g < cam && r < cam && in_bands(g, bd))
+      for (int k = 0; k < nchunk; ++k) s += part[(((size_t)b * nchunk + k) * cam + g) * cam + r];
+    tile[ty][tx] = s;
+  }
+  __syncthreads();
+  const int r = r0 + ty, g = g0 + tx;
+  if (g < cam && r < cam) frame[((size_t)b * cam + r) * cam + g] = tile[tx][ty];
 }
 
 // ---------------------------------------------------------------------------
@@ -633,9 +641,9 @@ static int launch_cols_plan(const float2* inter, const float2* my, int T, int B,
   }
   {
     const int cam = N / 4;
-    const size_t total = (size_t)B * cam * cam;
-    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, part, frames,
-                       B, nchunk, cam, bd);
+    const int nt = (cam + 31) / 32;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((size_t)B * nt * nt)), dim3(1024), 0, st, part, frames, B,
+                       nchunk, cam, bd);
     OOPAO_LAUNCH_CHECK();
   }
   return OK;
