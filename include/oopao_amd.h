@@ -152,12 +152,16 @@ int oopao_pwfs_focal(const float* opd, float scale, const float* amp, const void
  * Same field tables as oopao_pwfs_frames plus txs [T][n], the tip phasor of
  * each point in ty's point order (txs[k] = tx[p] for pstart[p] <= k <
  * pstart[p+1]). The modulation points are recomputed, nothing is stored
- * between forward and backward. Workspace from oopao_pwfs_adjoint_bytes. */
+ * between forward and backward. Workspace from oopao_pwfs_adjoint_bytes.
+ * bands: null, or detector-column bands {a0, a1, b0, b1} as in
+ * oopao_pwfs_frames outside which gframe is zero (any loss of the slopesMaps
+ * signal, which reads only the quadrant columns): only those columns are
+ * transformed, and gframe's other columns are ignored. */
 int oopao_pwfs_adjoint_bytes(int n, int N, int T, int B, int nchunk, size_t* bytes);
 int oopao_pwfs_adjoint(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
                        const void* ty, const void* txs, const void* mx, const void* my, const float* gframe, int n,
-                       int N, int T, int B, int nchunk, int pts_per_task, void* workspace, size_t ws_bytes,
-                       float* grad, void* stream);
+                       int N, int T, int B, int nchunk, int pts_per_task, const int* bands, void* workspace,
+                       size_t ws_bytes, float* grad, void* stream);
 
 /* ---------------------------------------------------------------------------
  * General PWFS path: any padded length N <= 2048, any N x N mask (rooftop,

@@ -50,7 +50,7 @@ SIGNATURES = {
     'oopao_pwfs_adjoint_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_adjoint': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p,
-                                   c_size_t, c_void_p, c_void_p]),
+                                   c_void_p, c_size_t, c_void_p, c_void_p]),
     'oopao_pwfs_generic_supported': (c_int, [c_int, c_int, c_int]),
     'oopao_pwfs_generic_bytes': (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
     'oopao_pwfs_generic_frames': (c_int, [c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,

@@ -34,22 +34,6 @@
 
 namespace oopao {
 
-// Detector columns a frame computes: [a0, a1) and [b0, b1), 0 <= a0 <= a1 <= b0 <= b1 <= cam. The
-// slopesMaps signal reads only the quadrant bands of the camera (Pyramid.py:838-853, 946-961: columns
-// [c-e-w, c-e) and [c+e, c+e+w)), so a slopes-only frame transforms those detector columns (4 pupil-
-// plane columns each) and no others; the full frame is the band [0, cam).
-struct Bands {
-  int a0, a1, b0, b1;
-};
-__device__ __forceinline__ bool in_bands(int g, Bands bd) {
-  return (g >= bd.a0 && g < bd.a1) || (g >= bd.b0 && g < bd.b1);
-}
-// the gi-th computed detector column
-__device__ __forceinline__ int band_col(int gi, Bands bd) {
-  const int na = bd.a1 - bd.a0;
-  return gi < na ? bd.a0 + gi : bd.b0 + (gi - na);
-}
-
 // Length of the twiddle-table prefix both passes read: the pruned forward (from span 4) and the full
 // inverse transform. At the sizes tw_lds() names, the 64-lane kernels copy it into LDS once per workgroup.
 template <int N, int TPT>
@@ -706,21 +690,6 @@ static size_t ws_raw_part_bytes(int N, int B, int nchunk) {
 
 static bool fast_supported(int n, int N, int bin) {
   return N == 4 * n && bin == 4 && fast_tpt(N) != 0;
-}
-
-// Column bands from the C-ABI argument: null = every detector column.
-static int read_bands(const int* bands, int cam, Bands* out) {
-  if (bands == nullptr) {
-    *out = Bands{0, cam, cam, cam};
-    return OK;
-  }
-  const Bands b{bands[0], bands[1], bands[2], bands[3]};
-  OOPAO_CHECK_ARG(0 <= b.a0 && b.a0 <= b.a1 && b.a1 <= b.b0 && b.b0 <= b.b1 && b.b1 <= cam &&
-                      (b.a1 - b.a0) + (b.b1 - b.b0) > 0,
-                  "invalid column bands [%d, %d) [%d, %d) for %d detector columns (need 0 <= a0 <= a1 <= b0 <= b1 "
-                  "<= cam and at least one column)", b.a0, b.a1, b.b0, b.b1, cam);
-  *out = b;
-  return OK;
 }
 
 static int check_slopes_args(const float* frames, int B, int cam, const unsigned char* valid, int w, const int* quad,

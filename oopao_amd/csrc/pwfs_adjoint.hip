@@ -15,6 +15,10 @@
 //   k_bwd_rows : per (point chunk, pupil row y): K_x^H of Z_i's row, keeping
 //                the n pupil columns, 2 Im(w conj(u)) summed over the chunk
 //   k_adj_sum  : fixed-order sum over chunks, times d(phi)/d(opd).
+// With column bands (a cotangent that is zero outside some detector columns, as for any loss of the
+// slopesMaps signal, which reads only the quadrant columns) k_bwd_cols transforms the band columns
+// only and k_bwd_rows reads the other columns of Z as zero: those E columns meet a zero g and add
+// nothing to w.
 #include "common.hpp"
 #include "fft_core.hpp"
 #include "pwfs_fast.hpp"
@@ -30,7 +34,7 @@ namespace oopao {
 template <int N, int TPT>
 __global__ void __launch_bounds__(64 * kWaves)
 k_bwd_cols(const float2* __restrict__ inter, const float2* __restrict__ my, const float* __restrict__ gframe,
-           const float2* __restrict__ tw, float2* __restrict__ Z, int B, int T) {
+           const float2* __restrict__ tw, float2* __restrict__ Z, int B, int T, Bands bd) {
   constexpr int n = N / 4;
   constexpr int E = N / TPT;
   constexpr int G = 64 / TPT;
@@ -42,8 +46,9 @@ k_bwd_cols(const float2* __restrict__ inter, const float2* __restrict__ my, cons
   const int h = lane / TPT;
   const int t = lane % TPT;
   const long task = kWaveCols ? (long)blockIdx.x : (long)blockIdx.x * kWaves + wave;
-  const int g = (int)(task % NG);
-  const long bi = task / NG;  // screen * T + point
+  const int ng = (bd.a1 - bd.a0) + (bd.b1 - bd.b0);   // computed detector columns
+  const int g = band_col((int)(task % ng), bd);
+  const long bi = task / ng;  // screen * T + point
   if (bi >= (long)B * T) return;  // uniform over the wavefront
   const int b = (int)(bi / T);
   float2* xl = lds[wave * G + h];
@@ -78,7 +83,7 @@ template <int N, int TPT>
 __global__ void __launch_bounds__(64 * kWaves)
 k_bwd_rows(const float2* __restrict__ Z, const float2* __restrict__ mx, const float2* __restrict__ field,
            const float2* __restrict__ txs, const float2* __restrict__ tys, const float2* __restrict__ tw,
-           float* __restrict__ gpart, int B, int T, int nchunk, float scale2) {
+           float* __restrict__ gpart, int B, int T, int nchunk, float scale2, Bands bd) {
   constexpr int n = N / 4;
   constexpr int E = N / TPT;
   constexpr int G = 64 / TPT;
@@ -111,8 +116,11 @@ k_bwd_rows(const float2* __restrict__ Z, const float2* __restrict__ mx, const fl
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
+      // columns outside the bands were not computed: their w contribution is zero
       const int X = t + TPT * m;
-      v[m] = zt[((size_t)(X >> 2) * n + y) * 4 + (X & 3)];
+      const int cm = (TPT / 4) * m, tq = t >> 2;   // X >> 2 = t / 4 + cm, compared on scalar-shifted limits
+      const bool in = (tq >= bd.a0 - cm && tq < bd.a1 - cm) || (tq >= bd.b0 - cm && tq < bd.b1 - cm);
+      v[m] = in ? zt[((size_t)(X >> 2) * n + y) * 4 + (X & 3)] : make_float2(0.f, 0.f);
     }
     fft_full<N, TPT, -1>(v, xl, tw, t);
 #pragma unroll
@@ -154,7 +162,7 @@ static size_t adj_inter_bytes(int n, int N, int T, int B) {
 template <int N>
 static int launch_adjoint(const float2* inter, const float2* field, const float2* mx, const float2* my,
                           const float2* txs, const float2* tys, const float* gframe, int T, int B, int nchunk,
-                          float scale2, float2* Z, float* gpart, float* grad, hipStream_t st) {
+                          float scale2, Bands bd, float2* Z, float* gpart, float* grad, hipStream_t st) {
   constexpr int TPT = fast_tpt(N);
   constexpr int n = N / 4;
   constexpr int G = 64 / TPT;
@@ -162,16 +170,16 @@ static int launch_adjoint(const float2* inter, const float2* field, const float2
   int rc = twiddles(N, &tw);
   if (rc) return rc;
   {
-    const long tasks = (long)B * T * (N / 4);
+    const long tasks = (long)B * T * ((bd.a1 - bd.a0) + (bd.b1 - bd.b0));
     dim3 grid((unsigned)(TPT == 64 ? tasks : (tasks + kWaves - 1) / kWaves));
-    hipLaunchKernelGGL((k_bwd_cols<N, TPT>), grid, dim3(64 * kWaves), 0, st, inter, my, gframe, tw, Z, B, T);
+    hipLaunchKernelGGL((k_bwd_cols<N, TPT>), grid, dim3(64 * kWaves), 0, st, inter, my, gframe, tw, Z, B, T, bd);
     OOPAO_LAUNCH_CHECK();
   }
   {
     constexpr int RPB = kWaves * G;
     const int nrb = (n + RPB - 1) / RPB;
     hipLaunchKernelGGL((k_bwd_rows<N, TPT>), dim3((unsigned)(nrb * nchunk * B)), dim3(64 * kWaves), 0, st, Z, mx,
-                       field, txs, tys, tw, gpart, B, T, nchunk, scale2);
+                       field, txs, tys, tw, gpart, B, T, nchunk, scale2, bd);
     OOPAO_LAUNCH_CHECK();
   }
   {
@@ -200,12 +208,15 @@ int oopao_pwfs_adjoint_bytes(int n, int N, int T, int B, int nchunk, size_t* byt
 
 int oopao_pwfs_adjoint(const float* opd, float scale, const float* amp, const void* tx, int P, const int* pstart,
                        const void* ty, const void* txs, const void* mx, const void* my, const float* gframe, int n,
-                       int N, int T, int B, int nchunk, int pts_per_task, void* workspace, size_t ws_bytes,
-                       float* grad, void* stream) {
+                       int N, int T, int B, int nchunk, int pts_per_task, const int* bands, void* workspace,
+                       size_t ws_bytes, float* grad, void* stream) {
   OOPAO_CHECK_ARG(opd && amp && tx && pstart && ty && txs && mx && my && gframe && workspace && grad,
                   "null pointer argument");
   size_t need = 0;
   int rc = oopao_pwfs_adjoint_bytes(n, N, T, B, nchunk, &need);
+  if (rc) return rc;
+  Bands bd;
+  rc = read_bands(bands, N / 4, &bd);
   if (rc) return rc;
   if (ws_bytes < need) {
     set_error("workspace too small: %zu < %zu bytes", ws_bytes, need);
@@ -218,7 +229,7 @@ int oopao_pwfs_adjoint(const float* opd, float scale, const float* amp, const vo
   float* gpart = reinterpret_cast<float*>(w + adj_inter_bytes(n, N, T, B) +
                                           al256a((size_t)B * T * (size_t)N * n * sizeof(float2)));
   // recompute the per-point row transforms (and the pupil field) exactly as the forward
-  rc = oopao_pwfs_rows(opd, scale, amp, tx, P, pstart, ty, mx, n, N, T, B, pts_per_task, 0, nullptr, inter,
+  rc = oopao_pwfs_rows(opd, scale, amp, tx, P, pstart, ty, mx, n, N, T, B, pts_per_task, 0, bands, inter,
                        adj_inter_bytes(n, N, T, B), stream);
   if (rc) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -229,7 +240,8 @@ int oopao_pwfs_adjoint(const float* opd, float scale, const float* amp, const vo
   switch (N) {
 #define OOPAO_CASE(NN) \
   case NN:             \
-    return launch_adjoint<NN>(inter, field, mxp, myp, txsp, tysp, gframe, T, B, nchunk, 2.0f * scale, Z, gpart, grad, st);
+    return launch_adjoint<NN>(inter, field, mxp, myp, txsp, tysp, gframe, T, B, nchunk, 2.0f * scale, bd, Z, gpart, \
+                              grad, st);
     OOPAO_FAST_SIZES(OOPAO_CASE)
 #undef OOPAO_CASE
     default:

@@ -1,6 +1,7 @@
 // Compile-time configuration of the separable fast PWFS path (N = 4 n, 4 x 4
 // detector binning), shared by the forward kernels and their adjoint.
 #pragma once
+#include "common.hpp"
 #include "fft_core.hpp"
 
 namespace oopao {
@@ -48,5 +49,37 @@ __host__ __device__ constexpr int cols_group_pad(int N, int TPT) { return (N == 
 #define OOPAO_FAST_SIZES(X) X(64) X(128) X(256) X(320) X(384) X(512) X(640) X(768) X(1024) X(1280) X(1536) X(1920) X(2048)
 
 constexpr int kWaves = 4;  // wavefronts per workgroup
+
+// Detector columns a frame computes: [a0, a1) and [b0, b1), 0 <= a0 <= a1 <= b0 <= b1 <= cam. The
+// slopesMaps signal reads only the quadrant bands of the camera (Pyramid.py:838-853, 946-961: columns
+// [c-e-w, c-e) and [c+e, c+e+w)), so a slopes-only frame transforms those detector columns (4 pupil-
+// plane columns each) and no others; the full frame is the band [0, cam).
+struct Bands {
+  int a0, a1, b0, b1;
+};
+__device__ __forceinline__ bool in_bands(int g, Bands bd) {
+  return (g >= bd.a0 && g < bd.a1) || (g >= bd.b0 && g < bd.b1);
+}
+// the gi-th computed detector column
+__device__ __forceinline__ int band_col(int gi, Bands bd) {
+  const int na = bd.a1 - bd.a0;
+  return gi < na ? bd.a0 + gi : bd.b0 + (gi - na);
+}
+
+// Column bands from the C-ABI argument: null = every detector column.
+static inline int read_bands(const int* bands, int cam, Bands* out) {
+  if (bands == nullptr) {
+    *out = Bands{0, cam, cam, cam};
+    return OK;
+  }
+  const Bands b{bands[0], bands[1], bands[2], bands[3]};
+  OOPAO_CHECK_ARG(0 <= b.a0 && b.a0 <= b.a1 && b.a1 <= b.b0 && b.b0 <= b.b1 && b.b1 <= cam &&
+                      (b.a1 - b.a0) + (b.b1 - b.b0) > 0,
+                  "invalid column bands [%d, %d) [%d, %d) for %d detector columns (need 0 <= a0 <= a1 <= b0 <= b1 "
+                  "<= cam and at least one column)", b.a0, b.a1, b.b0, b.b1, cam);
+  *out = b;
+  return OK;
+}
+
 
 }  // namespace oopao

@@ -17,26 +17,30 @@ from .tools import OopaoError
 
 
 class PWFSFrames(torch.autograd.Function):
-    """frames [B, cam, cam] = PWFS(opd [B, n, n]); phase = opd * scale."""
+    """frames [B, cam, cam] = PWFS(opd [B, n, n]); phase = opd * scale. With ``bands`` (detector-column
+    intervals, separable path) only those columns are computed, the others are zero, and the backward
+    treats the cotangent as zero there: for a downstream that reads only those columns (the slopesMaps
+    signal reads the quadrant columns) the values and gradients are those of the whole frame."""
 
     @staticmethod
-    def forward(ctx, opd, engine, amp, scale):
+    def forward(ctx, opd, engine, amp, scale, bands=None):
         opd = opd.detach().to(torch.float32).contiguous()
-        frames = engine.frames(opd, scale, amp)
+        frames = engine.frames(opd, scale, amp, bands=bands)
         ctx.save_for_backward(opd)
-        ctx.engine, ctx.amp, ctx.scale = engine, amp, scale
+        ctx.engine, ctx.amp, ctx.scale, ctx.bands = engine, amp, scale, bands
         return frames
 
     @staticmethod
     def backward(ctx, gframes):
         opd, = ctx.saved_tensors
-        grad = ctx.engine.adjoint(opd, ctx.scale, ctx.amp, gframes.detach().to(torch.float32).contiguous())
-        return grad, None, None, None
+        grad = ctx.engine.adjoint(opd, ctx.scale, ctx.amp, gframes.detach().to(torch.float32).contiguous(),
+                                  bands=ctx.bands)
+        return grad, None, None, None, None
 
 
-def pwfs_frames(engine, opd, amp, scale):
-    """Differentiable detector frames for OPD/phase maps opd [B, n, n]."""
-    return PWFSFrames.apply(opd, engine, amp, scale)
+def pwfs_frames(engine, opd, amp, scale, bands=None):
+    """Differentiable detector frames for OPD/phase maps opd [B, n, n] (``bands``: see PWFSFrames)."""
+    return PWFSFrames.apply(opd, engine, amp, scale, bands)
 
 
 def slopes_maps(frames, validI4Q, quad, w, ref2d=0.0, units=1.0):
@@ -54,14 +58,15 @@ def slopes_maps(frames, validI4Q, quad, w, ref2d=0.0, units=1.0):
     return s2d, s2d[:, vs]
 
 
-def sensor_frames(wfs, phase):
+def sensor_frames(wfs, phase, bands=None):
     """Differentiable detector frames of a drop-in Pyramid or Bi-O Edge for phase maps [B, n, n] in
     radians (before the detector's extra binning): the Pyramid's frame, or the Bi-O Edge's four mask
-    images tiled [[A, B], [C, D]] (BioEdge.py:510-544)."""
+    images tiled [[A, B], [C, D]] (BioEdge.py:510-544). ``bands`` (Pyramid on the separable path):
+    compute only those detector columns (see PWFSFrames)."""
     amp = wfs._amp()
     engines = getattr(wfs, 'engines', None)
     if engines is None:
-        return pwfs_frames(wfs.engine, phase, amp, 1.0)
+        return pwfs_frames(wfs.engine, phase, amp, 1.0, bands)
     t = [pwfs_frames(e, phase, amp, 1.0) for e in engines]
     return torch.cat((torch.cat((t[0], t[1]), dim=2), torch.cat((t[2], t[3]), dim=2)), dim=1)
 
@@ -75,7 +80,10 @@ def pwfs_signal(wfs, phase):
     if wfs.engine.fractional:
         raise OopaoError('the PWFS adjoint does not cover non-integer detector sampling')
     phase = phase.reshape(-1, wfs.engine.n, wfs.engine.n)
-    frames = sensor_frames(wfs, phase)
+    # the slopes read only the quadrant columns: the forward computes those columns and the adjoint
+    # transforms only them (a Pyramid on the separable path; the Bi-O Edge tiles four frames)
+    bands = wfs.slope_bands() if getattr(wfs, 'engines', None) is None else None
+    frames = sensor_frames(wfs, phase, bands)
     if wfs.binning != 1:                       # the detector's extra binning (Pyramid.py:800-808)
         b, c = wfs.binning, frames.shape[-1]
         frames = frames.reshape(-1, c // b, b, c // b, b).sum(dim=(2, 4))

@@ -445,11 +445,13 @@ class PWFSEngine:
                       _ptr(out), _stream_ptr(self.device))
         return out
 
-    def adjoint(self, opd, scale, amp, gframe, weighted=False):
+    def adjoint(self, opd, scale, amp, gframe, weighted=False, bands=None):
         """dL/dopd [B, n, n] for the cotangent gframe = dL/dframes [B, cam, cam] of
         frames(opd, scale, amp, weighted=weighted): the separable path's adjoint (oopao_pwfs_adjoint) or the
         general path's (oopao_pwfs_generic_adjoint: any N, mask and integer binning); the modulation points
-        are recomputed. Non-integer detector sampling (area-integrated pixels) has no adjoint."""
+        are recomputed. ``bands`` (separable path, as in frames): detector-column intervals outside which
+        gframe is zero (a loss of the slopesMaps signal); only those columns are transformed. Non-integer
+        detector sampling (area-integrated pixels) has no adjoint."""
         if self.fractional:
             raise OopaoError('the PWFS adjoint does not cover non-integer detector sampling')
         opd = self._check(opd, amp)
@@ -461,15 +463,15 @@ class PWFSEngine:
             return self._adjoint_generic(opd, scale, amp, gframe, weighted)
         # the backward kernels were tuned with 6 points per chunk
         nchunk = min(self.T, self.nchunk_user or max(1, -(-self.T // 6)))
-        _, ppt = self._knobs(B, whole=True)
+        _, ppt = self._knobs(B, whole=bands is None)
         nb = ctypes.c_size_t(0)
         _lib.call('oopao_pwfs_adjoint_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
         ws = self._scratch(nb.value)
         grad = torch.empty((B, self.n, self.n), dtype=torch.float32, device=self.device)
         _lib.call('oopao_pwfs_adjoint', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
                   _ptr(self.pstart), _ptr(self._ty_sorted(weighted)), _ptr(self.tx_s), _ptr(self.mx), _ptr(self.my), _ptr(gframe),
-                  self.n, self.N, self.T, B, nchunk, ppt, _ptr(ws), ctypes.c_size_t(nb.value), _ptr(grad),
-                  _stream_ptr(self.device))
+                  self.n, self.N, self.T, B, nchunk, ppt, column_bands(bands, self.cam), _ptr(ws),
+                  ctypes.c_size_t(nb.value), _ptr(grad), _stream_ptr(self.device))
         return grad
 
     def _adjoint_generic(self, opd, scale, amp, gframe, weighted):

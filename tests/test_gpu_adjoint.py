@@ -96,3 +96,32 @@ def test_slopes_loss_gradient_config1():
             an = float((phase.grad * d).sum().item())
             print('FD', fd, 'adjoint', an)
             assert abs(fd - an) <= 1e-2 * abs(an)
+
+
+@pytest.mark.parametrize('builder', [small_system, config1_system])
+def test_banded_adjoint_equals_full(builder):
+    """A cotangent that is zero outside two detector-column bands (as for any loss of the slopesMaps
+    signal): the banded adjoint (only those columns transformed) gives the full adjoint's gradient bit
+    for bit, and the banded forward gives the full frame on those columns bit for bit."""
+    import torch
+    o = builder()
+    eng, amp = _engine_for(o)
+    cam = o.geom.cam
+    bands = [(cam // 8, 3 * cam // 8), (5 * cam // 8, 7 * cam // 8)]
+    rs = np.random.RandomState(8)
+    ph = np.stack([random_phase(o.geom.n, s, 1.0, o.pupil) for s in (21, 22, 23)])
+    G = rs.randn(3, cam, cam)
+    cols = np.zeros(cam, bool)
+    for a, b in bands:
+        cols[a:b] = True
+    G[:, :, ~cols] = 0.0
+    t = torch.from_numpy(ph.astype(np.float32)).cuda().contiguous()
+    g = torch.from_numpy(G.astype(np.float32)).cuda()
+    full = eng.adjoint(t, 1.0, amp, g)
+    banded = eng.adjoint(t, 1.0, amp, g, bands=bands)
+    print('banded vs full adjoint: max |diff|', float((banded - full).abs().max()))
+    assert torch.equal(banded, full)
+    f_full = eng.frames(t, 1.0, amp)
+    f_band = eng.frames(t, 1.0, amp, bands=bands)
+    assert torch.equal(f_band[:, :, torch.from_numpy(cols).cuda()], f_full[:, :, torch.from_numpy(cols).cuda()])
+    assert float(f_band[:, :, torch.from_numpy(~cols).cuda()].abs().max()) == 0.0
