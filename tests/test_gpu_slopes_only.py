@@ -125,3 +125,30 @@ def test_band_and_buffer_argument_checks():
     buf['T'] = eng.T + 1
     with pytest.raises(OopaoError):
         eng.cols(buf, torch.empty((1, eng.cam, eng.cam), device=wfs.device))
+
+
+@pytest.mark.parametrize('n,nSubap,T', [(80, 20, 48), (320, 80, 72)])
+def test_rows_pass_work_split_does_not_change_frames(n, nSubap, T):
+    """The rows pass deals its (screen, tip table, row block) items to the resident workgroups with a
+    stride, in as many rounds as the cap of tables per workgroup needs: every cap (one table per
+    workgroup and many rounds, up to all tables in one round), slopes-only or whole, gives the same
+    frames bit for bit (the intermediate does not depend on the split)."""
+    tel, ngs, wfs = _pyramid(n, nSubap, T)
+    eng = wfs.engine
+    bands = wfs.slope_bands()
+    B = 3
+    ph = np.stack([random_phase(n, 60 + k, 1.5, tel.pupil) for k in range(B)]) * ngs.wavelength / (2 * np.pi)
+    opd = torch.from_numpy(ph.astype(np.float32)).to(wfs.device).contiguous()
+    amp = wfs._amp()
+    scale = 2 * np.pi / ngs.wavelength
+    for bd in (bands, None):
+        ref = None
+        for ppt in (1, 3, eng.P):
+            eng.pts_per_task_user = ppt
+            f = eng.frames(opd, scale, amp, bands=bd).clone()
+            if ref is None:
+                ref = f
+            print('n=%d bands=%s ppt=%d: max |diff| %.3g' % (n, bd is not None, ppt, float((f - ref).abs().max())),
+                  flush=True)
+            assert torch.equal(f, ref)
+    eng.pts_per_task_user = None
