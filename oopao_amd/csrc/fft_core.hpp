@@ -139,42 +139,66 @@ struct Dft<8, DIR> {
   }
 };
 
-// 20-point DFT as a prime-factor (Good-Thomas) 4 x 5 transform: input
-// n = (5 n1 + 4 n2) mod 20, output k = (5 k1 + 16 k2) mod 20 (k1 = k mod 4,
-// k2 = k mod 5), so X = DFT_4 over n1 of DFT_5 over n2 with no twiddles
-// between the two (every index is a compile-time register name).
-template <int DIR>
-struct Dft<20, DIR> {
+// Prime-factor (Good-Thomas) DFT of R = R1 R2 points with gcd(R1, R2) = 1: input n = (R2 n1 + R1 n2) mod R,
+// output k = (R2 u1 k1 + R1 u2 k2) mod R with u1 = R2^-1 mod R1, u2 = R1^-1 mod R2 (so k1 = k mod R1,
+// k2 = k mod R2), hence X = DFT_R1 over n1 of DFT_R2 over n2 with no twiddles between the two (every
+// index is a compile-time register name). 20 = 4 x 5: maps 5 n1 + 4 n2 and 5 k1 + 16 k2.
+__host__ __device__ constexpr int mod_inverse(int a, int m) {
+  int x = 1;
+  while (x < m && (a * x) % m != 1) ++x;
+  return x;
+}
+template <int R1, int R2, int DIR>
+struct Pfa {
   __device__ __forceinline__ static void run(float2* a) {
-    float2 y[4][5];
+    constexpr int R = R1 * R2;
+    constexpr int u1 = mod_inverse(R2 % R1, R1), u2 = mod_inverse(R1 % R2, R2);
+    float2 y[R1][R2];
 #pragma unroll
-    for (int n1 = 0; n1 < 4; ++n1) {
+    for (int n1 = 0; n1 < R1; ++n1) {
 #pragma unroll
-      for (int n2 = 0; n2 < 5; ++n2) y[n1][n2] = a[(5 * n1 + 4 * n2) % 20];
-      Dft<5, DIR>::run(y[n1]);
+      for (int n2 = 0; n2 < R2; ++n2) y[n1][n2] = a[(R2 * n1 + R1 * n2) % R];
+      Dft<R2, DIR>::run(y[n1]);
     }
 #pragma unroll
-    for (int k2 = 0; k2 < 5; ++k2) {
-      float2 z[4] = {y[0][k2], y[1][k2], y[2][k2], y[3][k2]};
-      Dft<4, DIR>::run(z);
+    for (int k2 = 0; k2 < R2; ++k2) {
+      float2 z[R1];
 #pragma unroll
-      for (int k1 = 0; k1 < 4; ++k1) a[(5 * k1 + 16 * k2) % 20] = z[k1];
+      for (int n1 = 0; n1 < R1; ++n1) z[n1] = y[n1][k2];
+      Dft<R1, DIR>::run(z);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) a[(R2 * u1 * k1 + R1 * u2 * k2) % R] = z[k1];
     }
   }
 };
+template <int DIR>
+struct Dft<20, DIR> : Pfa<4, 5, DIR> {};
 
 // ---------------------------------------------------------------------------
 // Compile-time radix plans. Every radix must divide E so that each lane owns
-// whole butterflies. Larger radices first: fewer LDS exchanges. When exactly 20
-// points remain and a lane holds a multiple of 20 values, the last stage is one
-// radix-20 stage (4 x 5 without internal twiddles): as the last stage it needs
-// no exchange, so the transform does one exchange fewer than with a radix-4
-// and a radix-5 stage.
+// whole butterflies. Larger radices first: fewer LDS exchanges. When a lane holds
+// a multiple of 20 values and the length is 20 times a product of radices
+// 8 / 4 / 2, those come first and the last stage is one radix-20 prime-factor
+// stage held in one lane: as the last stage it needs no exchange, and it has no
+// internal twiddles (N = 1280 with 20 values per lane: 4, 4, 4, 20 instead of
+// 4, 4, 4, 4, 5; N = 640 with 20 / 40: 4, 4, 2, 20 / 8, 4, 20 instead of
+// 4, 4, 4, 5, 2 / 8, 8, 5, 2). A radix-60 last stage for N = 1920 (60 values per
+// lane) was measured: 60 values and their twiddles in one lane spilled to scratch.
 // ---------------------------------------------------------------------------
 // (Measured on gfx950 at N = 1280: plans with the odd radices first run 2x slower, because spans 5,
 // 20, 80, 320 do not divide the 64-lane count and every twiddle becomes a per-butterfly load.)
+__host__ __device__ constexpr int pow2_radix(int x, int E) {
+  return (x % 8 == 0 && E % 8 == 0) ? 8 : (x % 4 == 0 && E % 4 == 0) ? 4 : (x % 2 == 0 && E % 2 == 0) ? 2 : 0;
+}
+__host__ __device__ constexpr bool pow2_plan(int x, int E) {
+  return x == 1 ? true : (pow2_radix(x, E) == 0 ? false : pow2_plan(x / pow2_radix(x, E), E));
+}
+// the prime-factor radix the plan of rem can end with (0: none)
+__host__ __device__ constexpr int pfa_last(int rem, int E) {
+  return (E % 20 == 0 && rem % 20 == 0 && pow2_plan(rem / 20, E)) ? 20 : 0;
+}
 __host__ __device__ constexpr int pick_radix(int rem, int E) {
-  return (rem == 20 && E % 20 == 0) ? 20
+  return pfa_last(rem, E) != 0 ? (rem == pfa_last(rem, E) ? rem : pow2_radix(rem / pfa_last(rem, E), E))
        : (rem % 8 == 0 && E % 8 == 0) ? 8
        : (rem % 4 == 0 && E % 4 == 0) ? 4
        : (rem % 5 == 0 && E % 5 == 0) ? 5
@@ -287,8 +311,10 @@ __device__ __forceinline__ void stockham_stage(float2* v, float2* lds, const flo
 #pragma unroll
     for (int r = 0; r < R; ++r) p0[r] = lds_swz(base0 + r * NS);
   }
+  // a stage with one butterfly per lane loads each twiddle where it is used (no array of R - 1 held)
+  constexpr bool kPreload = kUniform && NB > 1;
   float2 wu[R];
-  if (kUniform && NS > 1) {
+  if (kPreload && NS > 1) {
     const int k = t % NS;
 #pragma unroll
     for (int r = 1; r < R; ++r) {
@@ -308,7 +334,7 @@ __device__ __forceinline__ void stockham_stage(float2* v, float2* lds, const flo
 #pragma unroll
       for (int r = 1; r < R; ++r) {
         float2 w;
-        if (kUniform) {
+        if (kPreload) {
           w = wu[r];
         } else {
           w = twp[r * k * TS];
