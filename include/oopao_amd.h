@@ -31,9 +31,11 @@ extern "C" {
 const char* oopao_last_error(void);
 
 /* Build the device twiddle table exp(-2 pi i m / N) of transform length N on the
- * current device (once per device and N; kept until the process ends). The
- * transform entry points build it on their first call for a new N with a
- * synchronous allocation and copy, which a stream under graph capture does not
+ * current device (once per device and N; kept until the process ends), and for
+ * the separable PWFS lengths query the occupancy of the rows-pass kernels (their
+ * grid is the resident number of workgroups; the values are cached). The
+ * transform entry points do both on their first call for a new N, the table with
+ * a synchronous allocation and copy, which a stream under graph capture does not
  * allow: call this first for every N a captured call will use. */
 int oopao_prepare_tables(int N);
 

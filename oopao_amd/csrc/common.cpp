@@ -68,9 +68,3 @@ int resident_workgroups(const void* kernel, int block, int* out) {
 }  // namespace oopao
 
 extern "C" const char* oopao_last_error(void) { return oopao::get_error(); }
-
-extern "C" int oopao_prepare_tables(int N) {
-  OOPAO_CHECK_ARG(N > 0 && N <= 4096, "invalid transform length N=%d", N);
-  const float2* tw = nullptr;
-  return oopao::twiddles(N, &tw);
-}

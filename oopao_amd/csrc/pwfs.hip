@@ -744,11 +744,47 @@ static int slopes_scratch(double** out) {
   return OK;
 }
 
+// Occupancy of every rows-pass kernel of length N (the grid of a rows launch is the resident number of
+// workgroups): queried once here so that a first launch under stream capture finds the values cached.
+template <int N, int TPT, bool FWD_ONLY>
+static int prime_rows_plan() {
+  int slots = 0;
+  int rc = resident_workgroups(reinterpret_cast<const void*>(&k_rows_sep<N, TPT, FWD_ONLY, false>), 64 * kWaves,
+                               &slots);
+  if (rc || FWD_ONLY) return rc;
+  return resident_workgroups(reinterpret_cast<const void*>(&k_rows_sep<N, TPT, false, true>), 64 * kWaves, &slots);
+}
+template <int N>
+static int prime_rows() {
+  int rc = prime_rows_plan<N, fast_tpt_rows(N), false>();
+  if (!rc) rc = prime_rows_plan<N, fast_tpt_rows(N), true>();
+  if constexpr (alt_tpt_rows(N) > 0) {
+    if (!rc) rc = prime_rows_plan<N, alt_tpt_rows(N), false>();
+  }
+  return rc;
+}
+
 }  // namespace oopao
 
 using namespace oopao;
 
 extern "C" {
+
+int oopao_prepare_tables(int N) {
+  OOPAO_CHECK_ARG(N > 0 && N <= 4096, "invalid transform length N=%d", N);
+  const float2* tw = nullptr;
+  int rc = twiddles(N, &tw);
+  if (rc) return rc;
+  switch (N) {
+#define OOPAO_CASE(NN) \
+  case NN:             \
+    return prime_rows<NN>();
+    OOPAO_FAST_SIZES(OOPAO_CASE)
+#undef OOPAO_CASE
+    default:
+      return OK;
+  }
+}
 
 int oopao_pwfs_workspace_bytes(int n, int N, int T, int B, int nchunk, size_t* bytes) {
   OOPAO_CHECK_ARG(bytes != nullptr, "bytes is null");
