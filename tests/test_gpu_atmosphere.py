@@ -62,6 +62,11 @@ def test_update_matches_oracle(wind, direction, Ts):
             L.update()
         err = rel_l2(atm.OPD, atmosphere_opd(layers, atm.fractionalR0, 32))
         worst = max(worst, err)
+        # the clip range the warp uses: the extruded map's exact min / max (folded into the extrusion launch)
+        for i in range(atm.nLayer):
+            lay = getattr(atm, 'layer_%d' % (i + 1))
+            mm = lay.mm.cpu().numpy()
+            assert (mm[0], mm[1]) == (float(lay.map.min()), float(lay.map.max())), (step, i, mm[:2])
     print('worst rel_l2 over %d steps' % steps, worst)
     assert worst < 1e-6
 
