@@ -173,6 +173,14 @@ class PWFSEngine:
         # lanes per transform of the rows / columns pass (0 = the default plan; see plans())
         self.lanes_rows = 0
         self.lanes_cols = 0
+        if self.mode == 'fast':
+            rows, cols = self.plans()
+            self._default_lanes = (rows[0], cols[0])
+        else:
+            self._default_lanes = (0, 0)
+        # wavefronts the chip holds at the separable kernels' occupancy (three per SIMD, four SIMDs per CU)
+        self._wave_slots = 12 * torch.cuda.get_device_properties(self.device).multi_processor_count \
+            if self.device.type == 'cuda' else 3072
         self.workspace = Workspace(self.device) if workspace is None else workspace
         self.T = 0
         self.maskT = None
@@ -287,17 +295,46 @@ class PWFSEngine:
         self.lanes_rows, self.lanes_cols = int(lanes_rows), int(lanes_cols)
 
     # -------------------------------------------------------------- execution
-    def _knobs(self, B, whole=False):
-        """Chunk sizes: about 9 points per columns-pass chunk (config 2: 8 chunks, 134 us slopes-only
-        against 141 us for 12; config 1 at 64 screens: 6 chunks), and a cap of 4 tip tables per rows-pass
-        workgroup for slopes-only frames (config 2: one round of 768 workgroups for one screen, 57 us; 8
-        screens 403 us against 468 us with a cap of 12, whose longer-lived workgroups drift apart and
-        scatter their stores), 2 for whole frames, which write twice the intermediate (8 screens: 0.86 ms
-        against 1.0 ms with 4). Any value gives the same frames up to the float summation order over
-        points."""
-        nchunk = self.nchunk_user or max(1, -(-self.T // 9))
-        ppt = self.pts_per_task_user or max(1, min(self.P, 2 if whole else 4))
-        return min(nchunk, self.T), ppt
+    def _knobs(self, B, whole=False, fill=False):
+        """Tuning defaults of the separable passes for B screens. Any value gives the same frames up to
+        the float summation order over points; the chunk count does not depend on the column bands, so
+        that a slopes-only frame keeps the whole frame's band columns bit for bit.
+
+        Point chunks of the columns pass: about 9 points per chunk (config 2, one screen: 8 chunks,
+        141-146 us slopes-only; 4-24 chunks within a few percent except 4); with ``fill`` (the binned
+        columns pass), unless those chunks fill less than one round of the chip's wavefront slots: then
+        the most chunks whose wavefronts, counted over every detector column, still fit in one round
+        (config 1, one screen: 24 chunks, slopes-only 28 us against 64 us for 6 chunks, the whole frame
+        33 us against 64 us, where 48 chunks, 1.25 rounds, take 42 us).
+
+        Tip tables per rows-pass workgroup (the pass deals (table, row block) items out over the resident
+        workgroups; the cap sets how many rounds): with 64-lane transforms each 128-byte line of the
+        intermediate is completed by the four wavefronts of a workgroup, and workgroups that take several
+        items let those drift apart and split the lines' writes, so the whole frame (twice the stores of
+        a slopes-only one) takes one item per workgroup (config 2, 8 screens: 0.68 ms against 0.87 ms for
+        2, one screen 73-77 us against 77-78 us) and slopes-only frames 4 for one screen (one round of
+        768 workgroups, 60-62 us) and 2 for more (8 screens 417 us against 430 us for 4); with narrower
+        transforms a wavefront writes whole lines itself, and 2 (whole) or 4 tables amortise the
+        workgroup's start (config 1, 500 screens, whole: 1.35 ms against 1.45 ms for 1)."""
+        tpt_r = self.lanes_rows or self._default_lanes[0]
+        tpt_c = self.lanes_cols or self._default_lanes[1]
+        if self.nchunk_user:
+            nchunk = self.nchunk_user
+        else:
+            waves = lambda ppc: B * self.cam * (4 if tpt_c == 64 else 1) * -(-self.T // ppc)
+            ppc = 9
+            if fill and waves(ppc) < self._wave_slots:
+                ppc = 1
+                while ppc < 9 and waves(ppc) > self._wave_slots:
+                    ppc += 1
+            nchunk = -(-self.T // ppc)
+        if self.pts_per_task_user:
+            ppt = self.pts_per_task_user
+        elif tpt_r == 64:
+            ppt = 1 if whole else (4 if B == 1 else 2)
+        else:
+            ppt = 2 if whole else 4
+        return max(1, min(nchunk, self.T)), max(1, min(self.P, ppt))
 
     def _check(self, opd, amp):
         if self.T == 0:
@@ -321,12 +358,13 @@ class PWFSEngine:
     def bytes_per_screen(self):
         """Workspace bytes per phase screen of the current mode (for batch sizing)."""
         nb = ctypes.c_size_t(0)
+        # the chunk count of a large batch (a small one may use more chunks, for parallelism)
         if self.mode == 'fast' and self.fractional:
-            nchunk, _ = self._knobs(1)
+            nchunk, _ = self._knobs(1, whole=True)
             _lib.call('oopao_pwfs_raw_bytes', self.n, self.N, self.T, 1, nchunk, ctypes.byref(nb))
             return nb.value + 4 * self.N * self.N
         if self.mode == 'fast':
-            nchunk, _ = self._knobs(1)
+            nchunk, _ = self._knobs(1, whole=True)
             _lib.call('oopao_pwfs_workspace_bytes', self.n, self.N, self.T, 1, nchunk, ctypes.byref(nb))
         else:
             _lib.call('oopao_pwfs_generic_bytes', self.n, self.N, self.T, 1, self._generic_nchunk(1),
@@ -351,13 +389,14 @@ class PWFSEngine:
         if self.mode == 'generic' or raw is not None:
             return self._frames_generic(opd, scale, amp, out, raw, self._tx(weighted))
         ty = self._ty_sorted(weighted)
-        nchunk, ppt = self._knobs(B, whole=bands is None)
+        cb = column_bands(bands, self.cam)
+        nchunk, ppt = self._knobs(B, whole=bands is None, fill=True)
         nb = ctypes.c_size_t(0)
         _lib.call('oopao_pwfs_workspace_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(nb))
         ws = self._scratch(nb.value)
         _lib.call('oopao_pwfs_frames', _ptr(opd), ctypes.c_float(scale), _ptr(amp), _ptr(self.tx_d), self.P,
                   _ptr(self.pstart), _ptr(ty), _ptr(self.mx), _ptr(self.my), self.n, self.N, self.T, B, self.bin,
-                  nchunk, ppt, self.lanes_rows, self.lanes_cols, column_bands(bands, self.cam), _ptr(ws),
+                  nchunk, ppt, self.lanes_rows, self.lanes_cols, cb, _ptr(ws),
                   ctypes.c_size_t(nb.value), _ptr(out), _stream_ptr(self.device))
         return out
 
@@ -488,7 +527,7 @@ class PWFSEngine:
 
     # split passes (same result as frames(); used to time the passes separately)
     def buffers(self, B):
-        nchunk, ppt = self._knobs(B)
+        nchunk, ppt = self._knobs(B, fill=True)
         ib, pb = ctypes.c_size_t(0), ctypes.c_size_t(0)
         _lib.call('oopao_pwfs_intermediate_bytes', self.n, self.N, self.T, B, nchunk, ctypes.byref(ib),
                   ctypes.byref(pb))
