@@ -602,6 +602,10 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=200)
     ap.add_argument('--warmup', type=int, default=10)
+    ap.add_argument('--settle-ms', type=float, default=100.0,
+                    help='before the warm-up steps of each timed loop, repeat its step for this long (untimed) so '
+                         'that the GPU clock has left its idle state: after 10 steps (~2 ms) alone the config-2 '
+                         'frame measures ~3 %% slower than after 200')
     ap.add_argument('--cpu-frames', type=int, default=5)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--batch', type=int, default=8, help='screens per call for the batched form')
@@ -659,9 +663,18 @@ def main():
             se.slopes(frame, sig2d, sig)
         return step
 
+    def settle(step):
+        """Repeat ``step`` (untimed) for args.settle_ms of wall clock."""
+        t_end = time.perf_counter() + args.settle_ms * 1e-3
+        while time.perf_counter() < t_end:
+            for _ in range(10):
+                step()
+            torch.cuda.synchronize()
+
     def timed(step, K):
         """ms per step over K steps (bracketed by synchronisations) and the mean rows / columns
         pass durations from HIP events on the kernels' stream."""
+        settle(step)
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
@@ -718,6 +731,7 @@ def main():
         eng.rows(opdB, scale, amp, bufB, bands)
         eng.cols(bufB, frameB, bands)
         se.slopes(frameB, sig2dB, sigB)
+    settle(stepB)
     for _ in range(3):
         stepB()
     torch.cuda.synchronize()
@@ -788,6 +802,7 @@ def main():
         'n_gpus': 1,
         'steps': K,
         'warmup': args.warmup,
+        'settle_ms': args.settle_ms,
         'ms_per_step': round(ms_step, 5),
         'higher_is_better': True,
         'scaling': 'weak',
