@@ -35,14 +35,29 @@
 namespace oopao {
 
 // Length of the twiddle-table prefix both passes read: the pruned forward (from span 4) and the full
-// inverse transform. At the sizes tw_lds() names, the 64-lane kernels copy it into LDS once per workgroup.
+// inverse transform. The kernels copy it into LDS once per workgroup (every stage's twiddles are then
+// LDS reads instead of global loads with 64-bit addresses, queued behind the data loads), except for
+// the plans where the copy would lower the kernel's occupancy (waves per SIMD from the compiler's VGPR
+// and LDS counts): the columns passes at N = 640 (16 and 32 lanes) and the 32-lane one at N = 1280, the
+// columns pass at N = 768 (3 x 54.4 KB of LDS no longer fit a CU once the allocation is rounded up:
+// measured 19 % slower for whole frames), and the forward-only rows passes (focal plane) at N = 320,
+// 512, 640 and 1536. Elsewhere the copy keeps or raises the occupancy (N = 256 columns 4 -> 5 waves per
+// SIMD, N = 640 rows 2 -> 3, N = 1536 rows 1 -> 2) and lowers the spills at N = 1920 / 2048. Frames of 8
+// screens, slopes-only bands: N = 256 72 -> 56 us, 320 117 -> 86 us, 512 244 -> 188 us, 1536 3.51 ->
+// 2.58 ms, 2048 8.0 -> 7.1 ms (config 1, 500 screens: 4.89 -> 3.53 ms slopes-only, 8.72 -> 5.97 ms whole);
+// N = 1024 and 1280 (64 lanes) had the copy already.
 template <int N, int TPT>
 __host__ __device__ constexpr int sep_tw_len() {
   return (plan_tw_max(N, N / TPT, 4, N / 4) > plan_tw_max(N, N / TPT, 1, N) ? plan_tw_max(N, N / TPT, 4, N / 4)
                                                                                : plan_tw_max(N, N / TPT, 1, N)) +
          1;
 }
-__host__ __device__ constexpr bool tw_lds(int N) { return N == 1024 || N == 1280; }
+__host__ __device__ constexpr bool tw_lds_cols(int N, int TPT) {
+  return !(N == 640 || N == 768 || (N == 1280 && TPT == 32));
+}
+__host__ __device__ constexpr bool tw_lds_rows(int N, bool fwd_only) {
+  return !(fwd_only && (N == 320 || N == 512 || N == 640 || N == 1536));
+}
 
 // Pupil-plane field u[b][y][x] = amp[y][x] exp(i scale opd[b][y][x]), once per
 // screen (the reference re-forms it per modulation point, Pyramid.py:619-620).
@@ -75,8 +90,8 @@ __global__ void k_field(const float* __restrict__ opd, float scale, const float*
 // With BANDS only the pupil-plane columns of the detector columns in `bd` are
 // written (a separate instantiation: the whole-frame kernel carries no column
 // tests). FWD_ONLY stops after the forward transform (focal-plane field rows,
-// no mask). As in the columns pass, 64-lane transforms at the tw_lds() sizes
-// read their twiddles from an LDS copy (48 KB per workgroup).
+// no mask). As in the columns pass, the transforms read their twiddles from
+// an LDS copy of the table prefix (tw_lds_rows()).
 // ---------------------------------------------------------------------------
 template <int N, int TPT, bool FWD_ONLY = false, bool BANDS = false>
 __global__ void __launch_bounds__(64 * kWaves)
@@ -94,8 +109,9 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
   const int t = lane % TPT;
   constexpr int nrb = (n + RPB - 1) / RPB;
   float2* xl = lds[wave * G + h];
-  constexpr bool kTwLds = (TPT == 64) && tw_lds(N);
+  constexpr bool kTwLds = tw_lds_rows(N, FWD_ONLY);
   constexpr int kTw = kTwLds ? sep_tw_len<N, TPT>() : 1;
+  static_assert(!kTwLds || kTw <= N, "twiddle prefix within the table");
   __shared__ float2 twl[kTw];
   if constexpr (kTwLds) {
     for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
@@ -189,12 +205,14 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
 //  * narrower transforms: one wavefront per (b, chunk, g), its 64/TPT lane
 //    groups taking the columns.
 // Only the detector columns in `bd` are computed (one task per such column).
-// With 64-lane transforms at the sizes tw_lds() names, the workgroup first
-// copies the twiddle entries the plans read into LDS: every stage's twiddles
-// are then LDS reads (lgkm queue, ~100-cycle latency) instead of global loads
-// queued behind the column loads (the table loads cost ~9 % of the kernel at
-// N = 1280). The column sums reuse the exchange buffers' LDS, so the workgroup
-// needs 48 KB and three still fit per CU.
+// For the plans tw_lds_cols() names, the workgroup first copies the twiddle
+// entries the plans read into LDS: every stage's twiddles are then LDS reads
+// (lgkm queue, ~100-cycle latency) instead of global loads queued behind the
+// column loads (the table loads cost ~9 % of the kernel at N = 1280). With
+// 64-lane transforms the column sums reuse the exchange buffers' LDS, so the
+// workgroup needs 48 KB at N = 1280 and three still fit per CU; at N = 320
+// (16 lanes) the 2.5 KB prefix takes the workgroup to 44 KB, three per CU as
+// before.
 // Output: part[b][chunk][g][r] (detector column-major partial sums).
 // ---------------------------------------------------------------------------
 template <int N, int TPT>
@@ -206,7 +224,7 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   constexpr int G = 64 / TPT;
   constexpr int NG = N / 4;
   constexpr bool kWaveCols = (TPT == 64);
-  constexpr bool kTwLds = kWaveCols && tw_lds(N);
+  constexpr bool kTwLds = tw_lds_cols(N, TPT);
   constexpr int kTw = kTwLds ? sep_tw_len<N, TPT>() : 1;
   static_assert(!kTwLds || kTw <= N, "twiddle prefix within the table");
   static_assert(kWaves * NG <= 2 * kWaves * G * lds_len(N), "column sums fit in the exchange buffers");
@@ -222,14 +240,16 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   task /= ng;
   const int chunk = task % nchunk;
   const int b = task / nchunk;
-  if (b >= B) return;  // uniform over the wavefront (and over the workgroup when kWaveCols)
-  const int i0 = (int)(((long)chunk * T) / nchunk);
-  const int i1 = (int)(((long)(chunk + 1) * T) / nchunk);
-  float2* xl = lds[wave * G + h];
+  // the twiddle copy and its barrier come before the early exit, with every wavefront of the workgroup
+  // present (with narrow transforms the last workgroup's wavefronts past the tasks leave below)
   if constexpr (kTwLds) {
     for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
     __syncthreads();
   }
+  if (b >= B) return;  // uniform over the wavefront (and over the workgroup when kWaveCols)
+  const int i0 = (int)(((long)chunk * T) / nchunk);
+  const int i1 = (int)(((long)(chunk + 1) * T) / nchunk);
+  float2* xl = lds[wave * G + h];
   const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;
 
   float acc[E];
