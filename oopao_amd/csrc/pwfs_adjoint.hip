@@ -30,6 +30,17 @@ extern "C" int oopao_pwfs_rows(const float* opd, float scale, const float* amp, 
 
 namespace oopao {
 
+// Plans whose backward kernels copy the twiddle prefix into LDS (as the forward passes do): those of the
+// forward columns / rows passes, less the ones where the copy would lower the occupancy of the backward
+// kernel (columns N = 1536, 3 -> 2 waves per SIMD; N = 1920, more spills; rows N = 512, 5 -> 4). At
+// N = 320 the columns kernel drops from 4 to 3 waves per SIMD with the copy and still runs faster
+// (config 4, 64 screens: adjoint 1.82 -> 1.61 ms, 1.76 ms with the copy in the rows kernel only).
+// Adjoint of 8 screens at N = 256 / 512: 170 -> 130 us / 726 -> 613 us; 4 at N = 1024: 2.61 -> 1.87 ms.
+__host__ __device__ constexpr bool tw_lds_bwd_cols(int N, int TPT) {
+  return tw_lds_cols(N, TPT) && N != 1536 && N != 1920;
+}
+__host__ __device__ constexpr bool tw_lds_bwd_rows(int N) { return tw_lds_rows(N, false) && N != 512; }
+
 // Z layout: [B*T][N/4][n][4] complex64 (the forward intermediate's tiling).
 template <int N, int TPT>
 __global__ void __launch_bounds__(64 * kWaves)
@@ -40,7 +51,11 @@ k_bwd_cols(const float2* __restrict__ inter, const float2* __restrict__ my, cons
   constexpr int G = 64 / TPT;
   constexpr int NG = N / 4;
   constexpr bool kWaveCols = (TPT == 64);
+  constexpr bool kTwLds = tw_lds_bwd_cols(N, TPT);
+  constexpr int kTw = kTwLds ? sep_tw_len<N, TPT>() : 1;
+  static_assert(!kTwLds || kTw <= N, "twiddle prefix within the table");
   __shared__ float2 lds[kWaves * G][lds_len(N)];
+  __shared__ float2 twl[kTw];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int h = lane / TPT;
@@ -49,6 +64,13 @@ k_bwd_cols(const float2* __restrict__ inter, const float2* __restrict__ my, cons
   const int ng = (bd.a1 - bd.a0) + (bd.b1 - bd.b0);   // computed detector columns
   const int g = band_col((int)(task % ng), bd);
   const long bi = task / ng;  // screen * T + point
+  // the twiddle prefix into LDS, with every wavefront of the workgroup present (the last workgroup's
+  // wavefronts past the tasks leave below)
+  if constexpr (kTwLds) {
+    for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
+    __syncthreads();
+  }
+  const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;
   if (bi >= (long)B * T) return;  // uniform over the wavefront
   const int b = (int)(bi / T);
   float2* xl = lds[wave * G + h];
@@ -61,17 +83,17 @@ k_bwd_cols(const float2* __restrict__ inter, const float2* __restrict__ my, cons
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + c];
-    fft_fwd_pruned<N, TPT, 4, true>(v, xl, tw, t);
+    fft_fwd_pruned<N, TPT, 4, true>(v, xl, twp, t);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(v[m], launder(my)[t + TPT * m]);
-    fft_full<N, TPT, +1, true>(v, xl, tw, t);
+    fft_full<N, TPT, +1, true>(v, xl, twp, t);
     // E column at rows Y = t + TPT m; dL/dI there is the detector gradient of pixel (Y/4, X/4)
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cscale(v[m], gf[((t + TPT * m) >> 2) * NG + g]);
-    fft_full<N, TPT, -1, true>(v, xl, tw, t);
+    fft_full<N, TPT, -1, true>(v, xl, twp, t);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmulc(v[m], launder(my)[t + TPT * m]);
-    fft_full<N, TPT, +1, true>(v, xl, tw, t);
+    fft_full<N, TPT, +1, true>(v, xl, twp, t);
     // keep the pupil rows y < n (elements m < E/4)
 #pragma unroll
     for (int m = 0; m < E / 4; ++m) zt[(t + TPT * m) * 4 + c] = v[m];
@@ -90,7 +112,11 @@ k_bwd_rows(const float2* __restrict__ Z, const float2* __restrict__ mx, const fl
   constexpr int NG = N / 4;
   constexpr int RPB = kWaves * G;
   constexpr int nrb = (n + RPB - 1) / RPB;
+  constexpr bool kTwLds = tw_lds_bwd_rows(N);
+  constexpr int kTw = kTwLds ? sep_tw_len<N, TPT>() : 1;
+  static_assert(!kTwLds || kTw <= N, "twiddle prefix within the table");
   __shared__ float2 lds[kWaves * G][lds_len(N)];
+  __shared__ float2 twl[kTw];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int h = lane / TPT;
@@ -100,7 +126,12 @@ k_bwd_rows(const float2* __restrict__ Z, const float2* __restrict__ mx, const fl
   bid /= nrb;
   const int chunk = bid % nchunk;
   const int b = bid / nchunk;
-  if (b >= B) return;
+  if (b >= B) return;  // uniform over the workgroup
+  if constexpr (kTwLds) {
+    for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
+    __syncthreads();
+  }
+  const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;
   const int y_raw = rb * RPB + wave * G + h;
   const bool active = y_raw < n;
   const int y = active ? y_raw : n - 1;
@@ -122,10 +153,10 @@ k_bwd_rows(const float2* __restrict__ Z, const float2* __restrict__ mx, const fl
       const bool in = (tq >= bd.a0 - cm && tq < bd.a1 - cm) || (tq >= bd.b0 - cm && tq < bd.b1 - cm);
       v[m] = in ? zt[((size_t)(X >> 2) * n + y) * 4 + (X & 3)] : make_float2(0.f, 0.f);
     }
-    fft_full<N, TPT, -1>(v, xl, tw, t);
+    fft_full<N, TPT, -1>(v, xl, twp, t);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmulc(v[m], launder(mx)[t + TPT * m]);
-    fft_full<N, TPT, +1>(v, xl, tw, t);
+    fft_full<N, TPT, +1>(v, xl, twp, t);
     const float2 tyv = tys[(size_t)i * n + y];
     const float2* txi = launder(txs) + (size_t)i * n;
 #pragma unroll

@@ -46,6 +46,31 @@ __host__ __device__ constexpr int alt_tpt_cols(int N) {
 // would lose whole workgroups per CU (N = 640 already uses 80 KB per workgroup) and keep 0.
 __host__ __device__ constexpr int cols_group_pad(int N, int TPT) { return (N == 320 && TPT == 16) ? 4 : 0; }
 
+// Length of the twiddle-table prefix the passes read: the pruned forward (from span 4) and the full
+// transforms (the adjoint's forward ones read the same entries). The kernels copy it into LDS once per
+// workgroup (every stage's twiddles are then LDS reads instead of global loads with 64-bit addresses,
+// queued behind the data loads), except for the plans where the copy would lower the kernel's occupancy
+// (waves per SIMD from the compiler's VGPR and LDS counts): the columns passes at N = 640 (16 and 32
+// lanes) and the 32-lane one at N = 1280, the columns pass at N = 768 (3 x 54.4 KB of LDS no longer fit a
+// CU once the allocation is rounded up: measured 19 % slower for whole frames), and the forward-only rows
+// passes (focal plane) at N = 320, 512, 640 and 1536. Elsewhere the copy keeps or raises the occupancy (N
+// = 256 columns 4 -> 5 waves per SIMD, N = 640 rows 2 -> 3, N = 1536 rows 1 -> 2) and lowers the spills at
+// N = 1920 / 2048. Frames of 8 screens, slopes-only bands: N = 256 72 -> 56 us, 320 117 -> 86 us, 512 244
+// -> 188 us, 1536 3.51 -> 2.58 ms, 2048 8.0 -> 7.1 ms (config 1, 500 screens: 4.89 -> 3.53 ms slopes-only,
+// 8.72 -> 5.97 ms whole); N = 1024 and 1280 (64 lanes) had the copy already.
+template <int N, int TPT>
+__host__ __device__ constexpr int sep_tw_len() {
+  return (plan_tw_max(N, N / TPT, 4, N / 4) > plan_tw_max(N, N / TPT, 1, N) ? plan_tw_max(N, N / TPT, 4, N / 4)
+                                                                               : plan_tw_max(N, N / TPT, 1, N)) +
+         1;
+}
+__host__ __device__ constexpr bool tw_lds_cols(int N, int TPT) {
+  return !(N == 640 || N == 768 || (N == 1280 && TPT == 32));
+}
+__host__ __device__ constexpr bool tw_lds_rows(int N, bool fwd_only) {
+  return !(fwd_only && (N == 320 || N == 512 || N == 640 || N == 1536));
+}
+
 #define OOPAO_FAST_SIZES(X) X(64) X(128) X(256) X(320) X(384) X(512) X(640) X(768) X(1024) X(1280) X(1536) X(1920) X(2048)
 
 constexpr int kWaves = 4;  // wavefronts per workgroup
