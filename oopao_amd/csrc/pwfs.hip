@@ -294,7 +294,18 @@ k_cols_focal(const float2* __restrict__ ws, const float2* __restrict__ tw, float
   constexpr int E = N / TPT;
   constexpr int G = 64 / TPT;
   constexpr int NG = N / 4;
+  // (the pruned forward reads a part of the same prefix); the copy lowers this kernel's occupancy at
+  // N = 320, 512, 1280 and 1536
+  constexpr bool kTwLds = tw_lds_cols(N, TPT) && N != 320 && N != 512 && N != 1280 && N != 1536;
+  constexpr int kTw = kTwLds ? sep_tw_len<N, TPT>() : 1;
+  static_assert(!kTwLds || kTw <= N, "twiddle prefix within the table");
   __shared__ float2 lds[kWaves * G][lds_len(N)];
+  __shared__ float2 twl[kTw];
+  if constexpr (kTwLds) {
+    for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
+    __syncthreads();
+  }
+  const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int h = lane / TPT;
@@ -319,7 +330,7 @@ k_cols_focal(const float2* __restrict__ ws, const float2* __restrict__ tw, float
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + (X & 3)];
-    fft_fwd_pruned<N, TPT, 4>(v, xl, tw, t);
+    fft_fwd_pruned<N, TPT, 4>(v, xl, twp, t);
 #pragma unroll
     for (int m = 0; m < E; ++m) acc[m] += cabs2(v[m]);
   }
@@ -346,7 +357,18 @@ k_cols_raw(const float2* __restrict__ ws, const float2* __restrict__ my, const f
   constexpr int E = N / TPT;
   constexpr int G = 64 / TPT;
   constexpr int NG = N / 4;
+  // the columns pass's transforms and prefix; the copy lowers this kernel's occupancy at N = 320, 1280
+  // and 2048
+  constexpr bool kTwLds = tw_lds_cols(N, TPT) && N != 320 && N != 1280 && N != 2048;
+  constexpr int kTw = kTwLds ? sep_tw_len<N, TPT>() : 1;
+  static_assert(!kTwLds || kTw <= N, "twiddle prefix within the table");
   __shared__ float2 lds[kWaves * G][lds_len(N)];
+  __shared__ float2 twl[kTw];
+  if constexpr (kTwLds) {
+    for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
+    __syncthreads();
+  }
+  const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int h = lane / TPT;
@@ -369,10 +391,10 @@ k_cols_raw(const float2* __restrict__ ws, const float2* __restrict__ my, const f
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + (X & 3)];
-    fft_fwd_pruned<N, TPT, 4, true>(v, xl, tw, t);
+    fft_fwd_pruned<N, TPT, 4, true>(v, xl, twp, t);
 #pragma unroll
     for (int m = 0; m < E; ++m) v[m] = cmul(v[m], launder(my)[t + TPT * m]);
-    fft_full<N, TPT, +1, true>(v, xl, tw, t);
+    fft_full<N, TPT, +1, true>(v, xl, twp, t);
 #pragma unroll
     for (int m = 0; m < E; ++m) acc[m] += cabs2(v[m]);
   }
