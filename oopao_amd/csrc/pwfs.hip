@@ -77,6 +77,7 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
   constexpr int E = N / TPT;
   constexpr int G = 64 / TPT;
   constexpr int RPB = kWaves * G;  // rows per block
+  constexpr bool kStage = (n % TPT) == 0;   // the input rows staged through LDS (below)
   __shared__ float2 lds[kWaves * G][lds_len(N)];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -111,7 +112,26 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
     // a time, each followed by a wait for it (and, the counter being shared, for every store in
     // flight), 40 dependent round trips per item.
     float2 v[E];
-    {
+    if constexpr (kStage) {
+      // The row's n field values and the table's n tip values are loaded contiguously (n / TPT each per
+      // lane: every load instruction reads distinct elements), multiplied there, and spread to the
+      // replicated layout v[m] = x[(t + TPT m) / 4] of the pruned transform through the lane group's
+      // exchange buffer, which is free until the transform's first exchange. In the replicated layout
+      // each group of four lanes loaded the same element: four times the load instructions and products.
+      constexpr int Q = E / 4;
+      float2 a[Q], c[Q];
+#pragma unroll
+      for (int j = 0; j < Q; ++j) a[j] = fp[t + TPT * j];
+#pragma unroll
+      for (int j = 0; j < Q; ++j) c[j] = txi[t + TPT * j];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < Q; ++j) xl[t + TPT * j] = cmul(a[j], c[j]);
+      wave_lds_sync();
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = xl[(t + TPT * m) >> 2];
+      wave_lds_sync();
+    } else {
       float2 a[E], c[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) a[m] = fp[(t + TPT * m) >> 2];
