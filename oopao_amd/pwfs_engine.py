@@ -311,9 +311,9 @@ class PWFSEngine:
         workgroups; the cap sets how many rounds): with 64-lane transforms each 128-byte line of the
         intermediate is completed by the four wavefronts of a workgroup, and workgroups that take several
         items let those drift apart and split the lines' writes, so the whole frame (twice the stores of
-        a slopes-only one) takes one item per workgroup (config 2, 8 screens: 0.68 ms against 0.87 ms for
-        2, one screen 73-77 us against 77-78 us) and slopes-only frames 4 for one screen (one round of
-        768 workgroups, 60-62 us) and 2 for more (8 screens 417 us against 430 us for 4); with narrower
+        a slopes-only one) takes one item per workgroup (config 2, 8 screens: 0.59 ms against 0.76 ms for
+        2, one screen 68 us either way, 75-76 us for 4 or more) and slopes-only frames 4 (one screen: one
+        round of 768 workgroups, 58 us against 60 us for 2; 8 screens 367 us against 375 us); with narrower
         transforms a wavefront writes whole lines itself, and 2 (whole) or 4 tables amortise the
         workgroup's start (config 1, 500 screens, whole: 1.35 ms against 1.45 ms for 1)."""
         tpt_r = self.lanes_rows or self._default_lanes[0]
@@ -331,7 +331,7 @@ class PWFSEngine:
         if self.pts_per_task_user:
             ppt = self.pts_per_task_user
         elif tpt_r == 64:
-            ppt = 1 if whole else (4 if B == 1 else 2)
+            ppt = 1 if whole else 4
         else:
             ppt = 2 if whole else 4
         return max(1, min(nchunk, self.T)), max(1, min(self.P, ppt))

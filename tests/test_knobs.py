@@ -49,7 +49,8 @@ def test_rows_tables_per_workgroup():
     e64 = _engine(320, 72, 37, 64, 64)
     assert e64._knobs(1, whole=True)[1] == 1       # whole frames: one item per workgroup
     assert e64._knobs(1, whole=False)[1] == 4      # slopes-only, one screen
-    assert e64._knobs(8, whole=False)[1] == 2      # slopes-only, several screens
+    assert e64._knobs(8, whole=False)[1] == 4      # slopes-only, several screens
+    assert e64._knobs(8, whole=True)[1] == 1
     e16 = _engine(80, 48, 25, 16, 16)
     assert e16._knobs(1, whole=True)[1] == 2
     assert e16._knobs(64, whole=False)[1] == 4
