@@ -9,9 +9,9 @@ namespace oopao {
 // Lane count per transform for each supported padded length N (fast path: the
 // pupil occupies N/4 samples and the detector bins 4 x 4). Chosen so every
 // radix divides N/TPT and the pruned forward + inverse use few stages. N = 1920
-// = 2^7 x 15 takes 32 lanes (60 values per lane: radices 4, 4, 4, 5, 3, 2): its
-// kernels run at one wave per SIMD (the rows pass spills), still 4.5x faster
-// than the general path at that size.
+// = 2^7 x 15 takes 32 lanes (60 values per lane: radices 4, 4, 4, 2, 15): its
+// kernels run at one wave per SIMD, still several times faster than the general
+// path at that size.
 __host__ __device__ constexpr int fast_tpt(int N) {
   return N == 64 ? 16 : N == 128 ? 16 : N == 256 ? 32 : N == 320 ? 16 : N == 384 ? 16 : N == 512 ? 32
        : N == 640 ? 16 : N == 768 ? 32 : N == 1024 ? 64 : N == 1280 ? 64 : N == 1536 ? 64 : N == 1920 ? 32 : N == 2048 ? 64 : 0;

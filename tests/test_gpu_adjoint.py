@@ -49,6 +49,23 @@ def test_frame_vjp_matches_oracle(builder):
         assert err <= 1e-4
 
 
+@pytest.mark.parametrize('n', [320, 480])
+def test_frame_vjp_prime_factor_plans(n):
+    """The adjoint at the lengths whose plans end on a prime-factor stage: N = 1280 (radix 20, 4 x 5)
+    and N = 1920 (radix 15, 3 x 5), few points (the plan depends on the length only)."""
+    import torch
+    o = small_system(n=n, nSubap=n // 4, modulation=3, T=8)
+    assert o.geom.N == 4 * n
+    eng, amp = _engine_for(o)
+    ph = random_phase(o.geom.n, 21, 1.0, o.pupil)[None]
+    G = np.random.RandomState(8).randn(1, o.geom.cam, o.geom.cam)
+    t = torch.from_numpy(ph.astype(np.float32)).cuda().contiguous()
+    grad = eng.adjoint(t, 1.0, amp, torch.from_numpy(G.astype(np.float32)).cuda()).cpu().numpy()
+    err = rel_l2(grad[0], oadj.frame_vjp(o, ph[0], G[0]))
+    print('N', o.geom.N, 'frame VJP rel_l2', err)
+    assert err <= 1e-4
+
+
 def test_slopes_loss_gradient_config1():
     import torch
     from oopao_amd.differentiable import pwfs_frames, slopes_maps
