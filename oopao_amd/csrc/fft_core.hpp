@@ -175,6 +175,8 @@ template <int DIR>
 struct Dft<20, DIR> : Pfa<4, 5, DIR> {};
 template <int DIR>
 struct Dft<15, DIR> : Pfa<3, 5, DIR> {};
+template <int DIR>
+struct Dft<24, DIR> : Pfa<8, 3, DIR> {};
 
 // ---------------------------------------------------------------------------
 // Compile-time radix plans. Every radix must divide E so that each lane owns
@@ -184,10 +186,13 @@ struct Dft<15, DIR> : Pfa<3, 5, DIR> {};
 // stage held in one lane: as the last stage it needs no exchange, and it has no
 // internal twiddles (N = 1280 with 20 values per lane: 4, 4, 4, 20 instead of
 // 4, 4, 4, 4, 5; N = 640 with 20 / 40: 4, 4, 2, 20 / 8, 4, 20 instead of
-// 4, 4, 4, 5, 2 / 8, 8, 5, 2). Likewise a radix-15 (3 x 5) last stage when the
-// lane holds a multiple of 15 values and the rest is a power of two: N = 1920
-// with 60 values per lane runs 4, 4, 4, 2, 15 instead of 4, 4, 4, 5, 3, 2 (one
-// exchange fewer; frames 3-4 % faster, the columns pass's spills gone). A radix-60
+// 4, 4, 4, 5, 2 / 8, 8, 5, 2). Likewise radix-24 (8 x 3) and radix-15 (3 x 5)
+// last stages: N = 384 / 768 / 1536 with 24 values per lane run 8, 2, 24 /
+// 8, 4, 24 / 8, 8, 24 instead of 8, 8, 3, 2 / 8, 8, 4, 3 / 8, 8, 8, 3 (frames of
+// 8 screens 17 % / 0-4 % / 12 % faster; the focal-plane kernels at 768 and 1536
+// 4-11 % slower), and N = 1920 with 60 values per lane runs 4, 4, 4, 2, 15
+// instead of 4, 4, 4, 5, 3, 2 (frames 3-4 % faster, the columns pass's spills
+// gone). A radix-60
 // last stage for N = 1920 was measured: 60 values and their twiddles in one lane
 // spilled to scratch.
 // ---------------------------------------------------------------------------
@@ -202,6 +207,7 @@ __host__ __device__ constexpr bool pow2_plan(int x, int E) {
 // the prime-factor radix the plan of rem can end with (0: none)
 __host__ __device__ constexpr int pfa_last(int rem, int E) {
   return (E % 20 == 0 && rem % 20 == 0 && pow2_plan(rem / 20, E))   ? 20
+         : (E % 24 == 0 && rem % 24 == 0 && pow2_plan(rem / 24, E)) ? 24
          : (E % 15 == 0 && rem % 15 == 0 && pow2_plan(rem / 15, E)) ? 15
                                                                      : 0;
 }
