@@ -68,6 +68,10 @@ def n192_system():
     return _square_system(192, 48, 8)        # N = 768 (2^8 x 3), 32-lane transforms
 
 
+def n384_system():
+    return _square_system(384, 96, 4)        # N = 1536 (2^9 x 3), 64-lane transforms, 4 points
+
+
 def n480_system():
     # 120 x 120 subapertures at 4 px (N = 1920 = 2^7 x 15, 32-lane transforms), 4 points
     from oracle import pwfs as opw
@@ -78,7 +82,7 @@ def n480_system():
 
 
 @pytest.mark.parametrize('builder', [small_system, config1_system, config2_system, n96_system, n160_system,
-                                     n192_system, n512_system, n480_system])
+                                     n192_system, n384_system, n512_system, n480_system])
 def test_frames_match_oracle(builder):
     o = builder()
     eng, amp = _engine_for(o)
@@ -123,6 +127,31 @@ def test_slopes_match_oracle(builder):
     err32 = rel_l2(s.cpu().numpy()[0], o32.measure(ph)[1])
     print('slopes rel_l2 vs fp32 oracle', err32)
     assert err32 <= 1e-5
+
+
+def n320_t8_system():
+    return small_system(n=320, nSubap=80, modulation=3, T=8)     # N = 1280, 8 points
+
+
+@pytest.mark.parametrize('builder', [n96_system, n192_system, n320_t8_system, n384_system, n480_system])
+def test_raw_and_focal_at_prime_factor_plans(builder):
+    """The separable unbinned (raw_data) and focal-plane kernels at the lengths whose plans end on a
+    prime-factor stage (N = 384 / 768 / 1536: radix 24, 1280: radix 20, 1920: radix 15) against the float64
+    oracle, rel-L2 <= 1e-5."""
+    import torch
+    o = builder()
+    eng, amp = _engine_for(o)
+    ph = random_phase(o.geom.n, 9, 1.0, o.pupil)[None]
+    t = torch.from_numpy(ph.astype(np.float32)).to(amp.device).contiguous()
+    raw = eng.raw(t, 1.0, amp).cpu().numpy()
+    err = rel_l2(raw[0], o.raw(ph[0]))
+    print('N', o.geom.N, 'raw rel_l2', err)
+    assert err <= 1e-5
+    crop = o.geom.N // 4
+    foc = eng.focal(t, 1.0, amp, crop).cpu().numpy()
+    err = rel_l2(foc[0], o.focal(ph[0], crop))
+    print('N', o.geom.N, 'focal rel_l2', err)
+    assert err <= 1e-5
 
 
 @pytest.mark.parametrize('builder,crop', [(small_system, 20), (small_system, 64), (config1_system, 60),
