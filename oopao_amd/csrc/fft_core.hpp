@@ -300,6 +300,12 @@ __device__ __forceinline__ BufferRsrc buffer_rsrc(const void* base, unsigned byt
 __device__ __forceinline__ void buffer_store_c64(float2 v, BufferRsrc r, unsigned lane_off, int uniform_off) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)lane_off, uniform_off, 0);
 }
+__device__ __forceinline__ float2 buffer_load_c64(BufferRsrc r, unsigned lane_off, int uniform_off) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)lane_off, uniform_off, 0));
+}
+__device__ __forceinline__ float buffer_load_f32(BufferRsrc r, unsigned lane_off, int uniform_off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)lane_off, uniform_off, 0));
+}
 
 // One Stockham stage (radix R, already-sorted span NS) on the strided layout.
 // When NS divides TPT the twiddle index k = j mod NS equals t mod NS for all of

@@ -250,19 +250,32 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   float acc[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) acc[m] = 0.f;
+  // With kBuf the column values and mask values come through buffer loads: the lane part of each offset
+  // in one VGPR, the per-element part (a compile-time multiple of m) as a uniform offset, instead of a
+  // 64-bit address computed per element. Element ((t + TPT m) / 4) * 4 + c of a tile is
+  // ((t / 4) * 4 + c) + m TPT (TPT is a multiple of 4), below n * 4 = N / 4 * 4.
+  constexpr bool kBuf = buf_cols_sep(N);
+  const BufferRsrc msk = buffer_rsrc(my, (unsigned)N * sizeof(float2));
+  const unsigned msk_lane = (unsigned)t * sizeof(float2);
   for (int i = i0; i < i1; ++i) {
     const float2* tile = ws + (((size_t)b * T + i) * NG + g) * (size_t)n * 4;
+    const BufferRsrc col = buffer_rsrc(tile, (unsigned)n * 4 * sizeof(float2));
 #pragma unroll 1
     for (int c0 = 0; c0 < (kWaveCols ? 1 : 4); c0 += G) {
       const int c = kWaveCols ? wave : c0 + h;
+      const unsigned col_lane = (unsigned)(((t >> 2) * 4 + c) * (int)sizeof(float2));
       float2 v[E];
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + c];
+      for (int m = 0; m < E; ++m)
+        v[m] = kBuf ? buffer_load_c64(col, col_lane, m * TPT * (int)sizeof(float2))
+                    : tile[((t + TPT * m) >> 2) * 4 + c];
       // XOR-swizzled LDS exchange (conflict-free small-span stages), as in the
       // rows pass.
       fft_fwd_pruned<N, TPT, 4, true>(v, xl, twp, t);
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = cmul(v[m], launder(my)[t + TPT * m]);
+      for (int m = 0; m < E; ++m)
+        v[m] = cmul(v[m], kBuf ? buffer_load_c64(msk, msk_lane, m * TPT * (int)sizeof(float2))
+                               : launder(my)[t + TPT * m]);
       fft_full<N, TPT, +1, true>(v, xl, twp, t);
 #pragma unroll
       for (int m = 0; m < E; ++m) acc[m] += cabs2(v[m]);
@@ -345,11 +358,17 @@ k_cols_focal(const float2* __restrict__ ws, const float2* __restrict__ tw, float
   float acc[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) acc[m] = 0.f;
+  // buffer loads of the column values as in k_cols_sep
+  constexpr bool kBuf = buf_cols_focal(N);
+  const unsigned col_lane = (unsigned)(((t >> 2) * 4 + (X & 3)) * (int)sizeof(float2));
   for (int i = i0; i < i1; ++i) {
     const float2* tile = ws + (((size_t)b * T + i) * NG + (X >> 2)) * (size_t)n * 4;
+    const BufferRsrc col = buffer_rsrc(tile, (unsigned)n * 4 * sizeof(float2));
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + (X & 3)];
+    for (int m = 0; m < E; ++m)
+      v[m] = kBuf ? buffer_load_c64(col, col_lane, m * TPT * (int)sizeof(float2))
+                  : tile[((t + TPT * m) >> 2) * 4 + (X & 3)];
     fft_fwd_pruned<N, TPT, 4>(v, xl, twp, t);
 #pragma unroll
     for (int m = 0; m < E; ++m) acc[m] += cabs2(v[m]);
@@ -406,14 +425,24 @@ k_cols_raw(const float2* __restrict__ ws, const float2* __restrict__ my, const f
   float acc[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) acc[m] = 0.f;
+  // buffer loads of the column and mask values as in k_cols_sep
+  constexpr bool kBuf = buf_cols_raw(N);
+  const BufferRsrc msk = buffer_rsrc(my, (unsigned)N * sizeof(float2));
+  const unsigned msk_lane = (unsigned)t * sizeof(float2);
+  const unsigned col_lane = (unsigned)(((t >> 2) * 4 + (X & 3)) * (int)sizeof(float2));
   for (int i = i0; i < i1; ++i) {
     const float2* tile = ws + (((size_t)b * T + i) * NG + (X >> 2)) * (size_t)n * 4;
+    const BufferRsrc col = buffer_rsrc(tile, (unsigned)n * 4 * sizeof(float2));
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + (X & 3)];
+    for (int m = 0; m < E; ++m)
+      v[m] = kBuf ? buffer_load_c64(col, col_lane, m * TPT * (int)sizeof(float2))
+                  : tile[((t + TPT * m) >> 2) * 4 + (X & 3)];
     fft_fwd_pruned<N, TPT, 4, true>(v, xl, twp, t);
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cmul(v[m], launder(my)[t + TPT * m]);
+    for (int m = 0; m < E; ++m)
+      v[m] = cmul(v[m], kBuf ? buffer_load_c64(msk, msk_lane, m * TPT * (int)sizeof(float2))
+                             : launder(my)[t + TPT * m]);
     fft_full<N, TPT, +1, true>(v, xl, twp, t);
 #pragma unroll
     for (int m = 0; m < E; ++m) acc[m] += cabs2(v[m]);

@@ -77,26 +77,50 @@ k_bwd_cols(const float2* __restrict__ inter, const float2* __restrict__ my, cons
   const float2* tile = inter + ((size_t)bi * NG + g) * (size_t)n * 4;
   float2* zt = Z + ((size_t)bi * NG + g) * (size_t)n * 4;
   const float* gf = gframe + (size_t)b * NG * NG;
+  // With kBuf every per-element access goes through a buffer instruction: a lane offset in one VGPR and
+  // a compile-time uniform offset per element (TPT is a multiple of 4, so row (t + TPT m) / 4 is
+  // t / 4 + m TPT / 4), instead of a 64-bit address per element.
+  constexpr bool kBuf = buf_bwd_cols(N);
+  const BufferRsrc col = buffer_rsrc(tile, (unsigned)n * 4 * sizeof(float2));
+  const BufferRsrc zr = buffer_rsrc(zt, (unsigned)n * 4 * sizeof(float2));
+  const BufferRsrc msk = buffer_rsrc(my, (unsigned)N * sizeof(float2));
+  const BufferRsrc gr = buffer_rsrc(gf, (unsigned)(NG * NG) * sizeof(float));
+  const unsigned msk_lane = (unsigned)t * sizeof(float2);
+  const unsigned g_lane = (unsigned)(((t >> 2) * NG + g) * (int)sizeof(float));
 #pragma unroll 1
   for (int c0 = 0; c0 < (kWaveCols ? 1 : 4); c0 += G) {
     const int c = kWaveCols ? wave : c0 + h;
+    const unsigned col_lane = (unsigned)(((t >> 2) * 4 + c) * (int)sizeof(float2));
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = tile[((t + TPT * m) >> 2) * 4 + c];
+    for (int m = 0; m < E; ++m)
+      v[m] = kBuf ? buffer_load_c64(col, col_lane, m * TPT * (int)sizeof(float2))
+                  : tile[((t + TPT * m) >> 2) * 4 + c];
     fft_fwd_pruned<N, TPT, 4, true>(v, xl, twp, t);
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cmul(v[m], launder(my)[t + TPT * m]);
+    for (int m = 0; m < E; ++m)
+      v[m] = cmul(v[m], kBuf ? buffer_load_c64(msk, msk_lane, m * TPT * (int)sizeof(float2))
+                             : launder(my)[t + TPT * m]);
     fft_full<N, TPT, +1, true>(v, xl, twp, t);
     // E column at rows Y = t + TPT m; dL/dI there is the detector gradient of pixel (Y/4, X/4)
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cscale(v[m], gf[((t + TPT * m) >> 2) * NG + g]);
+    for (int m = 0; m < E; ++m)
+      v[m] = cscale(v[m], kBuf ? buffer_load_f32(gr, g_lane, m * (TPT / 4) * NG * (int)sizeof(float))
+                               : gf[((t + TPT * m) >> 2) * NG + g]);
     fft_full<N, TPT, -1, true>(v, xl, twp, t);
 #pragma unroll
-    for (int m = 0; m < E; ++m) v[m] = cmulc(v[m], launder(my)[t + TPT * m]);
+    for (int m = 0; m < E; ++m)
+      v[m] = cmulc(v[m], kBuf ? buffer_load_c64(msk, msk_lane, m * TPT * (int)sizeof(float2))
+                              : launder(my)[t + TPT * m]);
     fft_full<N, TPT, +1, true>(v, xl, twp, t);
     // keep the pupil rows y < n (elements m < E/4)
 #pragma unroll
-    for (int m = 0; m < E / 4; ++m) zt[(t + TPT * m) * 4 + c] = v[m];
+    for (int m = 0; m < E / 4; ++m) {
+      if constexpr (kBuf)
+        buffer_store_c64(v[m], zr, (unsigned)((t * 4 + c) * (int)sizeof(float2)), m * TPT * 4 * (int)sizeof(float2));
+      else
+        zt[(t + TPT * m) * 4 + c] = v[m];
+    }
   }
 }
 

@@ -71,6 +71,22 @@ __host__ __device__ constexpr bool tw_lds_rows(int N, bool fwd_only) {
   return !(fwd_only && (N == 320 || N == 512 || N == 640 || N == 1536));
 }
 
+// Buffer instructions for the per-element accesses of the column kernels (column values, mask values,
+// and in the adjoint the detector gradient and the Z stores): one VGPR of lane offset and compile-time
+// uniform offsets instead of a 64-bit address per element (k_cols_sep at N = 1280: 1697 -> 1643 VALU
+// instructions per (point, column) pair; the adjoint's columns kernel at N = 1920: 512 VGPRs and 47
+// spilled -> 238, none spilled). The compiler schedules the two forms differently, and the gain follows
+// the length, so each kernel takes them where they measured faster (frames of 1 and 8 screens; gradients
+// of 1-8 screens): k_cols_sep N = 640 2-6 %, 1280 2-3 %, 1536 6-7 %, 1920 3-4 % (1024 unchanged; 320,
+// 384, 768 and 2048 slower by 2-10 %); k_cols_raw N = 640 10 %, 1920 4 %; k_cols_focal N = 512 2 %,
+// 768 12 %, 1920 2 %; k_bwd_cols N = 512 2 %, 768 1 %, 1280 1 %, 1920 8 %.
+__host__ __device__ constexpr bool buf_cols_sep(int N) {
+  return N == 640 || N == 1024 || N == 1280 || N == 1536 || N == 1920;
+}
+__host__ __device__ constexpr bool buf_cols_raw(int N) { return N == 640 || N == 1920; }
+__host__ __device__ constexpr bool buf_cols_focal(int N) { return N == 512 || N == 768 || N == 1920; }
+__host__ __device__ constexpr bool buf_bwd_cols(int N) { return N == 512 || N == 768 || N == 1280 || N == 1920; }
+
 #define OOPAO_FAST_SIZES(X) X(64) X(128) X(256) X(320) X(384) X(512) X(640) X(768) X(1024) X(1280) X(1536) X(1920) X(2048)
 
 constexpr int kWaves = 4;  // wavefronts per workgroup
