@@ -70,14 +70,17 @@ __global__ void k_field(const float* __restrict__ opd, float scale, const float*
 // ---------------------------------------------------------------------------
 template <int N, int TPT, bool FWD_ONLY = false, bool BANDS = false>
 __global__ void __launch_bounds__(64 * kWaves)
-k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, const int* __restrict__ pstart,
-           const float2* __restrict__ ty, const float2* __restrict__ mx, const float2* __restrict__ tw,
-           float2* __restrict__ ws, int B, int P, int T, long items, long nwg, Bands bd) {
+k_rows_sep(const float* __restrict__ opd, float scale, const float* __restrict__ amp, float2* __restrict__ field,
+           const float2* __restrict__ tx, const int* __restrict__ pstart, const float2* __restrict__ ty,
+           const float2* __restrict__ mx, const float2* __restrict__ tw, float2* __restrict__ ws, int B, int P,
+           int T, long items, long nwg, Bands bd) {
   constexpr int n = N / 4;
   constexpr int E = N / TPT;
   constexpr int G = 64 / TPT;
   constexpr int RPB = kWaves * G;  // rows per block
   constexpr bool kStage = (n % TPT) == 0;   // the input rows staged through LDS (below)
+  constexpr bool kFuse = rows_form_field(N, TPT, FWD_ONLY);   // the field formed here (below)
+  static_assert(!kFuse || kStage, "the field is formed on the staged path only");
   __shared__ float2 lds[kWaves * G][lds_len(N)];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -104,8 +107,6 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
     const int y_raw = rb * RPB + wave * G + h;
     const bool active = y_raw < n;
     const int y = active ? y_raw : n - 1;
-    // the row's field is re-read per table (an L2 hit after the first) rather than held in registers
-    const float2* fp = launder(field) + ((size_t)b * n + y) * n;
     const float2* txi = launder(tx) + (size_t)i * n;
     // All of an item's global loads are issued before the first of them is used (the scheduling
     // barriers keep the products below the loads): left to itself the compiler issued one load pair at
@@ -113,18 +114,47 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
     // flight), 40 dependent round trips per item.
     float2 v[E];
     if constexpr (kStage) {
-      // The row's n field values and the table's n tip values are loaded contiguously (n / TPT each per
-      // lane: every load instruction reads distinct elements), multiplied there, and spread to the
-      // replicated layout v[m] = x[(t + TPT m) / 4] of the pruned transform through the lane group's
+      // The row's n field values and the table's n tip values are formed / loaded contiguously (n / TPT
+      // each per lane: every load instruction reads distinct elements), multiplied there, and spread to
+      // the replicated layout v[m] = x[(t + TPT m) / 4] of the pruned transform through the lane group's
       // exchange buffer, which is free until the transform's first exchange. In the replicated layout
-      // each group of four lanes loaded the same element: four times the load instructions and products.
+      // each group of four lanes had loaded the same element: four times the load instructions and
+      // products. With kFuse the field values amp exp(i scale opd) are formed here, per table, with
+      // k_field's expression (so the same values) instead of in a launch of their own, and the items of
+      // table 0 store them for the consumers of the workspace's field (the adjoint); otherwise the row's
+      // field is re-read per table (L2 hits after the first).
       constexpr int Q = E / 4;
       float2 a[Q], c[Q];
+      if constexpr (kFuse) {
+        const float* op = launder(opd) + ((size_t)b * n + y) * n;
+        const float* ap = launder(amp) + (size_t)y * n;
+        float o[Q], am[Q];
 #pragma unroll
-      for (int j = 0; j < Q; ++j) a[j] = fp[t + TPT * j];
+        for (int j = 0; j < Q; ++j) o[j] = op[t + TPT * j];
 #pragma unroll
-      for (int j = 0; j < Q; ++j) c[j] = txi[t + TPT * j];
-      __builtin_amdgcn_sched_barrier(0);
+        for (int j = 0; j < Q; ++j) am[j] = ap[t + TPT * j];
+#pragma unroll
+        for (int j = 0; j < Q; ++j) c[j] = txi[t + TPT * j];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+          float sn, cs;
+          sincosf(o[j] * scale, &sn, &cs);
+          a[j] = make_float2(am[j] * cs, am[j] * sn);
+        }
+        if (i == 0 && active) {
+          float2* fo = field + ((size_t)b * n + y) * n;
+#pragma unroll
+          for (int j = 0; j < Q; ++j) fo[t + TPT * j] = a[j];
+        }
+      } else {
+        const float2* fp = launder(field) + ((size_t)b * n + y) * n;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) a[j] = fp[t + TPT * j];
+#pragma unroll
+        for (int j = 0; j < Q; ++j) c[j] = txi[t + TPT * j];
+        __builtin_amdgcn_sched_barrier(0);
+      }
 #pragma unroll
       for (int j = 0; j < Q; ++j) xl[t + TPT * j] = cmul(a[j], c[j]);
       wave_lds_sync();
@@ -132,6 +162,8 @@ k_rows_sep(const float2* __restrict__ field, const float2* __restrict__ tx, cons
       for (int m = 0; m < E; ++m) v[m] = xl[(t + TPT * m) >> 2];
       wave_lds_sync();
     } else {
+      // the field from k_field's launch; the row's values are re-read per table (L2 hits after the first)
+      const float2* fp = launder(field) + ((size_t)b * n + y) * n;
       float2 a[E], c[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) a[m] = fp[(t + TPT * m) >> 2];
@@ -641,7 +673,8 @@ static int launch_rows_plan(const float* opd, float scale, const float* amp, con
   const long items = (long)B * nrb * P;
   const long per_round = (long)slots * pts_per_task;
   const long nwg = std::min(items, (long)slots * ((items + per_round - 1) / per_round));
-  {
+  if (!rows_form_field(N, TPT, FWD_ONLY)) {
+    // the plans that do not form the field themselves read it from this launch
     const size_t total = (size_t)B * n * n;
     hipLaunchKernelGGL(k_field, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, opd, scale, amp, field,
                        n, B);
@@ -649,11 +682,11 @@ static int launch_rows_plan(const float* opd, float scale, const float* amp, con
   }
   dim3 grid((unsigned)nwg);
   if (FWD_ONLY || all)
-    hipLaunchKernelGGL((k_rows_sep<N, TPT, FWD_ONLY, false>), grid, dim3(64 * kWaves), 0, st, field, tx, pstart, ty,
-                       mx, tw, inter, B, P, T, items, nwg, bd);
+    hipLaunchKernelGGL((k_rows_sep<N, TPT, FWD_ONLY, false>), grid, dim3(64 * kWaves), 0, st, opd, scale, amp,
+                       field, tx, pstart, ty, mx, tw, inter, B, P, T, items, nwg, bd);
   else
-    hipLaunchKernelGGL((k_rows_sep<N, TPT, false, true>), grid, dim3(64 * kWaves), 0, st, field, tx, pstart, ty, mx,
-                       tw, inter, B, P, T, items, nwg, bd);
+    hipLaunchKernelGGL((k_rows_sep<N, TPT, false, true>), grid, dim3(64 * kWaves), 0, st, opd, scale, amp, field,
+                       tx, pstart, ty, mx, tw, inter, B, P, T, items, nwg, bd);
   OOPAO_LAUNCH_CHECK();
   return OK;
 }

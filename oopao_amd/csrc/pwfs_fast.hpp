@@ -87,6 +87,17 @@ __host__ __device__ constexpr bool buf_cols_raw(int N) { return N == 640 || N ==
 __host__ __device__ constexpr bool buf_cols_focal(int N) { return N == 512 || N == 768 || N == 1920; }
 __host__ __device__ constexpr bool buf_bwd_cols(int N) { return N == 512 || N == 768 || N == 1280 || N == 1920; }
 
+// Rows-pass plans that form the pupil field amp exp(i scale opd) themselves, per tip table, instead of
+// reading it from a k_field launch: one launch (~4.6 us) fewer per call, for the sincos of every field
+// value once per tip table (37 tables at config 2). Staged plans only ((N / 4) % TPT == 0), and where it
+// measured faster: frames of 1 and 8 screens at N = 128-768 2-14 %, focal frames at N <= 768 2-9 %; config
+// 2 (N = 1280) single screen 4958 -> 4999 frames/s (rows pass 58.8 -> 57.2 us) but 8 screens per call
+// 0.8 % slower, focal frames unchanged; N = 1024-2048 unchanged to 1 % slower (the table count grows
+// with the modulation points while the launch saved stays the same).
+__host__ __device__ constexpr bool rows_form_field(int N, int TPT, bool fwd_only) {
+  return (N / 4) % TPT == 0 && (N <= 768 || (N == 1280 && !fwd_only));
+}
+
 #define OOPAO_FAST_SIZES(X) X(64) X(128) X(256) X(320) X(384) X(512) X(640) X(768) X(1024) X(1280) X(1536) X(1920) X(2048)
 
 constexpr int kWaves = 4;  // wavefronts per workgroup
