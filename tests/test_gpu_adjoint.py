@@ -49,11 +49,12 @@ def test_frame_vjp_matches_oracle(builder):
         assert err <= 1e-4
 
 
-@pytest.mark.parametrize('n', [96, 192, 320, 384, 480])
+@pytest.mark.parametrize('n', [96, 128, 192, 320, 384, 480])
 def test_frame_vjp_prime_factor_plans(n):
     """The adjoint at the lengths whose plans end on a prime-factor stage: N = 384, 768, 1536 (radix 24,
-    8 x 3), N = 1280 (radix 20, 4 x 5) and N = 1920 (radix 15, 3 x 5), few points (the plan depends on the
-    length only)."""
+    8 x 3), N = 1280 (radix 20, 4 x 5) and N = 1920 (radix 15, 3 x 5), and at N = 512, where the adjoint's
+    columns kernel loads and stores through buffer instructions (as at 768, 1280 and 1920); few points
+    (the plan depends on the length only)."""
     import torch
     o = small_system(n=n, nSubap=n // 4, modulation=3, T=8)
     assert o.geom.N == 4 * n

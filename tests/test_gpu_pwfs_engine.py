@@ -64,6 +64,10 @@ def n96_system():
     return _square_system(96, 24, 12)        # N = 384 (2^7 x 3), 16-lane transforms
 
 
+def n128_system():
+    return _square_system(128, 32, 8)        # N = 512, 32-lane transforms
+
+
 def n192_system():
     return _square_system(192, 48, 8)        # N = 768 (2^8 x 3), 32-lane transforms
 
@@ -133,11 +137,14 @@ def n320_t8_system():
     return small_system(n=320, nSubap=80, modulation=3, T=8)     # N = 1280, 8 points
 
 
-@pytest.mark.parametrize('builder', [n96_system, n192_system, n320_t8_system, n384_system, n480_system])
+@pytest.mark.parametrize('builder', [n96_system, n128_system, n160_system, n192_system, n320_t8_system,
+                                     n384_system, n480_system])
 def test_raw_and_focal_at_prime_factor_plans(builder):
-    """The separable unbinned (raw_data) and focal-plane kernels at the lengths whose plans end on a
-    prime-factor stage (N = 384 / 768 / 1536: radix 24, 1280: radix 20, 1920: radix 15) against the float64
-    oracle, rel-L2 <= 1e-5."""
+    """The separable unbinned (raw_data) and focal-plane kernels against the float64 oracle, rel-L2 <=
+    1e-5, at the lengths whose plans end on a prime-factor stage (N = 384 / 768 / 1536: radix 24, 1280:
+    radix 20, 1920: radix 15) and at N = 512 / 640, where the focal / raw columns kernels load through
+    buffer instructions and the rows pass forms the field (the kernels' compile-time variants follow the
+    length)."""
     import torch
     o = builder()
     eng, amp = _engine_for(o)
