@@ -22,6 +22,23 @@ __device__ __forceinline__ int xcd_task(int i, int G) {
   return x < tall ? x * per + slot : tall * per + (x - tall) * (per - 1) + slot;
 }
 
+// init + p[0] + p[stride] + ... + p[(count - 1) stride], added in index order (the same result as the
+// plain loop), with the loads issued eight at a time: in the plain loop each load waited for the one before
+// (the additions form a chain), count round trips to memory.
+template <typename T>
+__device__ __forceinline__ T sum_in_order(const T* p, size_t stride, int count, T init = T(0)) {
+  T s = init;
+  for (int k0 = 0; k0 < count; k0 += 8) {
+    T v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = (k0 + j < count) ? p[(size_t)(k0 + j) * stride] : T(0);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (k0 + j < count) s += v[j];
+  }
+  return s;
+}
+
 // Error codes returned by every C-ABI entry point (0 = success).
 enum : int {
   OK = 0,

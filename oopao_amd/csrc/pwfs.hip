@@ -497,7 +497,7 @@ __global__ void __launch_bounds__(256) k_raw_reduce(const float* __restrict__ pa
     const int X = X0 + r, Y = Y0 + tx;
     float s = 0.f;
     if (X < N && Y < N)
-      for (int k = 0; k < nchunk; ++k) s += part[(((size_t)b * nchunk + k) * N + X) * (size_t)N + Y];
+      s = sum_in_order(part + (((size_t)b * nchunk) * N + X) * (size_t)N + Y, (size_t)N * N, nchunk);
     tile[r][tx] = s;
   }
   __syncthreads();
@@ -537,9 +537,7 @@ __global__ void k_sum_chunks(const float* __restrict__ part, float* __restrict__
   if (idx >= (size_t)B * count) return;
   const int p = (int)(idx % count);
   const int b = (int)(idx / count);
-  float s = 0.f;
-  for (int k = 0; k < nchunk; ++k) s += part[((size_t)b * nchunk + k) * count + p];
-  out[idx] = s;
+  out[idx] = sum_in_order(part + ((size_t)b * nchunk) * count + p, (size_t)count, nchunk);
 }
 
 // frame[b][r][g] = sum_k part[b][k][g][r]  (fixed summation order), through a 32 x 32 LDS tile so that the
@@ -558,7 +556,7 @@ __global__ void __launch_bounds__(1024) k_reduce_chunks(const float* __restrict_
     const int g = g0 + ty, r = r0 + tx;
     float s = 0.f;
     if (g < cam && r < cam && in_bands(g, bd))
-      for (int k = 0; k < nchunk; ++k) s += part[(((size_t)b * nchunk + k) * cam + g) * cam + r];
+      s = sum_in_order(part + (((size_t)b * nchunk) * cam + g) * cam + r, (size_t)cam * cam, nchunk);
     tile[ty][tx] = s;
   }
   __syncthreads();
@@ -571,6 +569,20 @@ __global__ void __launch_bounds__(1024) k_reduce_chunks(const float* __restrict_
 // sums of the valid I4Q (double), then every slope normalised by their mean.
 // quad[0..3] = row origins, quad[4..7] = column origins of quadrants 1..4.
 // ---------------------------------------------------------------------------
+// The four quadrant pixels of subaperture p = (py, px), loaded together with its valid flag rather than
+// after it (the flag was a round trip to memory before the pixel loads could issue). The quadrants lie
+// inside the frame, as the reference cuts them (grabQuadrant); the indices are clamped to the frame all
+// the same, so that no subaperture reads outside it.
+struct Quad4 {
+  float I1, I2, I3, I4;
+};
+__device__ __forceinline__ Quad4 quad_pixels(const float* f, int cam, const int* quad, int w, int p) {
+  const int py = p / w, px = p % w;
+  const unsigned last = (unsigned)(cam * cam - 1);
+  auto at = [&](int r0, int c0) { return f[min((unsigned)((r0 + py) * cam + c0 + px), last)]; };
+  return {at(quad[0], quad[4]), at(quad[1], quad[5]), at(quad[2], quad[6]), at(quad[3], quad[7])};
+}
+
 __global__ void __launch_bounds__(256)
 k_slopes_partial(const float* __restrict__ frames, int cam, const unsigned char* __restrict__ valid, int w,
                  const int* __restrict__ quad, int nblk, double* __restrict__ part) {
@@ -578,14 +590,13 @@ k_slopes_partial(const float* __restrict__ frames, int cam, const unsigned char*
   const int b = blockIdx.y;
   const float* f = frames + (size_t)b * cam * cam;
   __shared__ double red[256];
-  const int r1 = quad[0], r2 = quad[1], r3 = quad[2], r4 = quad[3];
-  const int c1 = quad[4], c2 = quad[5], c3 = quad[6], c4 = quad[7];
   const int p = blockIdx.x * 256 + threadIdx.x;
   double acc = 0.0;
-  if (p < w * w && valid[p]) {
-    const int py = p / w, px = p % w;
-    acc = ((double)f[(r1 + py) * cam + c1 + px] + (double)f[(r2 + py) * cam + c2 + px]) +
-          ((double)f[(r3 + py) * cam + c3 + px] + (double)f[(r4 + py) * cam + c4 + px]);
+  if (p < w * w) {
+    const bool v = valid[p] != 0;
+    const Quad4 I = quad_pixels(f, cam, quad, w, p);
+    const double s4 = ((double)I.I1 + (double)I.I2) + ((double)I.I3 + (double)I.I4);
+    acc = v ? s4 : 0.0;
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -596,21 +607,15 @@ k_slopes_partial(const float* __restrict__ frames, int cam, const unsigned char*
   if (threadIdx.x == 0) part[(size_t)b * nblk + blockIdx.x] = red[0];
 }
 
-// Normalised slopes at flat index q of the [2w][w] map (q < w*w: Sx, else Sy), Pyramid.py:846-853.
-__device__ __forceinline__ float slope_at(const float* f, int cam, const unsigned char* valid, int w,
-                                          const int* quad, const float* ref2d, float units, float inv, int q) {
+// Unnormalised slope at flat index q of the [2w][w] map (q < w*w: Sx, else Sy), Pyramid.py:846-853.
+__device__ __forceinline__ float raw_slope(const float* f, int cam, const unsigned char* valid, int w,
+                                           const int* quad, int q) {
   const int ww = w * w;
   const int p = q < ww ? q : q - ww;
-  float s = 0.f;
-  if (valid[p]) {
-    const int py = p / w, px = p % w;
-    const float I1 = f[(quad[0] + py) * cam + quad[4] + px];
-    const float I2 = f[(quad[1] + py) * cam + quad[5] + px];
-    const float I3 = f[(quad[2] + py) * cam + quad[6] + px];
-    const float I4 = f[(quad[3] + py) * cam + quad[7] + px];
-    s = q < ww ? (I1 - I2) + (I4 - I3) : (I1 - I4) + (I2 - I3);
-  }
-  return (s * inv - ref2d[q]) * units;
+  const bool v = valid[p] != 0;
+  const Quad4 I = quad_pixels(f, cam, quad, w, p);
+  const float s = q < ww ? (I.I1 - I.I2) + (I.I4 - I.I3) : (I.I1 - I.I4) + (I.I2 - I.I3);
+  return v ? s : 0.f;
 }
 
 __global__ void __launch_bounds__(256)
@@ -619,23 +624,39 @@ k_slopes_finish(const float* __restrict__ frames, int cam, const unsigned char* 
                 const int* __restrict__ vidx, int nsig, int nvalid, int nblk, const double* __restrict__ part,
                 float* __restrict__ sig2d, float* __restrict__ sig) {
   const int b = blockIdx.y;
-  __shared__ float inv_s;
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int k = 0; k < nblk; ++k) t += part[(size_t)b * nblk + k];
-    inv_s = 1.0f / (float)(t / (double)nvalid);
-  }
-  __syncthreads();
-  const float inv = inv_s;
   const float* f = frames + (size_t)b * cam * cam;
   const int n2 = 2 * w * w;
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n2) {
-    sig2d[(size_t)b * n2 + i] = slope_at(f, cam, valid, w, quad, ref2d, units, inv, i);
-  } else if (i - n2 < nsig) {
-    const int j = i - n2;
-    sig[(size_t)b * nsig + j] = slope_at(f, cam, valid, w, quad, ref2d, units, inv, vidx[j]);
+  // this thread's unnormalised slope and reference come first: their loads are in flight while the
+  // block partials are gathered below
+  int q = -1;
+  if (i < n2)
+    q = i;
+  else if (i - n2 < nsig)
+    q = vidx[i - n2];
+  float s = 0.f, ref = 0.f;
+  if (q >= 0) {
+    s = raw_slope(f, cam, valid, w, quad, q);
+    ref = ref2d[q];
   }
+  // the normalisation: the block partials loaded by all threads at once into LDS and added there in index
+  // order by one thread (before, that thread loaded them one after another from memory)
+  __shared__ double pbuf[256];
+  __shared__ float inv_s;
+  double t = 0.0;
+  for (int k0 = 0; k0 < nblk; k0 += 256) {
+    if (k0 + (int)threadIdx.x < nblk) pbuf[threadIdx.x] = part[(size_t)b * nblk + k0 + threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) t = sum_in_order(static_cast<const double*>(pbuf), 1, min(256, nblk - k0), t);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) inv_s = 1.0f / (float)(t / (double)nvalid);
+  __syncthreads();
+  const float inv = inv_s;
+  if (i < n2)
+    sig2d[(size_t)b * n2 + i] = (s * inv - ref) * units;
+  else if (i - n2 < nsig)
+    sig[(size_t)b * nsig + (i - n2)] = (s * inv - ref) * units;
 }
 
 // ---------------------------------------------------------------------------

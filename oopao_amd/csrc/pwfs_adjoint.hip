@@ -202,9 +202,7 @@ __global__ void k_adj_sum(const float* __restrict__ gpart, float* __restrict__ g
   if (idx >= (size_t)B * count) return;
   const int p = (int)(idx % count);
   const size_t b = idx / count;
-  float s = 0.f;
-  for (int k = 0; k < nchunk; ++k) s += gpart[(b * nchunk + k) * count + p];
-  grad[idx] = s;
+  grad[idx] = sum_in_order(gpart + (b * nchunk) * count + p, (size_t)count, nchunk);
 }
 
 static size_t al256a(size_t x) { return (x + 255) & ~(size_t)255; }

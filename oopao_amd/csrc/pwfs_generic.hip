@@ -275,9 +275,7 @@ __global__ void k_gen_sum(const float* __restrict__ part, float* __restrict__ ou
   if (idx >= (size_t)B * count) return;
   const size_t q = idx % count;
   const size_t b = idx / count;
-  float s = 0.f;
-  for (int k = 0; k < nchunk; ++k) s += part[(b * nchunk + k) * count + q];
-  out[idx] = s;
+  out[idx] = sum_in_order(part + (b * nchunk) * count + q, count, nchunk);
 }
 
 // frames[b][R][C] = sum of the bin x bin block of raw (row-major order)
