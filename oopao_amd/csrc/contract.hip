@@ -489,8 +489,7 @@ __global__ void __launch_bounds__(256) k_gemv_finish(const float* __restrict__ p
                                                      float alpha, float beta) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
-  float s = 0.f;
-  for (int c = 0; c < nkc; ++c) s += part[(size_t)c * Rp + r];
+  const float s = sum_in_order(part + r, (size_t)Rp, nkc);
   float v = alpha * s;
   if (y0) v = fmaf(beta, y0[r], v);
   y[r] = v;

@@ -93,7 +93,7 @@ k_rows_sep(const float* __restrict__ opd, float scale, const float* __restrict__
   static_assert(!kTwLds || kTw <= N, "twiddle prefix within the table");
   __shared__ float2 twl[kTw];
   if constexpr (kTwLds) {
-    for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
+    copy_twiddles<kTw, 64 * kWaves>(twl, tw);
     __syncthreads();
   }
   const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;
@@ -270,7 +270,7 @@ k_cols_sep(const float2* __restrict__ ws, const float2* __restrict__ my,
   // the twiddle copy and its barrier come before the early exit, with every wavefront of the workgroup
   // present (with narrow transforms the last workgroup's wavefronts past the tasks leave below)
   if constexpr (kTwLds) {
-    for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
+    copy_twiddles<kTw, 64 * kWaves>(twl, tw);
     __syncthreads();
   }
   if (b >= B) return;  // uniform over the wavefront (and over the workgroup when kWaveCols)
@@ -367,7 +367,7 @@ k_cols_focal(const float2* __restrict__ ws, const float2* __restrict__ tw, float
   __shared__ float2 lds[kWaves * G][lds_len(N)];
   __shared__ float2 twl[kTw];
   if constexpr (kTwLds) {
-    for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
+    copy_twiddles<kTw, 64 * kWaves>(twl, tw);
     __syncthreads();
   }
   const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;
@@ -436,7 +436,7 @@ k_cols_raw(const float2* __restrict__ ws, const float2* __restrict__ my, const f
   __shared__ float2 lds[kWaves * G][lds_len(N)];
   __shared__ float2 twl[kTw];
   if constexpr (kTwLds) {
-    for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
+    copy_twiddles<kTw, 64 * kWaves>(twl, tw);
     __syncthreads();
   }
   const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;

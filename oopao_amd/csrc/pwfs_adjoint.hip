@@ -67,7 +67,7 @@ k_bwd_cols(const float2* __restrict__ inter, const float2* __restrict__ my, cons
   // the twiddle prefix into LDS, with every wavefront of the workgroup present (the last workgroup's
   // wavefronts past the tasks leave below)
   if constexpr (kTwLds) {
-    for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
+    copy_twiddles<kTw, 64 * kWaves>(twl, tw);
     __syncthreads();
   }
   const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;
@@ -152,7 +152,7 @@ k_bwd_rows(const float2* __restrict__ Z, const float2* __restrict__ mx, const fl
   const int b = bid / nchunk;
   if (b >= B) return;  // uniform over the workgroup
   if constexpr (kTwLds) {
-    for (int e = threadIdx.x; e < kTw; e += 64 * kWaves) twl[e] = tw[e];
+    copy_twiddles<kTw, 64 * kWaves>(twl, tw);
     __syncthreads();
   }
   const float2* twp = kTwLds ? static_cast<const float2*>(twl) : tw;

@@ -58,6 +58,25 @@ __host__ __device__ constexpr int cols_group_pad(int N, int TPT) { return (N == 
 // N = 1920 / 2048. Frames of 8 screens, slopes-only bands: N = 256 72 -> 56 us, 320 117 -> 86 us, 512 244
 // -> 188 us, 1536 3.51 -> 2.58 ms, 2048 8.0 -> 7.1 ms (config 1, 500 screens: 4.89 -> 3.53 ms slopes-only,
 // 8.72 -> 5.97 ms whole); N = 1024 and 1280 (64 lanes) had the copy already.
+// The first K twiddle-table entries into LDS, by the NT threads of a workgroup, every load issued before
+// the first LDS store: the plain strided loop waited for each load before storing it, one round trip to
+// memory per iteration at the start of every workgroup.
+template <int K, int NT>
+__device__ __forceinline__ void copy_twiddles(float2* dst, const float2* __restrict__ src) {
+  constexpr int kIt = (K + NT - 1) / NT;
+  float2 v[kIt];
+#pragma unroll
+  for (int j = 0; j < kIt; ++j) {
+    const int e = (int)threadIdx.x + j * NT;
+    v[j] = e < K ? src[e] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int j = 0; j < kIt; ++j) {
+    const int e = (int)threadIdx.x + j * NT;
+    if (e < K) dst[e] = v[j];
+  }
+}
+
 template <int N, int TPT>
 __host__ __device__ constexpr int sep_tw_len() {
   return (plan_tw_max(N, N / TPT, 4, N / 4) > plan_tw_max(N, N / TPT, 1, N) ? plan_tw_max(N, N / TPT, 4, N / 4)
