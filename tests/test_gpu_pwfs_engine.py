@@ -68,6 +68,14 @@ def n128_system():
     return _square_system(128, 32, 8)        # N = 512, 32-lane transforms
 
 
+def n32_system():
+    return _square_system(32, 8, 8)          # N = 128, 16-lane transforms
+
+
+def n64_system():
+    return _square_system(64, 16, 8)         # N = 256, 32-lane transforms
+
+
 def n192_system():
     return _square_system(192, 48, 8)        # N = 768 (2^8 x 3), 32-lane transforms
 
@@ -85,8 +93,9 @@ def n480_system():
                              nTheta_user_defined=4, n_jobs=4)
 
 
-@pytest.mark.parametrize('builder', [small_system, config1_system, config2_system, n96_system, n160_system,
-                                     n192_system, n384_system, n512_system, n480_system])
+@pytest.mark.parametrize('builder', [small_system, n32_system, n64_system, config1_system, config2_system,
+                                     n96_system, n128_system, n160_system, n192_system, n384_system, n512_system,
+                                     n480_system])
 def test_frames_match_oracle(builder):
     o = builder()
     eng, amp = _engine_for(o)
